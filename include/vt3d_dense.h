@@ -1,0 +1,281 @@
+/*
+ * vt3d_dense.h -- C ABI of the MI355X dense plane-sweep depth pass.
+ *
+ * This library replaces the single-reference dense sweep of video-to-3d-core:
+ *   crates/video-to-3d-core/src/dense_legacy.rs:149-156
+ *     fn estimate_depth_points_with_control(frames, cameras, sparse_points, focal,
+ *                                           fine_grid, control) -> DenseAnalysis
+ * and the per-reference remap that calls it:
+ *   crates/video-to-3d-core/src/dense.rs:674-747  fn reconstruct_reference_patch(...)
+ *
+ * Conventions shared by every entry point:
+ *  - Plain C types only: pointers, sizes, fixed-layout structs. No allocation is handed
+ *    across the boundary except vt3d_dense_result, which the library owns and frees.
+ *  - Return value 0 means the call ran. A run the reference would skip is still a 0 return
+ *    with result->stats.attempted == 0 and result->skip_code / skip_reason set to the exact
+ *    reference string (dense_legacy.rs:157-236, 274). Negative values are ABI misuse
+ *    (VT3D_E_INVALID) or a device error (VT3D_E_DEVICE); no CPU fallback exists.
+ *  - Calls are synchronous on the calling thread. Each call uses its own HIP stream on the
+ *    current device; concurrent calls from different threads are allowed.
+ */
+#ifndef VT3D_DENSE_H
+#define VT3D_DENSE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define VT3D_ABI_VERSION 1
+
+/* Return codes. */
+#define VT3D_OK 0
+#define VT3D_NO_PATCH 1 /* vt3d_dense_reference_patch: the reference returns None */
+#define VT3D_E_INVALID (-1)
+#define VT3D_E_DEVICE (-2)
+#define VT3D_E_NO_DEVICE (-3)
+#define VT3D_E_BUDGET (-4)
+#define VT3D_E_BUSY (-5)
+
+/* FrameInput (crates/video-to-3d-core/src/reconstruction/types.rs:16-21):
+ * RGBA8 row-major, 4*width bytes per row. */
+typedef struct vt3d_frame {
+    uint32_t width;
+    uint32_t height;
+    const uint8_t *rgba;
+} vt3d_frame;
+
+/* RegisteredCamera (crates/video-to-3d-core/src/multi_view.rs:85-95): world->camera,
+ * x_c = R x_w + t. R is column-major, the StageCamera layout
+ * (crates/video-to-3d-core/src/reconstruction_stages.rs:214-240). */
+typedef struct vt3d_camera {
+    uint64_t frame_index;
+    double rotation_colmajor[9];
+    double translation[3];
+} vt3d_camera;
+
+/* ReconstructionControl poll (crates/video-to-3d-core/src/reconstruction/control.rs:5-26).
+ * Nonzero = cancel. Called only on the calling thread, once per grid row (sticky). */
+typedef int32_t (*vt3d_poll_fn)(void *user);
+
+enum vt3d_cost_kind { VT3D_COST_ZSAD = 0, VT3D_COST_ZNCC = 1 };
+enum vt3d_ambiguity_kind { VT3D_AMBIGUITY_MARGIN = 0, VT3D_AMBIGUITY_RATIO = 1 };
+enum vt3d_storage { VT3D_STORAGE_F32 = 0, VT3D_STORAGE_F16 = 1, VT3D_STORAGE_BF16 = 2 };
+
+/* Sweep parameters. vt3d_sweep_params_reference() gives the dense_legacy.rs:6-21 constants
+ * (3x3 zero-mean SAD, 24 hypotheses, 4 sources, additive ambiguity margin 1.0);
+ * vt3d_sweep_params_zncc() gives the ZNCC preset (1 - ZNCC cost, ratio ambiguity). */
+typedef struct vt3d_sweep_params {
+    uint32_t struct_size;              /* sizeof(vt3d_sweep_params) */
+    int32_t cost_kind;                 /* vt3d_cost_kind */
+    int32_t patch_radius;              /* PATCH_RADIUS, dense_legacy.rs:9 */
+    int32_t depth_hypotheses;          /* DEPTH_HYPOTHESES, dense_legacy.rs:7 */
+    int32_t max_source_views;          /* MAX_SOURCE_VIEWS, dense_legacy.rs:6 (<= 32) */
+    int32_t grid_stride;               /* 0: dense.rs:215-218 rule */
+    int32_t ambiguity_kind;            /* vt3d_ambiguity_kind */
+    int32_t min_fusion_observations;   /* MIN_FUSION_OBSERVATIONS, dense_legacy.rs:15 */
+    int32_t surface_completion_passes; /* SURFACE_COMPLETION_PASSES, dense_legacy.rs:16 */
+    int32_t storage;                   /* vt3d_storage: image storage on the device */
+    int32_t emit_samples;              /* nonzero: fill vt3d_dense_result.samples */
+    int32_t reserved0;
+    double min_reference_contrast;     /* dense_legacy.rs:10 */
+    double max_photometric_error;      /* dense_legacy.rs:11 (cost units) */
+    double ambiguity_threshold;        /* margin (dense_legacy.rs:12) or ratio */
+    double max_reciprocal_relative_depth_error; /* dense_legacy.rs:13 */
+    double max_fusion_relative_position_error;  /* dense_legacy.rs:14 */
+    double error_confidence_scale;     /* 8.0 in dense_legacy.rs:390 */
+    double depth_margin_saturation;    /* DEPTH_MARGIN_SATURATION, dense_legacy.rs:21 */
+    double depth_min_override;         /* > 0 with max: primary sweep range; else envelope */
+    double depth_max_override;
+} vt3d_sweep_params;
+
+/* Point3 (crates/video-to-3d-core/src/reconstruction/types.rs:68-77). */
+typedef struct vt3d_point3 {
+    float x, y, z, confidence;
+    uint8_t r, g, b, pad;
+} vt3d_point3;
+
+/* DenseGridSite (dense_legacy.rs:55-59). */
+typedef struct vt3d_grid_site {
+    uint32_t x, y;
+} vt3d_grid_site;
+
+/* EvidencePointAttributes (crates/video-to-3d-core/src/reconstruction_evidence.rs:107-115). */
+typedef struct vt3d_point_attributes {
+    float reciprocal_consistency;
+    float depth_margin;
+} vt3d_point_attributes;
+
+/* DenseStats (dense_legacy.rs:23-53). Option<f32> fields carry a has_ flag. */
+typedef struct vt3d_dense_stats {
+    int32_t attempted;
+    int32_t has_reference_frame;
+    uint64_t reference_frame;
+    uint64_t source_views;
+    uint64_t source_frames[32];
+    uint64_t sampled_pixels;
+    uint64_t depth_hypotheses;
+    uint64_t accepted_points;
+    uint64_t surface_completion_proposals;
+    uint64_t surface_completed_points;
+    uint64_t surface_completion_rejected_texture;
+    uint64_t surface_completion_rejected_cross_view;
+    uint64_t surface_completion_rejected_reciprocal;
+    uint64_t surface_completion_rejected_fusion;
+    uint64_t surface_completion_rejected_footprint;
+    uint64_t grid_stride;
+    uint32_t grid_border;
+    uint32_t pad0;
+    uint64_t reciprocal_checked_points;
+    uint64_t reciprocal_rejected_points;
+    uint64_t reciprocal_consistent_points;
+    uint64_t fusion_input_observations;
+    uint64_t fusion_rejected_observations;
+    uint64_t fusion_rejected_points;
+    int32_t has_median_fusion_observations;
+    float median_fusion_observations;
+    int32_t has_median_supporting_views;
+    float median_supporting_views;
+    int32_t has_median_photometric_error;
+    float median_photometric_error;
+    int32_t has_search_depth;
+    float search_min_depth;
+    float search_max_depth;
+    int32_t pad1;
+} vt3d_dense_stats;
+
+/* Outcome of one texture-gated grid sample (diagnostics; no counterpart type in the
+ * reference, the codes name the `continue` branches of dense_legacy.rs:347-384). */
+enum vt3d_sample_status {
+    VT3D_SAMPLE_NO_CANDIDATE = 1,      /* dense_legacy.rs:347-349 */
+    VT3D_SAMPLE_REJECT_COST = 2,       /* dense_legacy.rs:350-352 */
+    VT3D_SAMPLE_REJECT_AMBIGUITY = 3,  /* dense_legacy.rs:360-362 */
+    VT3D_SAMPLE_REJECT_RECIPROCAL = 4, /* dense_legacy.rs:372-375 */
+    VT3D_SAMPLE_REJECT_FUSION = 5,     /* dense_legacy.rs:381-384 */
+    VT3D_SAMPLE_ACCEPTED = 6
+};
+
+typedef struct vt3d_sample_diag {
+    uint32_t x, y;
+    int32_t status;              /* vt3d_sample_status */
+    int32_t best_hypothesis;     /* -1: no candidate */
+    int32_t best_support;
+    int32_t second_hypothesis;   /* comparable second (same support), -1: none */
+    float best_cost;
+    float second_cost;
+    uint32_t reciprocal_mask;    /* bit s: source s gave a reciprocal observation */
+    uint32_t fused_observations; /* 0 unless fusion ran */
+} vt3d_sample_diag;
+
+/* Skip codes; vt3d_skip_reason() returns the exact reference string. */
+enum vt3d_skip_code {
+    VT3D_SKIP_NONE = 0,
+    VT3D_SKIP_NO_FRAMES = 1,           /* dense_legacy.rs:157-159 */
+    VT3D_SKIP_FEW_CAMERAS = 2,         /* dense_legacy.rs:160-162 */
+    VT3D_SKIP_FEW_LANDMARKS = 3,       /* dense_legacy.rs:163-167 */
+    VT3D_SKIP_SMALL_FRAMES = 4,        /* dense_legacy.rs:171-173 */
+    VT3D_SKIP_NO_REFERENCE = 5,        /* dense_legacy.rs:187-191 */
+    VT3D_SKIP_NO_ENVELOPE = 6,         /* dense_legacy.rs:196-200 */
+    VT3D_SKIP_NO_REFERENCE_FRAME = 7,  /* dense_legacy.rs:205-207 */
+    VT3D_SKIP_NO_SOURCES = 8,          /* dense_legacy.rs:232-236 */
+    VT3D_SKIP_CANCELED = 9             /* dense_legacy.rs:273-275 */
+};
+
+/* DenseAnalysis (dense_legacy.rs:61-68), library-owned. points/grid_sites/attributes are
+ * parallel arrays of n_points entries in reference order: grid row-major, then surface
+ * completion additions (dense_legacy.rs:727-733). */
+typedef struct vt3d_dense_result {
+    vt3d_dense_stats stats;
+    int32_t skip_code;
+    char skip_reason[256];
+    size_t n_points;
+    size_t primary_points; /* points before surface completion */
+    vt3d_point3 *points;
+    vt3d_grid_site *grid_sites;
+    vt3d_point_attributes *attributes;
+    size_t n_samples;      /* texture-gated samples, row-major (emit_samples only) */
+    vt3d_sample_diag *samples;
+} vt3d_dense_result;
+
+/* Device-side timing of the last solve on a session, milliseconds (HIP events on the
+ * session stream). */
+typedef struct vt3d_timing {
+    double total_ms;
+    double upload_ms;
+    double gate_ms;
+    double sweep_ms;
+    double reciprocal_ms;
+    double fuse_ms;
+    double completion_ms;
+    double download_ms;
+    uint64_t units; /* sampled_pixels * depth_hypotheses */
+} vt3d_timing;
+
+/* ---- parameters and strings ---------------------------------------------------------- */
+
+/* dense_legacy.rs:6-21 constants. */
+void vt3d_sweep_params_reference(vt3d_sweep_params *params);
+/* ZNCC preset: 1 - ZNCC cost, max cost 0.5, ratio ambiguity 0.9. */
+void vt3d_sweep_params_zncc(vt3d_sweep_params *params, int32_t patch_radius,
+                            int32_t depth_hypotheses, int32_t max_source_views);
+/* Exact skip string for a vt3d_skip_code (dense_legacy.rs:157-275). */
+const char *vt3d_skip_reason(int32_t skip_code);
+/* grid_stride (dense.rs:215-218). */
+uint32_t vt3d_grid_stride(uint32_t width, uint32_t height, int32_t fine_grid);
+int32_t vt3d_abi_version(void);
+
+/* ---- one-shot entry points (host buffers in, host result out) ------------------------ */
+
+/* Replaces estimate_depth_points_with_control (dense_legacy.rs:149-156). params == NULL
+ * means vt3d_sweep_params_reference(). */
+int32_t vt3d_dense_plane_sweep(const vt3d_frame *frames, size_t n_frames,
+                               const vt3d_camera *cameras, size_t n_cameras,
+                               const double *sparse_xyz, size_t n_sparse, double focal,
+                               int32_t fine_grid, const vt3d_sweep_params *params,
+                               vt3d_poll_fn poll, void *poll_user, vt3d_dense_result **out);
+
+/* Replaces reconstruct_reference_patch (dense.rs:674-747): remaps frames so that camera
+ * `reference_camera_index` is index 0 and the rest follow nearest-frame-first, keeps the
+ * landmarks visible in the reference, runs the sweep and maps the frame indices back.
+ * Returns VT3D_E_INVALID where the reference returns None before sweeping. */
+int32_t vt3d_dense_reference_patch(const vt3d_frame *frames, size_t n_frames,
+                                   const vt3d_camera *cameras, size_t n_cameras,
+                                   const double *sparse_xyz, size_t n_sparse, double focal,
+                                   size_t reference_camera_index, int32_t fine_grid,
+                                   const vt3d_sweep_params *params, vt3d_poll_fn poll,
+                                   void *poll_user, vt3d_dense_result **out);
+
+void vt3d_dense_result_free(vt3d_dense_result *result);
+
+/* ---- device-resident sessions (frames uploaded once, solved many times) -------------- */
+/* No counterpart in the reference: the reference re-reads host frames every call. A
+ * session keeps luma textures in HBM so repeated solves (benchmarks, batching, serving)
+ * measure the sweep with inputs already resident. */
+
+typedef struct vt3d_session vt3d_session;
+
+int32_t vt3d_session_create(int32_t device, vt3d_session **out);
+void vt3d_session_destroy(vt3d_session *session);
+/* Uploads frames (RGBA8) and converts them to luma textures in `storage` precision.
+ * Replaces any frames previously uploaded to the session. */
+int32_t vt3d_session_upload_frames(vt3d_session *session, const vt3d_frame *frames,
+                                   size_t n_frames, int32_t storage);
+/* Same contract as vt3d_dense_reference_patch (reference_camera_index >= 0) or
+ * vt3d_dense_plane_sweep (reference_camera_index < 0), on the uploaded frames. */
+int32_t vt3d_session_solve(vt3d_session *session, const vt3d_camera *cameras,
+                           size_t n_cameras, const double *sparse_xyz, size_t n_sparse,
+                           double focal, int64_t reference_camera_index, int32_t fine_grid,
+                           const vt3d_sweep_params *params, vt3d_poll_fn poll,
+                           void *poll_user, vt3d_dense_result **out);
+int32_t vt3d_session_last_timing(const vt3d_session *session, vt3d_timing *timing);
+
+/* Last device or validation error message of the calling thread ("" if none). */
+const char *vt3d_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VT3D_DENSE_H */

@@ -1,0 +1,1280 @@
+// vt3d_oracle.cpp -- FP64 CPU oracle of the dense plane-sweep depth pass.
+//
+// TEST INFRASTRUCTURE ONLY: the checker for the HIP path, and the CPU-baseline leg of the
+// benchmark. It restates crates/video-to-3d-core/src/dense_legacy.rs line for line in
+// double precision (every function below cites the lines it follows), generalised by
+// vt3d_sweep_params so that the reference constants (dense_legacy.rs:6-21) are one preset
+// and the ZNCC configurations another. Built with -ffp-contract=off so that every
+// multiply and add rounds separately, as Rust's f64 arithmetic does.
+#include "vt3d_oracle.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+#include <limits>
+#include <map>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+constexpr double kInf = std::numeric_limits<double>::infinity();
+constexpr size_t kMinVisibleSparsePoints = 8;        // dense_legacy.rs:8
+constexpr size_t kMinSurfaceNeighbors = 3;           // dense_legacy.rs:17
+constexpr double kMaxSurfaceNeighborSpread = 0.06;   // dense_legacy.rs:18
+constexpr double kSurfaceConfidenceScale = 0.85;     // dense_legacy.rs:19
+// ZNCC preset: a patch whose mean squared deviation is below this (intensity^2) has no
+// defined correlation and costs 1 (uncorrelated).
+constexpr double kZnccMinVariance = 1.0e-4;
+constexpr int kMaxRadius = 7;
+constexpr int kMaxTaps = (2 * kMaxRadius + 1) * (2 * kMaxRadius + 1);
+
+struct V3 {
+    double x, y, z;
+};
+inline V3 operator-(V3 a, V3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+inline V3 operator+(V3 a, V3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+inline double norm(V3 v) { return std::sqrt(v.x * v.x + v.y * v.y + v.z * v.z); }
+inline bool finite3(V3 v) { return std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z); }
+
+struct Cam {
+    uint64_t frame_index;
+    double R[3][3];  // row-major
+    double t[3];
+};
+
+Cam to_cam(const vt3d_camera &c) {
+    Cam k;
+    k.frame_index = c.frame_index;
+    for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 3; ++row) k.R[row][col] = c.rotation_colmajor[col * 3 + row];
+    for (int i = 0; i < 3; ++i) k.t[i] = c.translation[i];
+    return k;
+}
+
+// nalgebra's 3x3 * 3-vector accumulates column by column: (m0*v0 + m1*v1) + m2*v2.
+inline V3 mul(const double M[3][3], V3 v) {
+    return {(M[0][0] * v.x + M[0][1] * v.y) + M[0][2] * v.z,
+            (M[1][0] * v.x + M[1][1] * v.y) + M[1][2] * v.z,
+            (M[2][0] * v.x + M[2][1] * v.y) + M[2][2] * v.z};
+}
+inline V3 mul_t(const double M[3][3], V3 v) {
+    return {(M[0][0] * v.x + M[1][0] * v.y) + M[2][0] * v.z,
+            (M[0][1] * v.x + M[1][1] * v.y) + M[2][1] * v.z,
+            (M[0][2] * v.x + M[1][2] * v.y) + M[2][2] * v.z};
+}
+
+// RegisteredCamera::camera_center, core/src/multi_view.rs:92-94: -R^T t.
+V3 camera_center(const Cam &c) {
+    V3 r = mul_t(c.R, {c.t[0], c.t[1], c.t[2]});
+    return {-r.x, -r.y, -r.z};
+}
+
+struct Params {
+    int cost_kind, radius, D, max_sources, stride_override, ambiguity_kind, min_obs, passes;
+    double min_contrast, max_err, ambiguity, recip_rel, fusion_rel, err_scale, margin_sat;
+    double dmin, dmax;
+    bool diag;
+};
+
+Params to_params(const vt3d_sweep_params &p) {
+    Params q;
+    q.cost_kind = p.cost_kind;
+    q.radius = p.patch_radius;
+    q.D = p.depth_hypotheses;
+    q.max_sources = p.max_source_views;
+    q.stride_override = p.grid_stride;
+    q.ambiguity_kind = p.ambiguity_kind;
+    q.min_obs = p.min_fusion_observations;
+    q.passes = p.surface_completion_passes;
+    q.min_contrast = p.min_reference_contrast;
+    q.max_err = p.max_photometric_error;
+    q.ambiguity = p.ambiguity_threshold;
+    q.recip_rel = p.max_reciprocal_relative_depth_error;
+    q.fusion_rel = p.max_fusion_relative_position_error;
+    q.err_scale = p.error_confidence_scale;
+    q.margin_sat = p.depth_margin_saturation;
+    q.dmin = p.depth_min_override;
+    q.dmax = p.depth_max_override;
+    q.diag = p.emit_samples != 0;
+    return q;
+}
+
+struct Slack {
+    double cost = kInf, px = kInf, rel = kInf;
+    void c(double v) { cost = std::min(cost, std::fabs(v)); }
+    void p(double v) { px = std::min(px, std::fabs(v)); }
+    void r(double v) { rel = std::min(rel, std::fabs(v)); }
+    void merge(const Slack &o) {
+        cost = std::min(cost, o.cost);
+        px = std::min(px, o.px);
+        rel = std::min(rel, o.rel);
+    }
+};
+
+// unproject, dense_legacy.rs:996-1011.
+V3 unproject(const Cam &c, double x, double y, double depth, uint32_t w, uint32_t h, double f) {
+    V3 cp{(x - w * 0.5) / f * depth, (y - h * 0.5) / f * depth, depth};
+    return mul_t(c.R, cp - V3{c.t[0], c.t[1], c.t[2]});
+}
+
+// project, dense_legacy.rs:1013-1029.
+bool project(const Cam &c, V3 p, uint32_t w, uint32_t h, double f, double *ox, double *oy,
+             double *oz) {
+    V3 cp = mul(c.R, p) + V3{c.t[0], c.t[1], c.t[2]};
+    if (!finite3(cp) || cp.z <= 1.0e-4) return false;
+    *ox = f * cp.x / cp.z + w * 0.5;
+    *oy = f * cp.y / cp.z + h * 0.5;
+    *oz = cp.z;
+    return true;
+}
+
+// in_image_bounds, dense_legacy.rs:1129-1131.
+bool in_image_bounds(double x, double y, uint32_t w, uint32_t h) {
+    return std::isfinite(x) && std::isfinite(y) && x >= 0.0 && y >= 0.0 && x < (double)w &&
+           y < (double)h;
+}
+
+// in_bilinear_bounds, dense_legacy.rs:1133-1140 (saturating_sub).
+bool in_bilinear_bounds(double x, double y, uint32_t w, uint32_t h) {
+    double wm = (double)(w > 0 ? w - 1 : 0), hm = (double)(h > 0 ? h - 1 : 0);
+    return std::isfinite(x) && std::isfinite(y) && x >= 0.0 && y >= 0.0 && x < wm && y < hm;
+}
+
+// Distance of (x, y) from the edge of the bilinear-bounds region: inside, how far it
+// may move before leaving; outside, how far it must move to enter.
+double bilinear_edge_distance(double x, double y, uint32_t w, uint32_t h) {
+    double wm = (double)(w > 0 ? w - 1 : 0), hm = (double)(h > 0 ? h - 1 : 0);
+    if (!std::isfinite(x) || !std::isfinite(y)) return kInf;
+    if (in_bilinear_bounds(x, y, w, h)) return std::min(std::min(x, wm - x), std::min(y, hm - y));
+    double d = 0.0;
+    if (x < 0.0) d = std::max(d, -x);
+    if (x >= wm) d = std::max(d, x - wm);
+    if (y < 0.0) d = std::max(d, -y);
+    if (y >= hm) d = std::max(d, y - hm);
+    return d;
+}
+
+// in_patch_bounds, dense_legacy.rs:1142-1145 (margin = PATCH_RADIUS + 1).
+bool in_patch_bounds(double x, double y, uint32_t w, uint32_t h, int radius) {
+    double m = (double)(radius + 1);
+    return x >= m && y >= m && x < (double)w - m && y < (double)h - m;
+}
+
+// sample_bilinear, dense_legacy.rs:1113-1127.
+bool sample_bilinear(const uint8_t *l, uint32_t w, uint32_t h, double x, double y, double *out) {
+    if (!in_bilinear_bounds(x, y, w, h)) return false;
+    uint32_t x0 = (uint32_t)std::floor(x), y0 = (uint32_t)std::floor(y);
+    uint32_t x1 = x0 + 1, y1 = y0 + 1;
+    double tx = x - (double)x0, ty = y - (double)y0;
+    auto v = [&](uint32_t px, uint32_t py) { return (double)l[(size_t)py * w + px]; };
+    double top = v(x0, y0) * (1.0 - tx) + v(x1, y0) * tx;
+    double bottom = v(x0, y1) * (1.0 - tx) + v(x1, y1) * tx;
+    *out = top * (1.0 - ty) + bottom * ty;
+    return true;
+}
+
+// patch_contrast, dense_legacy.rs:1097-1111.
+double patch_contrast(const uint8_t *l, uint32_t w, uint32_t h, double x, double y, int r) {
+    double mn = kInf, mx = -kInf;
+    for (int dy = -r; dy <= r; ++dy)
+        for (int dx = -r; dx <= r; ++dx) {
+            double v;
+            if (!sample_bilinear(l, w, h, x + dx, y + dy, &v)) return 0.0;
+            mn = std::min(mn, v);
+            mx = std::max(mx, v);
+        }
+    return mx - mn;
+}
+
+// quantile, dense_legacy.rs:1168-1174 (f64::round rounds half away from zero, as round()).
+double quantile(const double *v, size_t n, double q) {
+    if (n == 0) return std::numeric_limits<double>::quiet_NaN();
+    double c = std::min(std::max(q, 0.0), 1.0);
+    size_t idx = (size_t)std::round((double)(n - 1) * c);
+    return v[idx];
+}
+
+// sort_by(partial_cmp.unwrap_or(Equal)) on values that are never NaN here: a stable sort.
+void sort_f64(std::vector<double> &v) { std::stable_sort(v.begin(), v.end()); }
+
+// median_option, dense_legacy.rs:1176-1182.
+bool median_option(std::vector<double> &v, double *out) {
+    if (v.empty()) return false;
+    sort_f64(v);
+    *out = quantile(v.data(), v.size(), 0.5);
+    return true;
+}
+
+// to_luma, dense_legacy.rs:1147-1157.
+std::vector<uint8_t> to_luma(const vt3d_frame &f) {
+    size_t n = (size_t)f.width * f.height;
+    std::vector<uint8_t> l(n);
+    for (size_t i = 0; i < n; ++i) {
+        const uint8_t *p = f.rgba + 4 * i;
+        l[i] = (uint8_t)(((uint16_t)(77 * (uint16_t)p[0] + 150 * (uint16_t)p[1] +
+                                     29 * (uint16_t)p[2])) >>
+                         8);
+    }
+    return l;
+}
+
+// patch_error, dense_legacy.rs:1031-1095, generalised over cost kind and radius. With a
+// slack tracker every tap is evaluated so that the validity slack is exact.
+bool patch_error(const Cam &rc, const Cam &sc, const uint8_t *rl, const uint8_t *sl, uint32_t rw,
+                 uint32_t rh, uint32_t sw, uint32_t sh, double x, double y, double depth, double f,
+                 int cost_kind, int r, double *out, Slack *sk) {
+    double rv[kMaxTaps], sv[kMaxTaps];
+    int n = 0;
+    bool valid = true;
+    double px_in = kInf, px_out = 0.0;
+    for (int dy = -r; dy <= r; ++dy) {
+        for (int dx = -r; dx <= r; ++dx) {
+            double rx = x + dx, ry = y + dy;
+            V3 world = unproject(rc, rx, ry, depth, rw, rh, f);
+            double sx, sy, sz;
+            if (!project(sc, world, sw, sh, f, &sx, &sy, &sz)) {
+                valid = false;
+                if (!sk) return false;
+                continue;
+            }
+            bool inb = in_bilinear_bounds(sx, sy, sw, sh);
+            if (sk) {
+                double d = bilinear_edge_distance(sx, sy, sw, sh);
+                if (inb) px_in = std::min(px_in, d);
+                else px_out = std::max(px_out, d);
+            }
+            if (!inb) {
+                valid = false;
+                if (!sk) return false;
+                continue;
+            }
+            double a, b;
+            if (!sample_bilinear(rl, rw, rh, rx, ry, &a)) {
+                valid = false;
+                if (!sk) return false;
+                continue;
+            }
+            sample_bilinear(sl, sw, sh, sx, sy, &b);
+            rv[n] = a;
+            sv[n] = b;
+            ++n;
+        }
+    }
+    if (sk) sk->p(valid ? px_in : px_out);
+    if (!valid) return false;
+    double rs = 0.0, ss = 0.0;
+    for (int i = 0; i < n; ++i) rs += rv[i];
+    for (int i = 0; i < n; ++i) ss += sv[i];
+    double rm = rs / n, sm = ss / n;
+    if (cost_kind == VT3D_COST_ZSAD) {
+        double acc = 0.0;
+        for (int i = 0; i < n; ++i) acc += std::fabs((rv[i] - rm) - (sv[i] - sm));
+        *out = acc / n;
+    } else {
+        double cov = 0.0, vr = 0.0, vs = 0.0;
+        for (int i = 0; i < n; ++i) {
+            double a = rv[i] - rm, b = sv[i] - sm;
+            cov += a * b;
+            vr += a * a;
+            vs += b * b;
+        }
+        if (vr / n < kZnccMinVariance || vs / n < kZnccMinVariance) *out = 1.0;
+        else *out = 1.0 - cov / std::sqrt(vr * vs);
+    }
+    return true;
+}
+
+// Inverse-depth hypothesis schedule, dense_legacy.rs:285-293 and 909-916.
+inline double hypothesis_depth(int h, int D, double mn, double mx) {
+    double t = D == 1 ? 0.0 : (double)h / (double)(D - 1);
+    double inv = (1.0 / mn) * (1.0 - t) + (1.0 / mx) * t;
+    return 1.0 / inv;
+}
+
+// visible_depths, dense_legacy.rs:959-973.
+std::vector<double> visible_depths(const Cam &c, const std::vector<V3> &sp, uint32_t w,
+                                   uint32_t h, double f) {
+    std::vector<double> out;
+    for (const V3 &p : sp) {
+        double x, y, z;
+        if (!project(c, p, w, h, f, &x, &y, &z)) continue;
+        if (in_image_bounds(x, y, w, h)) out.push_back(z);
+    }
+    return out;
+}
+
+// depth_search_bounds, dense_legacy.rs:755-775.
+bool depth_search_bounds(const Cam &c, const std::vector<V3> &sp, uint32_t w, uint32_t h,
+                         double f, double *mn, double *mx) {
+    std::vector<double> d = visible_depths(c, sp, w, h, f);
+    if (d.size() < kMinVisibleSparsePoints) return false;
+    sort_f64(d);
+    double lower = quantile(d.data(), d.size(), 0.10);
+    double upper = quantile(d.data(), d.size(), 0.90);
+    if (!std::isfinite(lower) || !std::isfinite(upper) || lower <= 0.0 || upper <= 0.0)
+        return false;
+    double minimum = std::max(lower * 0.8, 1.0e-4);
+    double maximum = std::max(upper * 1.25, minimum * 1.2);
+    *mn = minimum;
+    *mx = maximum;
+    return true;
+}
+
+// projects_reference_center, dense_legacy.rs:975-994 (the source is projected with the
+// reference's width and height, as the reference does).
+bool projects_reference_center(const Cam &ref, const Cam &src, double depth, uint32_t w,
+                               uint32_t h, double f, int radius) {
+    V3 p = unproject(ref, w * 0.5, h * 0.5, depth, w, h, f);
+    double x, y, z;
+    if (!project(src, p, w, h, f, &x, &y, &z)) return false;
+    return in_patch_bounds(x, y, w, h, radius);
+}
+
+struct SourceView {
+    Cam cam;
+    uint32_t w, h;
+    const vt3d_frame *frame;
+    std::vector<uint8_t> luma;
+    double min_depth, max_depth;
+};
+
+// estimate_single_view_depth, dense_legacy.rs:887-949.
+bool single_view_depth(const Cam &rc, const Cam &sc, const uint8_t *rl, const uint8_t *sl,
+                       uint32_t rw, uint32_t rh, uint32_t sw, uint32_t sh, double x, double y,
+                       double mn, double mx, double f, const Params &p, double *out, Slack *sk) {
+    double contrast = patch_contrast(rl, rw, rh, x, y, p.radius);
+    if (sk) sk->c(contrast - p.min_contrast);
+    if (contrast < p.min_contrast) return false;
+    struct C {
+        double depth, error;
+    };
+    std::vector<C> cands;
+    cands.reserve(p.D);
+    for (int hyp = 0; hyp < p.D; ++hyp) {
+        double depth = hypothesis_depth(hyp, p.D, mn, mx);
+        double e;
+        if (!patch_error(rc, sc, rl, sl, rw, rh, sw, sh, x, y, depth, f, p.cost_kind, p.radius,
+                         &e, sk))
+            continue;
+        if (sk) sk->c(e - p.max_err);
+        if (e <= p.max_err) cands.push_back({depth, e});
+    }
+    std::stable_sort(cands.begin(), cands.end(), [](const C &a, const C &b) {
+        if (a.error != b.error) return a.error < b.error;
+        return a.depth < b.depth;
+    });
+    if (cands.empty()) return false;
+    if (sk && cands.size() > 1) sk->c(cands[1].error - cands[0].error);
+    *out = cands[0].depth;
+    return true;
+}
+
+// reciprocal_depth_agrees, dense_legacy.rs:951-957.
+bool reciprocal_agrees(double expected, double estimated, double max_rel, Slack *sk) {
+    if (!std::isfinite(expected) || !std::isfinite(estimated) || expected <= 0.0 ||
+        estimated <= 0.0)
+        return false;
+    double scale = std::max(std::max(expected, estimated), 1.0e-9);
+    double rel = std::fabs(expected - estimated) / scale;
+    if (sk) sk->r(rel - max_rel);
+    return rel <= max_rel;
+}
+
+// reciprocal_depth_observations, dense_legacy.rs:777-857. mask gets bit s for each source
+// that contributed a position.
+std::vector<V3> reciprocal_observations(V3 position, const Cam &rc, uint32_t rw, uint32_t rh,
+                                        const uint8_t *rl, const std::vector<SourceView> &srcs,
+                                        double f, const Params &p, uint32_t *mask, Slack *sk) {
+    std::vector<V3> out;
+    *mask = 0;
+    double xr, yr, dr;
+    if (!project(rc, position, rw, rh, f, &xr, &yr, &dr)) return out;
+    for (size_t s = 0; s < srcs.size(); ++s) {
+        const SourceView &src = srcs[s];
+        double direct;
+        if (!patch_error(rc, src.cam, rl, src.luma.data(), rw, rh, src.w, src.h, xr, yr, dr, f,
+                         p.cost_kind, p.radius, &direct, sk))
+            continue;
+        if (sk) sk->c(direct - p.max_err);
+        if (direct > p.max_err) continue;
+        double xs, ys, zs;
+        if (!project(src.cam, position, src.w, src.h, f, &xs, &ys, &zs)) continue;
+        double recip;
+        if (!single_view_depth(src.cam, rc, src.luma.data(), rl, src.w, src.h, rw, rh, xs, ys,
+                               src.min_depth, src.max_depth, f, p, &recip, sk))
+            continue;
+        if (!reciprocal_agrees(zs, recip, p.recip_rel, sk)) continue;
+        V3 pos = unproject(src.cam, xs, ys, recip, src.w, src.h, f);
+        if (!finite3(pos)) continue;
+        out.push_back(pos);
+        *mask |= 1u << s;
+    }
+    return out;
+}
+
+struct Fused {
+    V3 position;
+    size_t observations, rejected;
+};
+
+// fuse_depth_observations, dense_legacy.rs:859-884.
+Fused fuse(V3 primary, const std::vector<V3> &recip, double ref_depth, double max_rel,
+           Slack *sk) {
+    double radius = std::max(std::fabs(ref_depth) * max_rel, 1.0e-4);
+    V3 sum = primary;
+    size_t obs = 1, rej = 0;
+    for (const V3 &q : recip) {
+        double dist = finite3(q) ? norm(q - primary) : kInf;
+        if (sk && std::isfinite(dist)) sk->r((dist - radius) / radius);
+        if (!finite3(q) || dist > radius) {
+            ++rej;
+            continue;
+        }
+        sum = sum + q;
+        ++obs;
+    }
+    return {{sum.x / (double)obs, sum.y / (double)obs, sum.z / (double)obs}, obs, rej};
+}
+
+// depth_margin_score, dense_legacy.rs:473-479.
+double depth_margin_score(double margin, double saturation) {
+    if (std::isfinite(margin)) return std::min(std::max(margin / saturation, 0.0), 1.0);
+    return 1.0;
+}
+
+// surface_depth_prediction, dense_legacy.rs:739-753.
+bool surface_depth_prediction(std::vector<double> &d, double *out) {
+    if (d.size() < kMinSurfaceNeighbors) return false;
+    sort_f64(d);
+    double mn = d.front(), mx = d.back();
+    if (!std::isfinite(mn) || !std::isfinite(mx) || mn <= 0.0 || mx <= 0.0) return false;
+    if ((mx - mn) / mx > kMaxSurfaceNeighborSpread) return false;
+    *out = quantile(d.data(), d.size(), 0.5);
+    return true;
+}
+
+struct SampleOut {
+    uint32_t x, y;
+    int status;
+    int best_h, best_support, second_h;
+    double best_err, second_err;
+    uint32_t recip_mask;
+    size_t recip_count, fused_obs, fused_rejected;
+    vt3d_point3 point;
+    vt3d_point_attributes attr;
+    Slack slack;
+};
+
+struct Output {
+    std::vector<vt3d_point3> points;
+    std::vector<vt3d_grid_site> sites;
+    std::vector<vt3d_point_attributes> attrs;
+    std::vector<double> errors, supports;
+};
+
+struct SurfaceStats {
+    size_t proposals = 0, accepted = 0, rej_texture = 0, rej_cross = 0, rej_recip = 0,
+           rej_fusion = 0, rej_footprint = 0;
+};
+
+struct Control {
+    vt3d_poll_fn poll;
+    void *user;
+    bool canceled = false;
+    // ReconstructionControl::is_canceled, core/src/reconstruction/control.rs:16-19.
+    bool is_canceled() {
+        canceled = canceled || (poll && poll(user) != 0);
+        return canceled;
+    }
+};
+
+struct Sweep {
+    const Params &p;
+    const Cam &ref;
+    const vt3d_frame &ref_frame;
+    const std::vector<uint8_t> &ref_luma;
+    const std::vector<SourceView> &srcs;
+    uint32_t w, h;
+    double f, search_min, search_max;
+
+    // One grid sample of the hot loop, dense_legacy.rs:277-412.
+    bool sample(uint32_t x, uint32_t y, SampleOut *o) const {
+        if (patch_contrast(ref_luma.data(), w, h, x, y, p.radius) < p.min_contrast) return false;
+        Slack sk;
+        Slack *skp = p.diag ? &sk : nullptr;
+        o->x = x;
+        o->y = y;
+        o->best_h = o->second_h = -1;
+        o->best_support = 0;
+        o->best_err = o->second_err = kInf;
+        o->recip_mask = 0;
+        o->recip_count = o->fused_obs = o->fused_rejected = 0;
+        struct Cand {
+            double depth;
+            V3 position;
+            size_t support;
+            double error;
+            int h;
+        };
+        std::vector<Cand> cands;
+        cands.reserve(p.D);
+        std::vector<double> errs;
+        errs.reserve(srcs.size());
+        for (int hyp = 0; hyp < p.D; ++hyp) {
+            double depth = hypothesis_depth(hyp, p.D, search_min, search_max);
+            V3 position = unproject(ref, x, y, depth, w, h, f);
+            errs.clear();
+            for (const SourceView &src : srcs) {
+                double e;
+                if (!patch_error(ref, src.cam, ref_luma.data(), src.luma.data(), w, h, src.w,
+                                 src.h, x, y, depth, f, p.cost_kind, p.radius, &e, skp))
+                    continue;
+                if (skp) skp->c(e - p.max_err);
+                if (e <= p.max_err) errs.push_back(e);
+            }
+            if (errs.empty()) continue;
+            sort_f64(errs);
+            cands.push_back({depth, position, errs.size(), quantile(errs.data(), errs.size(), 0.5),
+                             hyp});
+        }
+        // dense_legacy.rs:332-346: support descending, error ascending, depth ascending.
+        std::stable_sort(cands.begin(), cands.end(), [](const Cand &a, const Cand &b) {
+            if (a.support != b.support) return a.support > b.support;
+            if (a.error != b.error) return a.error < b.error;
+            return a.depth < b.depth;
+        });
+        o->slack = sk;
+        if (cands.empty()) {
+            o->status = VT3D_SAMPLE_NO_CANDIDATE;
+            return true;
+        }
+        const Cand best = cands[0];
+        o->best_h = best.h;
+        o->best_support = (int)best.support;
+        o->best_err = best.error;
+        const Cand *second = nullptr;
+        // dense_legacy.rs:354-357: first later candidate with the same support.
+        for (size_t i = 1; i < cands.size(); ++i)
+            if (cands[i].support == best.support) {
+                second = &cands[i];
+                break;
+            }
+        if (second) {
+            o->second_h = second->h;
+            o->second_err = second->error;
+            if (skp) skp->c(second->error - best.error);
+        }
+        if (best.error > p.max_err) {
+            o->status = VT3D_SAMPLE_REJECT_COST;
+            o->slack = sk;
+            return true;
+        }
+        double margin = second ? second->error - best.error : kInf;
+        bool ambiguous;
+        if (p.ambiguity_kind == VT3D_AMBIGUITY_MARGIN) {
+            ambiguous = margin < p.ambiguity;
+            if (skp && second) skp->c(margin - p.ambiguity);
+        } else {
+            ambiguous = second && best.error > p.ambiguity * second->error;
+            if (skp && second) skp->c(best.error - p.ambiguity * second->error);
+        }
+        if (ambiguous) {
+            o->status = VT3D_SAMPLE_REJECT_AMBIGUITY;
+            o->slack = sk;
+            return true;
+        }
+        uint32_t mask = 0;
+        std::vector<V3> recip = reciprocal_observations(best.position, ref, w, h, ref_luma.data(),
+                                                        srcs, f, p, &mask, skp);
+        o->recip_mask = mask;
+        if (recip.empty()) {
+            o->status = VT3D_SAMPLE_REJECT_RECIPROCAL;
+            o->slack = sk;
+            return true;
+        }
+        o->recip_count = recip.size();
+        Fused fu = fuse(best.position, recip, best.depth, p.fusion_rel, skp);
+        o->fused_obs = fu.observations;
+        o->fused_rejected = fu.rejected;
+        o->slack = sk;
+        if (fu.observations < (size_t)p.min_obs) {
+            o->status = VT3D_SAMPLE_REJECT_FUSION;
+            return true;
+        }
+        // dense_legacy.rs:387-410.
+        double S = (double)srcs.size();
+        double support_conf = (double)best.support / S;
+        double recip_conf = std::min(std::max((double)(fu.observations - 1) / S, 0.0), 1.0);
+        double err_conf = 1.0 / (1.0 + best.error / p.err_scale);
+        double dm = depth_margin_score(margin, p.margin_sat);
+        double margin_conf = std::max(dm, 0.2);
+        double conf = std::min(std::max(support_conf * recip_conf * err_conf * margin_conf, 0.05),
+                               1.0);
+        const uint8_t *px = ref_frame.rgba + ((size_t)y * ref_frame.width + x) * 4;
+        o->point = {(float)fu.position.x, (float)fu.position.y, (float)fu.position.z, (float)conf,
+                    px[0], px[1], px[2], 0};
+        o->attr = {(float)recip_conf, (float)dm};
+        o->status = VT3D_SAMPLE_ACCEPTED;
+        return true;
+    }
+
+    // complete_surface_samples_with_control, dense_legacy.rs:519-737.
+    SurfaceStats complete(Output &out, uint32_t border, size_t stride, Control &ctl) const {
+        SurfaceStats st;
+        if (stride == 0 || srcs.empty()) return st;
+        const int64_t si = (int64_t)stride;
+        for (int pass = 0; pass < p.passes; ++pass) {
+            std::map<std::pair<uint32_t, uint32_t>, size_t> occupied;
+            for (size_t i = 0; i < out.sites.size(); ++i)
+                occupied[{out.sites[i].x, out.sites[i].y}] = i;
+            struct Add {
+                vt3d_point3 point;
+                vt3d_grid_site site;
+                vt3d_point_attributes attr;
+                double error, support;
+            };
+            std::vector<Add> additions;
+            for (uint32_t y = border; y < h - border; y += (uint32_t)stride) {
+                if (ctl.is_canceled()) return st;
+                for (uint32_t x = border; x < w - border; x += (uint32_t)stride) {
+                    if (occupied.count({x, y})) continue;
+                    std::vector<double> nd, nc, nm;
+                    for (int64_t oy = -1; oy <= 1; ++oy)
+                        for (int64_t ox = -1; ox <= 1; ++ox) {
+                            if (ox == 0 && oy == 0) continue;
+                            int64_t nx = (int64_t)x + ox * si, ny = (int64_t)y + oy * si;
+                            if (nx < 0 || ny < 0 || nx > 0xffffffffll || ny > 0xffffffffll)
+                                continue;
+                            auto it = occupied.find({(uint32_t)nx, (uint32_t)ny});
+                            if (it == occupied.end()) continue;
+                            if (it->second >= out.points.size()) continue;
+                            const vt3d_point3 &pt = out.points[it->second];
+                            double dx_, dy_, dz_;
+                            if (!project(ref, {(double)pt.x, (double)pt.y, (double)pt.z}, w, h, f,
+                                         &dx_, &dy_, &dz_))
+                                continue;
+                            nd.push_back(dz_);
+                            nc.push_back((double)pt.confidence);
+                            nm.push_back(it->second < out.attrs.size()
+                                             ? (double)out.attrs[it->second].depth_margin
+                                             : 0.0);
+                        }
+                    double predicted;
+                    if (!surface_depth_prediction(nd, &predicted)) continue;
+                    st.proposals++;
+                    if (patch_contrast(ref_luma.data(), w, h, x, y, p.radius) < p.min_contrast) {
+                        st.rej_texture++;
+                        continue;
+                    }
+                    V3 ppos = unproject(ref, x, y, predicted, w, h, f);
+                    std::vector<double> direct;
+                    for (const SourceView &src : srcs) {
+                        double e;
+                        if (!patch_error(ref, src.cam, ref_luma.data(), src.luma.data(), w, h,
+                                         src.w, src.h, x, y, predicted, f, p.cost_kind, p.radius,
+                                         &e, nullptr))
+                            continue;
+                        if (e <= p.max_err) direct.push_back(e);
+                    }
+                    if (direct.empty()) {
+                        st.rej_cross++;
+                        continue;
+                    }
+                    sort_f64(direct);
+                    double derr = quantile(direct.data(), direct.size(), 0.5);
+                    uint32_t mask;
+                    std::vector<V3> recip = reciprocal_observations(
+                        ppos, ref, w, h, ref_luma.data(), srcs, f, p, &mask, nullptr);
+                    if (recip.empty()) {
+                        st.rej_recip++;
+                        continue;
+                    }
+                    Fused fu = fuse(ppos, recip, predicted, p.fusion_rel, nullptr);
+                    if (fu.observations < (size_t)p.min_obs) {
+                        st.rej_fusion++;
+                        continue;
+                    }
+                    double px_, py_, pz_;
+                    if (!project(ref, fu.position, w, h, f, &px_, &py_, &pz_)) {
+                        st.rej_footprint++;
+                        continue;
+                    }
+                    double grid_limit = (double)stride * 0.5;
+                    if (std::fabs(px_ - (double)x) >= grid_limit ||
+                        std::fabs(py_ - (double)y) >= grid_limit) {
+                        st.rej_footprint++;
+                        continue;
+                    }
+                    double ncs = 0.0;
+                    for (double v : nc) ncs += v;
+                    double neighbor_conf = ncs / (double)nc.size();
+                    double S = (double)srcs.size();
+                    double support_conf = (double)direct.size() / S;
+                    double recip_conf =
+                        std::min(std::max((double)(fu.observations - 1) / S, 0.0), 1.0);
+                    double err_conf = 1.0 / (1.0 + derr / p.err_scale);
+                    double conf = std::min(
+                        std::max(neighbor_conf * support_conf * recip_conf * err_conf *
+                                     kSurfaceConfidenceScale,
+                                 0.05),
+                        0.9);
+                    double nms = 0.0;
+                    for (double v : nm) nms += v;
+                    double dm = nms / (double)nm.size();
+                    const uint8_t *pxl = ref_frame.rgba + ((size_t)y * ref_frame.width + x) * 4;
+                    additions.push_back({{(float)fu.position.x, (float)fu.position.y,
+                                          (float)fu.position.z, (float)conf, pxl[0], pxl[1],
+                                          pxl[2], 0},
+                                         {x, y},
+                                         {(float)recip_conf,
+                                          (float)std::min(std::max(dm, 0.0), 1.0)},
+                                         derr,
+                                         (double)direct.size()});
+                }
+            }
+            if (additions.empty()) break;
+            st.accepted += additions.size();
+            for (const Add &a : additions) {
+                out.points.push_back(a.point);
+                out.sites.push_back(a.site);
+                out.attrs.push_back(a.attr);
+                out.errors.push_back(a.error);
+                out.supports.push_back(a.support);
+            }
+        }
+        return st;
+    }
+};
+
+const char *skip_string(int code) {
+    switch (code) {
+    case VT3D_SKIP_NO_FRAMES: return "no sampled frames are available";
+    case VT3D_SKIP_FEW_CAMERAS: return "fewer than two accepted registered cameras are available";
+    case VT3D_SKIP_FEW_LANDMARKS: return "fewer than 8 accepted sparse landmarks are available";
+    case VT3D_SKIP_SMALL_FRAMES: return "sampled frames are too small for coarse depth estimation";
+    case VT3D_SKIP_NO_REFERENCE:
+        return "no registered camera sees at least 8 accepted sparse landmarks inside its image";
+    case VT3D_SKIP_NO_ENVELOPE:
+        return "visible sparse landmarks do not provide a finite positive depth-search envelope";
+    case VT3D_SKIP_NO_REFERENCE_FRAME: return "the selected reference camera has no sampled frame";
+    case VT3D_SKIP_NO_SOURCES:
+        return "no second registered camera has sufficient baseline, overlap, and at least 8 "
+               "visible sparse landmarks for reciprocal depth consistency";
+    case VT3D_SKIP_CANCELED: return "dense reconstruction canceled";
+    default: return "";
+    }
+}
+
+vt3d_dense_result *new_result() {
+    vt3d_dense_result *r = (vt3d_dense_result *)std::calloc(1, sizeof(vt3d_dense_result));
+    return r;
+}
+
+vt3d_dense_result *skipped(int code) {
+    vt3d_dense_result *r = new_result();
+    r->skip_code = code;
+    std::strncpy(r->skip_reason, skip_string(code), sizeof(r->skip_reason) - 1);
+    return r;
+}
+
+template <class T> T *copy_out(const std::vector<T> &v) {
+    if (v.empty()) return nullptr;
+    T *p = (T *)std::malloc(sizeof(T) * v.size());
+    std::memcpy(p, v.data(), sizeof(T) * v.size());
+    return p;
+}
+
+uint32_t grid_stride_rule(uint32_t w, uint32_t h, bool fine) {
+    // grid_stride, dense.rs:215-218.
+    uint32_t rows = fine ? 64 : 48;
+    uint32_t v = std::min(w, h) / rows;
+    return std::min(std::max(v, 4u), 12u);
+}
+
+// estimate_depth_points_with_control, dense_legacy.rs:149-470.
+vt3d_dense_result *plane_sweep(const vt3d_frame *frames, size_t n_frames,
+                               const vt3d_camera *cams_in, size_t n_cams, const double *sparse_xyz,
+                               size_t n_sparse, double f, bool fine_grid, const Params &p,
+                               Control &ctl, int threads, vt3d_oracle_slack **slack_out) {
+    if (n_frames == 0) return skipped(VT3D_SKIP_NO_FRAMES);
+    if (n_cams < 2) return skipped(VT3D_SKIP_FEW_CAMERAS);
+    if (n_sparse < kMinVisibleSparsePoints) return skipped(VT3D_SKIP_FEW_LANDMARKS);
+    const uint32_t w = frames[0].width, h = frames[0].height;
+    if (w < 16 || h < 16) return skipped(VT3D_SKIP_SMALL_FRAMES);
+
+    std::vector<Cam> cams(n_cams);
+    for (size_t i = 0; i < n_cams; ++i) cams[i] = to_cam(cams_in[i]);
+    std::vector<V3> sparse(n_sparse);
+    for (size_t i = 0; i < n_sparse; ++i)
+        sparse[i] = {sparse_xyz[3 * i], sparse_xyz[3 * i + 1], sparse_xyz[3 * i + 2]};
+
+    // dense_legacy.rs:175-191: max_by keeps the last of equal maxima.
+    int ref_i = -1;
+    std::vector<double> ref_depths;
+    for (size_t i = 0; i < n_cams; ++i) {
+        std::vector<double> d = visible_depths(cams[i], sparse, w, h, f);
+        if (d.size() < kMinVisibleSparsePoints) continue;
+        if (ref_i < 0) {
+            ref_i = (int)i;
+            ref_depths = d;
+            continue;
+        }
+        // compare(best, candidate): len, then reversed frame index.
+        int cmp;
+        if (ref_depths.size() != d.size()) cmp = ref_depths.size() < d.size() ? -1 : 1;
+        else if (cams[i].frame_index != cams[ref_i].frame_index)
+            cmp = cams[i].frame_index < cams[ref_i].frame_index ? -1 : 1;
+        else cmp = 0;
+        if (cmp <= 0) {
+            ref_i = (int)i;
+            ref_depths = d;
+        }
+    }
+    if (ref_i < 0) return skipped(VT3D_SKIP_NO_REFERENCE);
+    const Cam &ref = cams[ref_i];
+    sort_f64(ref_depths);
+    double lower = quantile(ref_depths.data(), ref_depths.size(), 0.10);
+    double upper = quantile(ref_depths.data(), ref_depths.size(), 0.90);
+    if (!std::isfinite(lower) || !std::isfinite(upper) || lower <= 0.0 || upper <= 0.0)
+        return skipped(VT3D_SKIP_NO_ENVELOPE);
+    double search_min = std::max(lower * 0.8, 1.0e-4);
+    double search_max = std::max(upper * 1.25, search_min * 1.2);
+    double median_depth = quantile(ref_depths.data(), ref_depths.size(), 0.50);
+    if (ref.frame_index >= n_frames) return skipped(VT3D_SKIP_NO_REFERENCE_FRAME);
+    const vt3d_frame &ref_frame = frames[ref.frame_index];
+    // The reference sizes its grid from frames[0] but reads the reference frame's pixels
+    // (dense_legacy.rs:169-170, 238); Rust panics on the out-of-range read when the two
+    // differ, which this contract reports as invalid input instead.
+    if (ref_frame.width != w || ref_frame.height != h) return nullptr;
+
+    // dense_legacy.rs:209-236.
+    struct Pick {
+        size_t cam;
+        double mn, mx;
+    };
+    std::vector<Pick> picks;
+    V3 rc = camera_center(ref);
+    for (size_t i = 0; i < n_cams; ++i) {
+        const Cam &c = cams[i];
+        if (c.frame_index == ref.frame_index) continue;
+        if (c.frame_index >= n_frames) continue;
+        const vt3d_frame &fr = frames[c.frame_index];
+        double baseline = norm(camera_center(c) - rc);
+        if (baseline < median_depth * 0.005 ||
+            !projects_reference_center(ref, c, median_depth, w, h, f, p.radius))
+            continue;
+        double mn, mx;
+        if (!depth_search_bounds(c, sparse, fr.width, fr.height, f, &mn, &mx)) continue;
+        picks.push_back({i, mn, mx});
+    }
+    auto absdiff = [](uint64_t a, uint64_t b) { return a > b ? a - b : b - a; };
+    std::stable_sort(picks.begin(), picks.end(), [&](const Pick &a, const Pick &b) {
+        uint64_t da = absdiff(cams[a.cam].frame_index, ref.frame_index);
+        uint64_t db = absdiff(cams[b.cam].frame_index, ref.frame_index);
+        if (da != db) return da < db;
+        return cams[a.cam].frame_index < cams[b.cam].frame_index;
+    });
+    if (picks.size() > (size_t)p.max_sources) picks.resize((size_t)p.max_sources);
+    if (picks.empty()) return skipped(VT3D_SKIP_NO_SOURCES);
+
+    std::vector<uint8_t> ref_luma = to_luma(ref_frame);
+    std::vector<SourceView> srcs;
+    for (const Pick &pk : picks) {
+        const Cam &c = cams[pk.cam];
+        const vt3d_frame &fr = frames[c.frame_index];
+        srcs.push_back({c, fr.width, fr.height, &fr, to_luma(fr), pk.mn, pk.mx});
+    }
+
+    // Explicit primary-sweep range of the ZNCC configurations (the reference always uses
+    // the landmark envelope; sources keep theirs for the reciprocal sweep).
+    double sweep_min = search_min, sweep_max = search_max;
+    if (p.dmin > 0.0 && p.dmax > p.dmin) {
+        sweep_min = p.dmin;
+        sweep_max = p.dmax;
+    }
+    const size_t stride =
+        p.stride_override > 0 ? (size_t)p.stride_override : grid_stride_rule(w, h, fine_grid);
+    const uint32_t border = (uint32_t)(p.radius + 2);
+    Sweep sw{p, ref, ref_frame, ref_luma, srcs, w, h, f, sweep_min, sweep_max};
+
+    std::vector<uint32_t> rows;
+    for (uint32_t y = border; y < h - border; y += (uint32_t)stride) rows.push_back(y);
+    std::vector<std::vector<SampleOut>> row_out(rows.size());
+    auto run_row = [&](size_t ri) {
+        uint32_t y = rows[ri];
+        for (uint32_t x = border; x < w - border; x += (uint32_t)stride) {
+            SampleOut o;
+            if (sw.sample(x, y, &o)) row_out[ri].push_back(o);
+        }
+    };
+    int nt = std::max(1, threads);
+    size_t band = (size_t)nt;
+    for (size_t r0 = 0; r0 < rows.size(); r0 += band) {
+        size_t r1 = std::min(rows.size(), r0 + band);
+        for (size_t ri = r0; ri < r1; ++ri)
+            if (ctl.is_canceled()) return skipped(VT3D_SKIP_CANCELED);
+        if (nt > 1) {
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
+            for (long ri = (long)r0; ri < (long)r1; ++ri) run_row((size_t)ri);
+        } else {
+            for (size_t ri = r0; ri < r1; ++ri) run_row(ri);
+        }
+    }
+
+    // Serial reduction in row-major order (dense_legacy.rs:363-412).
+    Output out;
+    size_t sampled = 0, checked = 0, rejected = 0, consistent = 0, fin = 0, frej_obs = 0,
+           frej_pts = 0;
+    std::vector<double> fusion_counts;
+    std::vector<vt3d_sample_diag> diags;
+    std::vector<vt3d_oracle_slack> slacks;
+    for (auto &ro : row_out)
+        for (const SampleOut &o : ro) {
+            ++sampled;
+            if (p.diag) {
+                vt3d_sample_diag d;
+                d.x = o.x;
+                d.y = o.y;
+                d.status = o.status;
+                d.best_hypothesis = o.best_h;
+                d.best_support = o.best_support;
+                d.second_hypothesis = o.second_h;
+                d.best_cost = (float)o.best_err;
+                d.second_cost = (float)o.second_err;
+                d.reciprocal_mask = o.recip_mask;
+                d.fused_observations = (uint32_t)o.fused_obs;
+                diags.push_back(d);
+                slacks.push_back({o.slack.cost, o.slack.px, o.slack.rel});
+            }
+            if (o.status >= VT3D_SAMPLE_REJECT_RECIPROCAL) ++checked;
+            if (o.status == VT3D_SAMPLE_REJECT_RECIPROCAL) ++rejected;
+            if (o.status >= VT3D_SAMPLE_REJECT_FUSION) {
+                ++consistent;
+                fin += 1 + o.recip_count;
+                frej_obs += o.fused_rejected;
+            }
+            if (o.status == VT3D_SAMPLE_REJECT_FUSION) ++frej_pts;
+            if (o.status == VT3D_SAMPLE_ACCEPTED) {
+                fusion_counts.push_back((double)o.fused_obs);
+                out.points.push_back(o.point);
+                out.sites.push_back({o.x, o.y});
+                out.attrs.push_back(o.attr);
+                out.errors.push_back(o.best_err);
+                out.supports.push_back((double)o.best_support);
+            }
+        }
+    size_t primary = out.points.size();
+    SurfaceStats ss = sw.complete(out, border, stride, ctl);
+
+    vt3d_dense_result *r = new_result();
+    vt3d_dense_stats &s = r->stats;
+    s.attempted = 1;
+    s.has_reference_frame = 1;
+    s.reference_frame = ref.frame_index;
+    s.source_views = srcs.size();
+    for (size_t i = 0; i < srcs.size() && i < 32; ++i) s.source_frames[i] = srcs[i].cam.frame_index;
+    s.sampled_pixels = sampled;
+    s.depth_hypotheses = (uint64_t)p.D;
+    s.accepted_points = out.points.size();
+    s.surface_completion_proposals = ss.proposals;
+    s.surface_completed_points = ss.accepted;
+    s.surface_completion_rejected_texture = ss.rej_texture;
+    s.surface_completion_rejected_cross_view = ss.rej_cross;
+    s.surface_completion_rejected_reciprocal = ss.rej_recip;
+    s.surface_completion_rejected_fusion = ss.rej_fusion;
+    s.surface_completion_rejected_footprint = ss.rej_footprint;
+    s.grid_stride = stride;
+    s.grid_border = border;
+    s.reciprocal_checked_points = checked;
+    s.reciprocal_rejected_points = rejected;
+    s.reciprocal_consistent_points = consistent;
+    s.fusion_input_observations = fin;
+    s.fusion_rejected_observations = frej_obs;
+    s.fusion_rejected_points = frej_pts;
+    double m;
+    if (median_option(fusion_counts, &m)) {
+        s.has_median_fusion_observations = 1;
+        s.median_fusion_observations = (float)m;
+    }
+    if (median_option(out.supports, &m)) {
+        s.has_median_supporting_views = 1;
+        s.median_supporting_views = (float)m;
+    }
+    if (median_option(out.errors, &m)) {
+        s.has_median_photometric_error = 1;
+        s.median_photometric_error = (float)m;
+    }
+    s.has_search_depth = 1;
+    s.search_min_depth = (float)sweep_min;
+    s.search_max_depth = (float)sweep_max;
+    r->n_points = out.points.size();
+    r->primary_points = primary;
+    r->points = copy_out(out.points);
+    r->grid_sites = copy_out(out.sites);
+    r->attributes = copy_out(out.attrs);
+    r->n_samples = diags.size();
+    r->samples = copy_out(diags);
+    if (slack_out) *slack_out = copy_out(slacks);
+    return r;
+}
+
+bool valid_params(const vt3d_sweep_params &p) {
+    return p.patch_radius >= 1 && p.patch_radius <= kMaxRadius && p.depth_hypotheses >= 1 &&
+           p.max_source_views >= 1 && p.max_source_views <= 32 &&
+           (p.cost_kind == VT3D_COST_ZSAD || p.cost_kind == VT3D_COST_ZNCC) &&
+           p.grid_stride >= 0 && p.min_fusion_observations >= 1 &&
+           p.surface_completion_passes >= 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+void vt3d_oracle_params_reference(vt3d_sweep_params *p) {
+    std::memset(p, 0, sizeof(*p));
+    p->struct_size = sizeof(*p);
+    p->cost_kind = VT3D_COST_ZSAD;
+    p->patch_radius = 1;
+    p->depth_hypotheses = 24;
+    p->max_source_views = 4;
+    p->grid_stride = 0;
+    p->ambiguity_kind = VT3D_AMBIGUITY_MARGIN;
+    p->min_fusion_observations = 2;
+    p->surface_completion_passes = 2;
+    p->storage = VT3D_STORAGE_F32;
+    p->min_reference_contrast = 14.0;
+    p->max_photometric_error = 18.0;
+    p->ambiguity_threshold = 1.0;
+    p->max_reciprocal_relative_depth_error = 0.08;
+    p->max_fusion_relative_position_error = 0.08;
+    p->error_confidence_scale = 8.0;
+    p->depth_margin_saturation = 6.0;
+}
+
+void vt3d_oracle_params_zncc(vt3d_sweep_params *p, int32_t radius, int32_t hyps, int32_t srcs) {
+    vt3d_oracle_params_reference(p);
+    p->cost_kind = VT3D_COST_ZNCC;
+    p->patch_radius = radius;
+    p->depth_hypotheses = hyps;
+    p->max_source_views = srcs;
+    p->ambiguity_kind = VT3D_AMBIGUITY_RATIO;
+    p->ambiguity_threshold = 0.9;
+    p->max_photometric_error = 0.5;
+    p->error_confidence_scale = 0.25;
+    p->depth_margin_saturation = 0.15;
+}
+
+const char *vt3d_oracle_skip_reason(int32_t code) { return skip_string(code); }
+
+uint32_t vt3d_oracle_grid_stride(uint32_t w, uint32_t h, int32_t fine) {
+    return grid_stride_rule(w, h, fine != 0);
+}
+
+int32_t vt3d_oracle_plane_sweep(const vt3d_frame *frames, size_t n_frames,
+                                const vt3d_camera *cameras, size_t n_cameras,
+                                const double *sparse_xyz, size_t n_sparse, double focal,
+                                int32_t fine_grid, const vt3d_sweep_params *params,
+                                vt3d_poll_fn poll, void *poll_user, int32_t threads,
+                                vt3d_dense_result **out, vt3d_oracle_slack **slack_out) {
+    if (!out) return VT3D_E_INVALID;
+    *out = nullptr;
+    if (slack_out) *slack_out = nullptr;
+    vt3d_sweep_params def;
+    vt3d_oracle_params_reference(&def);
+    const vt3d_sweep_params &pp = params ? *params : def;
+    if (!valid_params(pp)) return VT3D_E_INVALID;
+    if ((n_frames && !frames) || (n_cameras && !cameras) || (n_sparse && !sparse_xyz))
+        return VT3D_E_INVALID;
+    Params p = to_params(pp);
+    Control ctl{poll, poll_user};
+    *out = plane_sweep(frames, n_frames, cameras, n_cameras, sparse_xyz, n_sparse, focal,
+                       fine_grid != 0, p, ctl, threads, slack_out);
+    return *out ? VT3D_OK : VT3D_E_INVALID;
+}
+
+int32_t vt3d_oracle_reference_patch(const vt3d_frame *frames, size_t n_frames,
+                                    const vt3d_camera *cameras, size_t n_cameras,
+                                    const double *sparse_xyz, size_t n_sparse, double focal,
+                                    size_t ref_idx, int32_t fine_grid,
+                                    const vt3d_sweep_params *params, vt3d_poll_fn poll,
+                                    void *poll_user, int32_t threads, vt3d_dense_result **out,
+                                    vt3d_oracle_slack **slack_out) {
+    // reconstruct_reference_patch, dense.rs:674-747.
+    if (!out) return VT3D_E_INVALID;
+    *out = nullptr;
+    if (slack_out) *slack_out = nullptr;
+    if (ref_idx >= n_cameras) return VT3D_NO_PATCH;
+    const vt3d_camera &rcam = cameras[ref_idx];
+    if (rcam.frame_index >= n_frames) return VT3D_NO_PATCH;
+    const vt3d_frame &rframe = frames[rcam.frame_index];
+    Cam rc = to_cam(rcam);
+    std::vector<double> vis;
+    for (size_t i = 0; i < n_sparse; ++i) {
+        V3 q{sparse_xyz[3 * i], sparse_xyz[3 * i + 1], sparse_xyz[3 * i + 2]};
+        double x, y, z;
+        if (project(rc, q, rframe.width, rframe.height, focal, &x, &y, &z) &&
+            in_image_bounds(x, y, rframe.width, rframe.height)) {
+            vis.push_back(q.x);
+            vis.push_back(q.y);
+            vis.push_back(q.z);
+        }
+    }
+    if (vis.size() / 3 < kMinVisibleSparsePoints) return VT3D_NO_PATCH;
+    std::vector<size_t> src_idx;
+    for (size_t i = 0; i < n_cameras; ++i)
+        if (i != ref_idx && cameras[i].frame_index < n_frames) src_idx.push_back(i);
+    auto absdiff = [](uint64_t a, uint64_t b) { return a > b ? a - b : b - a; };
+    std::stable_sort(src_idx.begin(), src_idx.end(), [&](size_t a, size_t b) {
+        uint64_t da = absdiff(cameras[a].frame_index, rcam.frame_index);
+        uint64_t db = absdiff(cameras[b].frame_index, rcam.frame_index);
+        if (da != db) return da < db;
+        return cameras[a].frame_index < cameras[b].frame_index;
+    });
+    std::vector<size_t> order;
+    order.push_back(ref_idx);
+    order.insert(order.end(), src_idx.begin(), src_idx.end());
+    std::vector<vt3d_frame> rf;
+    std::vector<vt3d_camera> rcams;
+    Control ctl{poll, poll_user};
+    for (size_t k = 0; k < order.size(); ++k) {
+        if (ctl.is_canceled()) return VT3D_NO_PATCH;
+        rf.push_back(frames[cameras[order[k]].frame_index]);
+        vt3d_camera c = cameras[order[k]];
+        c.frame_index = k;
+        rcams.push_back(c);
+    }
+    vt3d_sweep_params def;
+    vt3d_oracle_params_reference(&def);
+    const vt3d_sweep_params &pp = params ? *params : def;
+    if (!valid_params(pp)) return VT3D_E_INVALID;
+    Params p = to_params(pp);
+    vt3d_dense_result *r =
+        plane_sweep(rf.data(), rf.size(), rcams.data(), rcams.size(), vis.data(), vis.size() / 3,
+                    focal, fine_grid != 0, p, ctl, threads, slack_out);
+    if (!r) return VT3D_E_INVALID;
+    r->stats.has_reference_frame = 1;
+    r->stats.reference_frame = rcam.frame_index;
+    for (size_t i = 0; i < r->stats.source_views && i < 32; ++i)
+        r->stats.source_frames[i] = cameras[order[r->stats.source_frames[i]]].frame_index;
+    *out = r;
+    return VT3D_OK;
+}
+
+void vt3d_oracle_result_free(vt3d_dense_result *r) {
+    if (!r) return;
+    std::free(r->points);
+    std::free(r->grid_sites);
+    std::free(r->attributes);
+    std::free(r->samples);
+    std::free(r);
+}
+
+void vt3d_oracle_free(void *p) { std::free(p); }
+
+void vt3d_oracle_to_luma(const uint8_t *rgba, size_t n, uint8_t *luma) {
+    vt3d_frame f{(uint32_t)n, 1, rgba};
+    std::vector<uint8_t> l = to_luma(f);
+    std::memcpy(luma, l.data(), n);
+}
+
+double vt3d_oracle_quantile(const double *sorted, size_t n, double q) {
+    return quantile(sorted, n, q);
+}
+
+int32_t vt3d_oracle_project(const vt3d_camera *camera, const double *point, uint32_t w,
+                            uint32_t h, double f, double *o) {
+    Cam c = to_cam(*camera);
+    return project(c, {point[0], point[1], point[2]}, w, h, f, &o[0], &o[1], &o[2]) ? 1 : 0;
+}
+
+void vt3d_oracle_unproject(const vt3d_camera *camera, double x, double y, double depth,
+                           uint32_t w, uint32_t h, double f, double *o) {
+    Cam c = to_cam(*camera);
+    V3 v = unproject(c, x, y, depth, w, h, f);
+    o[0] = v.x;
+    o[1] = v.y;
+    o[2] = v.z;
+}
+
+int32_t vt3d_oracle_sample_bilinear(const uint8_t *l, uint32_t w, uint32_t h, double x, double y,
+                                    double *v) {
+    return sample_bilinear(l, w, h, x, y, v) ? 1 : 0;
+}
+
+double vt3d_oracle_patch_contrast(const uint8_t *l, uint32_t w, uint32_t h, double x, double y,
+                                  int32_t r) {
+    return patch_contrast(l, w, h, x, y, r);
+}
+
+int32_t vt3d_oracle_patch_error(const vt3d_camera *rc, const vt3d_camera *sc,
+                                const uint8_t *rl, const uint8_t *sl, uint32_t rw, uint32_t rh,
+                                uint32_t sw, uint32_t sh, double x, double y, double depth,
+                                double f, int32_t kind, int32_t r, double *cost) {
+    if (r < 1 || r > kMaxRadius) return 0;
+    Cam a = to_cam(*rc), b = to_cam(*sc);
+    return patch_error(a, b, rl, sl, rw, rh, sw, sh, x, y, depth, f, kind, r, cost, nullptr) ? 1
+                                                                                            : 0;
+}
+
+size_t vt3d_oracle_visible_depths(const vt3d_camera *camera, const double *xyz, size_t n,
+                                  uint32_t w, uint32_t h, double f, double *depths) {
+    Cam c = to_cam(*camera);
+    std::vector<V3> sp(n);
+    for (size_t i = 0; i < n; ++i) sp[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    std::vector<double> d = visible_depths(c, sp, w, h, f);
+    std::copy(d.begin(), d.end(), depths);
+    return d.size();
+}
+
+int32_t vt3d_oracle_depth_search_bounds(const vt3d_camera *camera, const double *xyz, size_t n,
+                                        uint32_t w, uint32_t h, double f, double *mn,
+                                        double *mx) {
+    Cam c = to_cam(*camera);
+    std::vector<V3> sp(n);
+    for (size_t i = 0; i < n; ++i) sp[i] = {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]};
+    return depth_search_bounds(c, sp, w, h, f, mn, mx) ? 1 : 0;
+}
+
+int32_t vt3d_oracle_single_view_depth(const vt3d_camera *rc, const vt3d_camera *sc,
+                                      const uint8_t *rl, const uint8_t *sl, uint32_t rw,
+                                      uint32_t rh, uint32_t sw, uint32_t sh, double x, double y,
+                                      double mn, double mx, double f,
+                                      const vt3d_sweep_params *params, double *depth) {
+    vt3d_sweep_params def;
+    vt3d_oracle_params_reference(&def);
+    Params p = to_params(params ? *params : def);
+    Cam a = to_cam(*rc), b = to_cam(*sc);
+    return single_view_depth(a, b, rl, sl, rw, rh, sw, sh, x, y, mn, mx, f, p, depth, nullptr)
+               ? 1
+               : 0;
+}
+
+int32_t vt3d_oracle_reciprocal_agrees(double expected, double estimated, double max_rel) {
+    return reciprocal_agrees(expected, estimated, max_rel, nullptr) ? 1 : 0;
+}
+
+void vt3d_oracle_fuse(const double *primary, const double *recip, size_t n, double ref_depth,
+                      double max_rel, double *pos, uint64_t *obs, uint64_t *rej) {
+    std::vector<V3> r(n);
+    for (size_t i = 0; i < n; ++i) r[i] = {recip[3 * i], recip[3 * i + 1], recip[3 * i + 2]};
+    Fused fu = fuse({primary[0], primary[1], primary[2]}, r, ref_depth, max_rel, nullptr);
+    pos[0] = fu.position.x;
+    pos[1] = fu.position.y;
+    pos[2] = fu.position.z;
+    *obs = fu.observations;
+    *rej = fu.rejected;
+}
+
+int32_t vt3d_oracle_surface_depth_prediction(double *depths, size_t n, double *predicted) {
+    std::vector<double> d(depths, depths + n);
+    bool ok = surface_depth_prediction(d, predicted);
+    std::copy(d.begin(), d.end(), depths);
+    return ok ? 1 : 0;
+}
+
+}  // extern "C"

@@ -1,0 +1,115 @@
+/*
+ * vt3d_oracle.h -- FP64 CPU oracle of the dense plane-sweep depth pass.
+ *
+ * TEST INFRASTRUCTURE ONLY. This library restates
+ *   crates/video-to-3d-core/src/dense_legacy.rs (the sweep, reciprocal check, fusion and
+ *   surface completion) and the grid stride / reference remap of
+ *   crates/video-to-3d-core/src/dense.rs:215-218, 674-747
+ * in double precision, so that tests, the smoke check and the CPU-baseline leg of the
+ * benchmark can check the HIP path against it. Nothing in the product links it.
+ *
+ * It shares the data contract (structs, parameters, result layout) of include/vt3d_dense.h.
+ */
+#ifndef VT3D_ORACLE_H
+#define VT3D_ORACLE_H
+
+#include "../include/vt3d_dense.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Per-sample decision slack: how far each decision that shaped the sample's outcome was
+ * from flipping. cost: cost units (photometric gate, best/second ties, ambiguity,
+ * reciprocal texture gate, reverse-sweep ties). px: source pixels (bilinear bounds of a
+ * patch). rel: relative units (reciprocal agreement, fusion radius). */
+typedef struct vt3d_oracle_slack {
+    double cost;
+    double px;
+    double rel;
+} vt3d_oracle_slack;
+
+/* Same contract as vt3d_dense_plane_sweep. threads > 1 runs grid rows on OpenMP threads
+ * (identical output). slack_out (may be NULL) receives n_samples entries when
+ * params->emit_samples is set; free it with vt3d_oracle_free. */
+int32_t vt3d_oracle_plane_sweep(const vt3d_frame *frames, size_t n_frames,
+                                const vt3d_camera *cameras, size_t n_cameras,
+                                const double *sparse_xyz, size_t n_sparse, double focal,
+                                int32_t fine_grid, const vt3d_sweep_params *params,
+                                vt3d_poll_fn poll, void *poll_user, int32_t threads,
+                                vt3d_dense_result **out, vt3d_oracle_slack **slack_out);
+
+/* Same contract as vt3d_dense_reference_patch. */
+int32_t vt3d_oracle_reference_patch(const vt3d_frame *frames, size_t n_frames,
+                                    const vt3d_camera *cameras, size_t n_cameras,
+                                    const double *sparse_xyz, size_t n_sparse, double focal,
+                                    size_t reference_camera_index, int32_t fine_grid,
+                                    const vt3d_sweep_params *params, vt3d_poll_fn poll,
+                                    void *poll_user, int32_t threads, vt3d_dense_result **out,
+                                    vt3d_oracle_slack **slack_out);
+
+void vt3d_oracle_result_free(vt3d_dense_result *result);
+void vt3d_oracle_free(void *p);
+
+/* ---- single functions, for known-answer tests ---------------------------------------- */
+void vt3d_oracle_params_reference(vt3d_sweep_params *params);
+void vt3d_oracle_params_zncc(vt3d_sweep_params *params, int32_t patch_radius,
+                             int32_t depth_hypotheses, int32_t max_source_views);
+const char *vt3d_oracle_skip_reason(int32_t skip_code);
+uint32_t vt3d_oracle_grid_stride(uint32_t width, uint32_t height, int32_t fine_grid);
+/* to_luma (dense_legacy.rs:1147-1157) over n pixels. */
+void vt3d_oracle_to_luma(const uint8_t *rgba, size_t n_pixels, uint8_t *luma);
+/* quantile over a sorted array (dense_legacy.rs:1168-1174). */
+double vt3d_oracle_quantile(const double *sorted, size_t n, double q);
+/* project (dense_legacy.rs:1013-1029): returns 1 and writes xyz, or 0 for None. */
+int32_t vt3d_oracle_project(const vt3d_camera *camera, const double *point, uint32_t width,
+                            uint32_t height, double focal, double *out_xyz);
+/* unproject (dense_legacy.rs:996-1011). */
+void vt3d_oracle_unproject(const vt3d_camera *camera, double x, double y, double depth,
+                           uint32_t width, uint32_t height, double focal, double *out_xyz);
+/* sample_bilinear (dense_legacy.rs:1113-1127): returns 1 and writes value, or 0. */
+int32_t vt3d_oracle_sample_bilinear(const uint8_t *luma, uint32_t width, uint32_t height,
+                                    double x, double y, double *value);
+/* patch_contrast (dense_legacy.rs:1097-1111) with the given radius. */
+double vt3d_oracle_patch_contrast(const uint8_t *luma, uint32_t width, uint32_t height,
+                                  double x, double y, int32_t radius);
+/* patch_error (dense_legacy.rs:1031-1095) for the given cost kind / radius: 1 + cost or 0. */
+int32_t vt3d_oracle_patch_error(const vt3d_camera *reference_camera,
+                                const vt3d_camera *source_camera, const uint8_t *reference_luma,
+                                const uint8_t *source_luma, uint32_t reference_width,
+                                uint32_t reference_height, uint32_t source_width,
+                                uint32_t source_height, double x, double y, double depth,
+                                double focal, int32_t cost_kind, int32_t radius,
+                                double *cost);
+/* visible_depths (dense_legacy.rs:959-973): writes up to n_sparse depths, returns count. */
+size_t vt3d_oracle_visible_depths(const vt3d_camera *camera, const double *sparse_xyz,
+                                  size_t n_sparse, uint32_t width, uint32_t height,
+                                  double focal, double *depths);
+/* depth_search_bounds (dense_legacy.rs:755-775): 1 + (min, max) or 0. */
+int32_t vt3d_oracle_depth_search_bounds(const vt3d_camera *camera, const double *sparse_xyz,
+                                        size_t n_sparse, uint32_t width, uint32_t height,
+                                        double focal, double *min_depth, double *max_depth);
+/* estimate_single_view_depth (dense_legacy.rs:887-949): 1 + depth or 0. */
+int32_t vt3d_oracle_single_view_depth(const vt3d_camera *reference_camera,
+                                      const vt3d_camera *source_camera,
+                                      const uint8_t *reference_luma, const uint8_t *source_luma,
+                                      uint32_t reference_width, uint32_t reference_height,
+                                      uint32_t source_width, uint32_t source_height, double x,
+                                      double y, double min_depth, double max_depth,
+                                      double focal, const vt3d_sweep_params *params,
+                                      double *depth);
+/* reciprocal_depth_agrees (dense_legacy.rs:951-957) with the given threshold. */
+int32_t vt3d_oracle_reciprocal_agrees(double expected, double estimated, double max_rel);
+/* fuse_depth_observations (dense_legacy.rs:859-884). reciprocal: n x 3. */
+void vt3d_oracle_fuse(const double *primary, const double *reciprocal, size_t n,
+                      double reference_depth, double max_rel, double *out_position,
+                      uint64_t *observations, uint64_t *rejected);
+/* surface_depth_prediction (dense_legacy.rs:739-753): 1 + depth or 0. depths is sorted
+ * in place. */
+int32_t vt3d_oracle_surface_depth_prediction(double *depths, size_t n, double *predicted);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* VT3D_ORACLE_H */

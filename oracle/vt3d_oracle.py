@@ -1,0 +1,186 @@
+"""ctypes binding of the FP64 CPU oracle (libvt3d_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the CPU-baseline
+leg of bench.py, as the checker of the HIP path. The product never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+import sys
+from typing import Callable, Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_REPO = os.path.dirname(_HERE)
+sys.path.insert(0, os.path.join(_REPO, "video-to-3d_amd"))
+
+from vt3d_dense import abi  # noqa: E402  (shared data contract)
+
+LIBRARY_PATH = os.path.join(_HERE, "libvt3d_oracle.so")
+SLACK_DTYPE = np.dtype([("cost", "<f8"), ("px", "<f8"), ("rel", "<f8")])
+
+_lib = None
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def library() -> C.CDLL:
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY_PATH):
+        build()
+    lib = C.CDLL(LIBRARY_PATH)
+    P = C.POINTER
+    lib.vt3d_oracle_params_reference.argtypes = [P(abi.SweepParams)]
+    lib.vt3d_oracle_params_zncc.argtypes = [P(abi.SweepParams), C.c_int32, C.c_int32, C.c_int32]
+    lib.vt3d_oracle_skip_reason.restype = C.c_char_p
+    lib.vt3d_oracle_skip_reason.argtypes = [C.c_int32]
+    lib.vt3d_oracle_grid_stride.restype = C.c_uint32
+    lib.vt3d_oracle_grid_stride.argtypes = [C.c_uint32, C.c_uint32, C.c_int32]
+    common = [P(abi.Frame), C.c_size_t, P(abi.Camera), C.c_size_t, P(C.c_double), C.c_size_t,
+              C.c_double]
+    lib.vt3d_oracle_plane_sweep.argtypes = common + [C.c_int32, P(abi.SweepParams), abi.POLL_FN,
+                                                     C.c_void_p, C.c_int32,
+                                                     P(P(abi.DenseResult)), P(C.c_void_p)]
+    lib.vt3d_oracle_plane_sweep.restype = C.c_int32
+    lib.vt3d_oracle_reference_patch.argtypes = common + [C.c_size_t, C.c_int32,
+                                                         P(abi.SweepParams), abi.POLL_FN,
+                                                         C.c_void_p, C.c_int32,
+                                                         P(P(abi.DenseResult)), P(C.c_void_p)]
+    lib.vt3d_oracle_reference_patch.restype = C.c_int32
+    lib.vt3d_oracle_result_free.argtypes = [P(abi.DenseResult)]
+    lib.vt3d_oracle_free.argtypes = [C.c_void_p]
+    lib.vt3d_oracle_to_luma.argtypes = [P(C.c_uint8), C.c_size_t, P(C.c_uint8)]
+    lib.vt3d_oracle_quantile.argtypes = [P(C.c_double), C.c_size_t, C.c_double]
+    lib.vt3d_oracle_quantile.restype = C.c_double
+    lib.vt3d_oracle_project.argtypes = [P(abi.Camera), P(C.c_double), C.c_uint32, C.c_uint32,
+                                        C.c_double, P(C.c_double)]
+    lib.vt3d_oracle_project.restype = C.c_int32
+    lib.vt3d_oracle_unproject.argtypes = [P(abi.Camera), C.c_double, C.c_double, C.c_double,
+                                          C.c_uint32, C.c_uint32, C.c_double, P(C.c_double)]
+    lib.vt3d_oracle_sample_bilinear.argtypes = [P(C.c_uint8), C.c_uint32, C.c_uint32, C.c_double,
+                                                C.c_double, P(C.c_double)]
+    lib.vt3d_oracle_sample_bilinear.restype = C.c_int32
+    lib.vt3d_oracle_patch_contrast.argtypes = [P(C.c_uint8), C.c_uint32, C.c_uint32, C.c_double,
+                                               C.c_double, C.c_int32]
+    lib.vt3d_oracle_patch_contrast.restype = C.c_double
+    lib.vt3d_oracle_patch_error.argtypes = [P(abi.Camera), P(abi.Camera), P(C.c_uint8),
+                                            P(C.c_uint8), C.c_uint32, C.c_uint32, C.c_uint32,
+                                            C.c_uint32, C.c_double, C.c_double, C.c_double,
+                                            C.c_double, C.c_int32, C.c_int32, P(C.c_double)]
+    lib.vt3d_oracle_patch_error.restype = C.c_int32
+    lib.vt3d_oracle_visible_depths.argtypes = [P(abi.Camera), P(C.c_double), C.c_size_t,
+                                               C.c_uint32, C.c_uint32, C.c_double, P(C.c_double)]
+    lib.vt3d_oracle_visible_depths.restype = C.c_size_t
+    lib.vt3d_oracle_depth_search_bounds.argtypes = [P(abi.Camera), P(C.c_double), C.c_size_t,
+                                                    C.c_uint32, C.c_uint32, C.c_double,
+                                                    P(C.c_double), P(C.c_double)]
+    lib.vt3d_oracle_depth_search_bounds.restype = C.c_int32
+    lib.vt3d_oracle_single_view_depth.argtypes = [P(abi.Camera), P(abi.Camera), P(C.c_uint8),
+                                                  P(C.c_uint8), C.c_uint32, C.c_uint32,
+                                                  C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                                  C.c_double, C.c_double, C.c_double,
+                                                  P(abi.SweepParams), P(C.c_double)]
+    lib.vt3d_oracle_single_view_depth.restype = C.c_int32
+    lib.vt3d_oracle_reciprocal_agrees.argtypes = [C.c_double, C.c_double, C.c_double]
+    lib.vt3d_oracle_reciprocal_agrees.restype = C.c_int32
+    lib.vt3d_oracle_fuse.argtypes = [P(C.c_double), P(C.c_double), C.c_size_t, C.c_double,
+                                     C.c_double, P(C.c_double), P(C.c_uint64), P(C.c_uint64)]
+    lib.vt3d_oracle_surface_depth_prediction.argtypes = [P(C.c_double), C.c_size_t,
+                                                         P(C.c_double)]
+    lib.vt3d_oracle_surface_depth_prediction.restype = C.c_int32
+    _lib = lib
+    return lib
+
+
+def params(settings: Optional[dict] = None, **overrides) -> abi.SweepParams:
+    lib = library()
+    return abi.make_params(lib.vt3d_oracle_params_reference, lib.vt3d_oracle_params_zncc,
+                           settings, **overrides)
+
+
+def _rgba(frames):
+    from vt3d_dense.scenes import to_rgba
+    return [f if f.ndim == 3 else to_rgba(f) for f in frames]
+
+
+def _poll(control):
+    if control is None:
+        return abi.POLL_FN(0)
+    return abi.POLL_FN(lambda _user: 1 if control() else 0)
+
+
+def _take(out, slack) -> Tuple[abi.DenseAnalysis, Optional[np.ndarray]]:
+    lib = library()
+    res = abi.unpack_result(out.contents)
+    sl = None
+    if slack:
+        n = int(out.contents.n_samples)
+        sl = np.frombuffer(C.string_at(slack, n * SLACK_DTYPE.itemsize), SLACK_DTYPE).copy()
+        lib.vt3d_oracle_free(slack)
+    lib.vt3d_oracle_result_free(out)
+    return res, sl
+
+
+def plane_sweep(frames, cameras, sparse_points, focal: float, fine_grid: bool = False,
+                control: Optional[Callable[[], bool]] = None,
+                sweep: Optional[abi.SweepParams] = None, threads: int = 1):
+    """Oracle of estimate_depth_points_with_control; returns (DenseAnalysis, slack)."""
+    lib = library()
+    inp = abi.Inputs(_rgba(frames), cameras, sparse_points)
+    out = C.POINTER(abi.DenseResult)()
+    slack = C.c_void_p()
+    rc = lib.vt3d_oracle_plane_sweep(inp.frames, inp.n_frames, inp.cameras, inp.n_cameras,
+                                     inp.sparse_ptr, inp.n_sparse, float(focal), int(fine_grid),
+                                     C.byref(sweep) if sweep is not None else None,
+                                     _poll(control), None, int(threads), C.byref(out),
+                                     C.byref(slack))
+    if rc < 0:
+        raise ValueError(f"oracle rejected the input ({rc})")
+    return _take(out, slack.value)
+
+
+def reference_patch(frames, cameras, sparse_points, focal: float, reference_camera_index: int,
+                    fine_grid: bool = False, control: Optional[Callable[[], bool]] = None,
+                    sweep: Optional[abi.SweepParams] = None, threads: int = 1):
+    """Oracle of reconstruct_reference_patch; (None, None) where the reference returns None."""
+    lib = library()
+    inp = abi.Inputs(_rgba(frames), cameras, sparse_points)
+    out = C.POINTER(abi.DenseResult)()
+    slack = C.c_void_p()
+    rc = lib.vt3d_oracle_reference_patch(inp.frames, inp.n_frames, inp.cameras, inp.n_cameras,
+                                         inp.sparse_ptr, inp.n_sparse, float(focal),
+                                         int(reference_camera_index), int(fine_grid),
+                                         C.byref(sweep) if sweep is not None else None,
+                                         _poll(control), None, int(threads), C.byref(out),
+                                         C.byref(slack))
+    if rc == abi.VT3D_NO_PATCH:
+        return None, None
+    if rc < 0:
+        raise ValueError(f"oracle rejected the input ({rc})")
+    return _take(out, slack.value)
+
+
+def camera_struct(R, t, frame_index: int = 0) -> abi.Camera:
+    c = abi.Camera()
+    c.frame_index = frame_index
+    Rc = np.asarray(R, dtype=np.float64).reshape(3, 3).T.reshape(-1)
+    for k in range(9):
+        c.rotation_colmajor[k] = float(Rc[k])
+    for k in range(3):
+        c.translation[k] = float(t[k])
+    return c
+
+
+def u8ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_uint8))
+
+
+def f64ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))

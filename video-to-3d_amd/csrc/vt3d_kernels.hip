@@ -1,0 +1,877 @@
+// vt3d_kernels.hip -- gfx950 kernels of the dense plane-sweep depth pass.
+//
+// Pipeline of one solve (every kernel is launched on the solve's stream; no atomics, no
+// host round trips until the final totals):
+//   k_build_view    RGBA8 -> u8 luma + quad texels              (to_luma, dense_legacy.rs:1147)
+//   k_gate          grid site -> texture-gate flag              (dense_legacy.rs:272-282)
+//   k_scan          per-block counts -> offsets and totals
+//   k_compact_sites flags -> row-major sample list
+//   k_sweep         (sample, hypothesis block) -> top-2         (dense_legacy.rs:284-346)
+//   k_decide        merge top-2, cost + ambiguity gates         (dense_legacy.rs:347-362)
+//   k_recip         (sample, source) reciprocal observation     (dense_legacy.rs:777-857)
+//   k_fuse          fusion, confidence, counters                (dense_legacy.rs:363-412)
+//   k_compact_points accepted samples -> output arrays in reference order
+//
+// Geometry: a plane at depth d in the reference maps reference pixel (u, v, 1) to source
+// homogeneous coordinates by a 3x3 homography composed in FP64 on the host; it equals the
+// unproject/project pair of patch_error (dense_legacy.rs:1052-1062). Each patch centre is
+// warped in FP64; taps are warped in FP32 relative to the centre, which keeps the tap
+// position error near 1e-6 px at any image size.
+#include "vt3d_internal.h"
+
+#include "../../include/vt3d_dense.h"
+
+namespace vt3d {
+
+constexpr float kInfF = __builtin_huge_valf();
+
+// ---- texel storage ----------------------------------------------------------------------
+
+template <int ST> struct Store;
+
+template <> struct Store<VT3D_STORAGE_F32> {
+    static constexpr size_t kBytes = 16;
+    __device__ static __forceinline__ float4 load(const void *q, uint32_t i) {
+        return reinterpret_cast<const float4 *>(q)[i];
+    }
+    __device__ static __forceinline__ void put(void *q, uint32_t i, float a, float b, float c,
+                                               float d) {
+        reinterpret_cast<float4 *>(q)[i] = make_float4(a, b, c, d);
+    }
+};
+
+__device__ __forceinline__ float half_lo(uint32_t u) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(u & 0xffffu));
+}
+__device__ __forceinline__ float half_hi(uint32_t u) {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(u >> 16));
+}
+__device__ __forceinline__ uint32_t half_bits(float v) {
+    return (uint32_t)__builtin_bit_cast(uint16_t, (_Float16)v);
+}
+
+template <> struct Store<VT3D_STORAGE_F16> {
+    static constexpr size_t kBytes = 8;
+    __device__ static __forceinline__ float4 load(const void *q, uint32_t i) {
+        uint2 u = reinterpret_cast<const uint2 *>(q)[i];
+        return make_float4(half_lo(u.x), half_hi(u.x), half_lo(u.y), half_hi(u.y));
+    }
+    __device__ static __forceinline__ void put(void *q, uint32_t i, float a, float b, float c,
+                                               float d) {
+        reinterpret_cast<uint2 *>(q)[i] =
+            make_uint2(half_bits(a) | (half_bits(b) << 16), half_bits(c) | (half_bits(d) << 16));
+    }
+};
+
+// Luma is an integer 0..255, so BF16 (8 significant bits) stores it exactly; its bits are
+// the top half of the FP32 pattern.
+template <> struct Store<VT3D_STORAGE_BF16> {
+    static constexpr size_t kBytes = 8;
+    __device__ static __forceinline__ float4 load(const void *q, uint32_t i) {
+        uint2 u = reinterpret_cast<const uint2 *>(q)[i];
+        return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
+                           __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
+    }
+    __device__ static __forceinline__ void put(void *q, uint32_t i, float a, float b, float c,
+                                               float d) {
+        auto bits = [](float v) { return __float_as_uint(v) >> 16; };
+        reinterpret_cast<uint2 *>(q)[i] =
+            make_uint2(bits(a) | (bits(b) << 16), bits(c) | (bits(d) << 16));
+    }
+};
+
+__device__ __forceinline__ float bilerp(float4 q, float tx, float ty) {
+    const float top = fmaf(tx, q.y - q.x, q.x);
+    const float bottom = fmaf(tx, q.w - q.z, q.z);
+    return fmaf(ty, bottom - top, top);
+}
+
+// ---- k_build_view -------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t luma_of(const uint8_t *p) {
+    // to_luma, dense_legacy.rs:1154: (77 R + 150 G + 29 B) >> 8 in u16.
+    return (77u * p[0] + 150u * p[1] + 29u * p[2]) >> 8;
+}
+
+template <int ST>
+__global__ __launch_bounds__(kBlock) void k_build_view(const uint8_t *__restrict__ rgba,
+                                                       uint32_t w, uint32_t h,
+                                                       uint8_t *__restrict__ luma,
+                                                       void *__restrict__ quad) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= w * h) return;
+    const uint32_t x = i % w, y = i / w;
+    const uint32_t x1 = min(x + 1, w - 1), y1 = min(y + 1, h - 1);
+    const uint32_t a = luma_of(rgba + 4 * ((size_t)y * w + x));
+    const uint32_t b = luma_of(rgba + 4 * ((size_t)y * w + x1));
+    const uint32_t c = luma_of(rgba + 4 * ((size_t)y1 * w + x));
+    const uint32_t d = luma_of(rgba + 4 * ((size_t)y1 * w + x1));
+    luma[i] = (uint8_t)a;
+    Store<ST>::put(quad, i, (float)a, (float)b, (float)c, (float)d);
+}
+
+// ---- patch statistics and cost ------------------------------------------------------------
+
+struct RefStats {
+    float sum_hat;  // ZNCC: sum of normalised deviations (0 up to rounding)
+    bool flat;      // ZNCC: no variance, every cost is 1
+};
+
+// Prepares reference values in place: ZNCC -> (r - mean) / ||r - mean||; ZSAD -> r - mean.
+template <int COST, int T>
+__device__ __forceinline__ RefStats ref_stats(float (&rp)[T]) {
+    float s = 0.f;
+#pragma unroll
+    for (int k = 0; k < T; ++k) s += rp[k];
+    const float mean = s / (float)T;
+    RefStats st{0.f, false};
+    if constexpr (COST == VT3D_COST_ZSAD) {
+#pragma unroll
+        for (int k = 0; k < T; ++k) rp[k] -= mean;
+    } else {
+        float ss = 0.f;
+#pragma unroll
+        for (int k = 0; k < T; ++k) {
+            rp[k] -= mean;
+            ss = fmaf(rp[k], rp[k], ss);
+        }
+        st.flat = !(ss > kZnccMinVariance * (float)T);
+        const float inv = st.flat ? 0.f : rsqrtf(ss);
+        float sh = 0.f;
+#pragma unroll
+        for (int k = 0; k < T; ++k) {
+            rp[k] *= inv;
+            sh += rp[k];
+        }
+        st.sum_hat = sh;
+    }
+    return st;
+}
+
+// Bilinear patch of a u8 plane centred at a fractional point (sample_bilinear,
+// dense_legacy.rs:1113-1127, at x + dx, y + dy). Integer offsets share one weight pair.
+// Returns false if any tap leaves [0, w-1) x [0, h-1).
+template <int R>
+__device__ __forceinline__ bool sample_patch(const uint8_t *__restrict__ l, uint32_t w,
+                                             uint32_t h, double cx, double cy,
+                                             float (&out)[(2 * R + 1) * (2 * R + 1)]) {
+    constexpr int P = 2 * R + 1;
+    if (!(cx >= -1.0e8 && cx <= 1.0e8 && cy >= -1.0e8 && cy <= 1.0e8)) return false;
+    const double fx = floor(cx), fy = floor(cy);
+    const int ix = (int)fx, iy = (int)fy;
+    const float tx = (float)(cx - fx), ty = (float)(cy - fy);
+    const bool ok = ix - R >= 0 && iy - R >= 0 && ix + R <= (int)w - 2 && iy + R <= (int)h - 2;
+    if (!ok) return false;
+#pragma unroll
+    for (int dy = -R; dy <= R; ++dy) {
+        const uint8_t *row0 = l + (size_t)(iy + dy) * w;
+        const uint8_t *row1 = row0 + w;
+#pragma unroll
+        for (int dx = -R; dx <= R; ++dx) {
+            const int xx = ix + dx;
+            const float4 q =
+                make_float4((float)row0[xx], (float)row0[xx + 1], (float)row1[xx], (float)row1[xx + 1]);
+            out[(dy + R) * P + (dx + R)] = bilerp(q, tx, ty);
+        }
+    }
+    return true;
+}
+
+// Centre warp of one patch: FP64 homography at the centre, FP32 tap offsets from it.
+struct Warp {
+    int ixc, iyc;
+    float fxc, fyc;
+    float g, e;    // projective terms of the tap denominator 1 + g dx + e dy
+    float ax, bx;  // x offset numerator ax dx + bx dy
+    float ay, by;  // y offset numerator
+    float dc;      // source depth of the centre
+    bool ok;
+};
+
+__device__ __forceinline__ Warp make_warp(const double *__restrict__ H, double cx, double cy) {
+    const double nc = fma(H[0], cx, fma(H[1], cy, H[2]));
+    const double mc = fma(H[3], cx, fma(H[4], cy, H[5]));
+    const double dc = fma(H[6], cx, fma(H[7], cy, H[8]));
+    Warp w;
+    // The centre is a tap: project() rejects it unless its source depth exceeds 1e-4.
+    w.ok = dc > 1.0e-4;
+    const double rd = 1.0 / (w.ok ? dc : 1.0);
+    const double xc = nc * rd, yc = mc * rd;
+    w.ok = w.ok && fabs(xc) < 1.0e8 && fabs(yc) < 1.0e8;
+    const double fx = w.ok ? floor(xc) : 0.0, fy = w.ok ? floor(yc) : 0.0;
+    w.ixc = (int)fx;
+    w.iyc = (int)fy;
+    w.fxc = (float)(xc - fx);
+    w.fyc = (float)(yc - fy);
+    w.g = (float)(H[6] * rd);
+    w.e = (float)(H[7] * rd);
+    w.ax = (float)((H[0] - xc * H[6]) * rd);
+    w.bx = (float)((H[1] - xc * H[7]) * rd);
+    w.ay = (float)((H[3] - yc * H[6]) * rd);
+    w.by = (float)((H[4] - yc * H[7]) * rd);
+    w.dc = (float)dc;
+    return w;
+}
+
+// patch_error (dense_legacy.rs:1031-1095) for one (sample, hypothesis, source): every tap of
+// the reference patch is warped into the source texture and sampled bilinearly; any tap
+// outside [0, w-1) x [0, h-1) or with source depth <= 1e-4 makes the source not count
+// (+inf). ZSAD: mean |(r - r_mean) - (s - s_mean)|; ZNCC: 1 - ZNCC.
+template <int R, int COST, int ST>
+__device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
+                                            const RefStats &st, const double *__restrict__ H,
+                                            double cx, double cy, const void *__restrict__ tex,
+                                            uint32_t sw, uint32_t sh) {
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    const Warp w = make_warp(H, cx, cy);
+    bool ok = w.ok;
+    const uint32_t wm2 = sw - 2u, hm2 = sh - 2u, nlast = sw * sh - 1u;
+    float s1 = 0.f, s2 = 0.f, srs = 0.f, pivot = 0.f;
+    float sv[COST == VT3D_COST_ZSAD ? T : 1];
+#pragma unroll
+    for (int dy = -R; dy <= R; ++dy) {
+        const float fdy = (float)dy;
+        const float row_den = fmaf(w.e, fdy, 1.f);
+        const float row_x = w.bx * fdy;
+        const float row_y = w.by * fdy;
+#pragma unroll
+        for (int dx = -R; dx <= R; ++dx) {
+            const int k = (dy + R) * P + (dx + R);
+            const float fdx = (float)dx;
+            const float den = fmaf(w.g, fdx, row_den);
+            const float rinv = __builtin_amdgcn_rcpf(den);
+            const float qx = fmaf(fmaf(w.ax, fdx, row_x), rinv, w.fxc);
+            const float qy = fmaf(fmaf(w.ay, fdx, row_y), rinv, w.fyc);
+            const float flx = floorf(qx), fly = floorf(qy);
+            const int ix = w.ixc + (int)flx, iy = w.iyc + (int)fly;
+            ok = ok && ((uint32_t)ix <= wm2) && ((uint32_t)iy <= hm2) && (w.dc * den > 1.0e-4f);
+            const uint32_t idx = min((uint32_t)iy * sw + (uint32_t)ix, nlast);
+            const float v = bilerp(Store<ST>::load(tex, idx), qx - flx, qy - fly);
+            if constexpr (COST == VT3D_COST_ZSAD) {
+                sv[k] = v;
+                s1 += v;
+            } else {
+                if (k == 0) pivot = v;
+                const float d = v - pivot;
+                s1 += d;
+                s2 = fmaf(d, d, s2);
+                srs = fmaf(rp[k], d, srs);
+            }
+        }
+    }
+    if (!ok) return kInfF;
+    if constexpr (COST == VT3D_COST_ZSAD) {
+        const float sm = s1 / (float)T;
+        float acc = 0.f;
+#pragma unroll
+        for (int k = 0; k < T; ++k) acc += fabsf(rp[k] - (sv[k] - sm));
+        return acc / (float)T;
+    } else {
+        const float m = s1 / (float)T;
+        const float var = fmaf(-s1, m, s2);
+        if (st.flat || !(var > kZnccMinVariance * (float)T)) return 1.f;
+        const float cov = fmaf(-m, st.sum_hat, srs);
+        return 1.f - cov * rsqrtf(var);
+    }
+}
+
+// ---- candidate order -----------------------------------------------------------------------
+
+__device__ __forceinline__ bool better(int sa, float ca, int ha, int sb, float cb, int hb) {
+    return sa > sb || (sa == sb && (ca < cb || (ca == cb && ha < hb)));
+}
+
+__device__ __forceinline__ void top2_insert(Top2 &t, int s, float c, int h) {
+    if (s <= 0) return;
+    if (better(s, c, h, t.s1, t.c1, t.h1)) {
+        t.h2 = t.h1;
+        t.s2 = t.s1;
+        t.c2 = t.c1;
+        t.h1 = h;
+        t.s1 = s;
+        t.c1 = c;
+    } else if (better(s, c, h, t.s2, t.c2, t.h2)) {
+        t.h2 = h;
+        t.s2 = s;
+        t.c2 = c;
+    }
+}
+
+// ---- k_gate / k_scan / compaction ----------------------------------------------------------
+
+__device__ __forceinline__ uint64_t lanemask_lt() {
+    return (1ull << __lane_id()) - 1ull;
+}
+
+// Per-block category counts: each thread contributes c[k]; the block's totals land in
+// block_counts[blockIdx.x][k]. Integer sums, so the result does not depend on order.
+__device__ __forceinline__ void block_count(const uint32_t (&c)[kCounters], uint32_t *dst) {
+    __shared__ uint32_t red[kBlock / 64][kCounters];
+    const int wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < kCounters; ++k) {
+        uint32_t v = c[k];
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+        if (__lane_id() == 0) red[wave][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < kCounters) {
+        uint32_t s = 0;
+        for (int wv = 0; wv < kBlock / 64; ++wv) s += red[wv][threadIdx.x];
+        dst[blockIdx.x * kCounters + threadIdx.x] = s;
+    }
+}
+
+// Rank of this thread among the block's threads with flag set (thread order), and the
+// block-wide prefix base read from block_offsets.
+__device__ __forceinline__ uint32_t block_rank(bool flag) {
+    __shared__ uint32_t wave_tot[kBlock / 64];
+    const uint64_t m = __ballot(flag);
+    const int wave = threadIdx.x >> 6;
+    if (__lane_id() == 0) wave_tot[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    uint32_t base = 0;
+    for (int wv = 0; wv < wave; ++wv) base += wave_tot[wv];
+    return base + (uint32_t)__popcll(m & lanemask_lt());
+}
+
+__global__ __launch_bounds__(kBlock) void k_gate(Args a) {
+    const SolveDev &sd = *a.solve;
+    const uint32_t nsites = sd.grid_nx * sd.grid_ny;
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t c[kCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (j < nsites) {
+        const uint32_t x = sd.border + (j % sd.grid_nx) * sd.stride;
+        const uint32_t y = sd.border + (j / sd.grid_nx) * sd.stride;
+        // patch_contrast at integer taps (dense_legacy.rs:1097-1111): bilinear at an integer
+        // coordinate returns the pixel exactly. The grid border r + 2 keeps taps in bounds.
+        const int R = sd.radius;
+        const uint8_t *l = sd.ref.luma;
+        const uint32_t w = sd.ref.w;
+        uint32_t mn = 255, mx = 0;
+        for (int dy = -R; dy <= R; ++dy)
+            for (int dx = -R; dx <= R; ++dx) {
+                const uint32_t v = l[(size_t)(y + dy) * w + (x + dx)];
+                mn = min(mn, v);
+                mx = max(mx, v);
+            }
+        const bool pass = (float)(mx - mn) >= sd.min_contrast;
+        a.site_flag[j] = pass ? 1 : 0;
+        c[0] = pass ? 1 : 0;
+    }
+    block_count(c, a.block_counts);
+}
+
+// Exclusive scan of block_counts over blocks, per category; one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_scan(Args a, uint32_t *n_out) {
+    __shared__ uint32_t wave_sum[16];
+    __shared__ uint32_t carry_s;
+    const uint32_t nb = a.n_blocks;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    for (int k = 0; k < kCounters; ++k) {
+        if (threadIdx.x == 0) carry_s = 0;
+        __syncthreads();
+        for (uint32_t base = 0; base < nb; base += 1024) {
+            const uint32_t b = base + threadIdx.x;
+            const uint32_t v = b < nb ? a.block_counts[b * kCounters + k] : 0;
+            uint32_t incl = v;
+#pragma unroll
+            for (int off = 1; off < 64; off <<= 1) {
+                const uint32_t o = __shfl_up(incl, off, 64);
+                if (lane >= off) incl += o;
+            }
+            if (lane == 63) wave_sum[wave] = incl;
+            __syncthreads();
+            uint32_t wbase = 0;
+            for (int wv = 0; wv < wave; ++wv) wbase += wave_sum[wv];
+            const uint32_t carry = carry_s;
+            if (b < nb) a.block_offsets[b * kCounters + k] = carry + wbase + incl - v;
+            __syncthreads();
+            if (threadIdx.x == 1023) carry_s = carry + wbase + incl;
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) {
+            a.totals[k] = carry_s;
+            if (k == 0 && n_out) *n_out = carry_s;
+        }
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_sites(Args a) {
+    const SolveDev &sd = *a.solve;
+    const uint32_t nsites = sd.grid_nx * sd.grid_ny;
+    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    const bool flag = j < nsites && a.site_flag[j] != 0;
+    const uint32_t r = block_rank(flag);
+    if (flag) {
+        const uint32_t x = sd.border + (j % sd.grid_nx) * sd.stride;
+        const uint32_t y = sd.border + (j / sd.grid_nx) * sd.stride;
+        a.samples[a.block_offsets[blockIdx.x * kCounters] + r] = x | (y << 16);
+    }
+}
+
+// ---- k_sweep -------------------------------------------------------------------------------
+
+// Inserts c into an ascending register list (min stays, max moves on): a sorting network
+// with static indices, so the list never spills.
+template <int N>
+__device__ __forceinline__ void sorted_insert(float (&l)[N], float c) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const float lo = fminf(l[k], c);
+        c = fmaxf(l[k], c);
+        l[k] = lo;
+    }
+}
+
+// One lane per texture-gated sample; blockIdx.y selects a block of hypotheses. For each
+// hypothesis: the cost against every source, the photometric gate, support = number of
+// sources that pass, error = quantile(sorted errors, 0.5) (index n >> 1, dense_legacy.rs:
+// 1168-1174), then the running top-2 in candidate order.
+template <int R, int COST, int ST, int SMAX>
+__global__ __launch_bounds__(kBlock) void k_sweep(Args a) {
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    const SolveDev &sd = *a.solve;
+    const uint32_t n = *a.n_samples;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t pk = a.samples[i];
+    const int x = (int)(pk & 0xffffu), y = (int)(pk >> 16);
+    float rp[T];
+    {
+        const uint8_t *l = sd.ref.luma;
+        const uint32_t w = sd.ref.w;
+#pragma unroll
+        for (int dy = -R; dy <= R; ++dy)
+#pragma unroll
+            for (int dx = -R; dx <= R; ++dx)
+                rp[(dy + R) * P + (dx + R)] = (float)l[(size_t)(y + dy) * w + (x + dx)];
+    }
+    const RefStats st = ref_stats<COST, T>(rp);
+    const int S = sd.S, D = sd.D;
+    const int h0 = (int)blockIdx.y * a.hblock;
+    const int h1 = min(h0 + a.hblock, D);
+    const float maxc = sd.max_cost;
+    const double cx = (double)x, cy = (double)y;
+    Top2 t{-1, 0, kInfF, -1, 0, kInfF};
+    for (int hyp = h0; hyp < h1; ++hyp) {
+        float l[SMAX];
+#pragma unroll
+        for (int k = 0; k < SMAX; ++k) l[k] = kInfF;
+        for (int s = 0; s < S; ++s) {
+            const SourceDev &src = a.src[s];
+            const double *H = a.Hp + ((size_t)s * D + hyp) * 9;
+            float c = patch_cost<R, COST, ST>(rp, st, H, cx, cy, src.tex.quad, src.tex.w,
+                                              src.tex.h);
+            c = c <= maxc ? c : kInfF;
+            sorted_insert<SMAX>(l, c);
+        }
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < SMAX; ++k) cnt += l[k] < kInfF ? 1 : 0;
+        if (cnt > 0) {
+            const int mid = cnt >> 1;
+            float med = l[0];
+#pragma unroll
+            for (int k = 1; k < SMAX; ++k) med = k == mid ? l[k] : med;
+            top2_insert(t, cnt, med, hyp);
+        }
+    }
+    a.partial[(size_t)blockIdx.y * a.max_samples + i] = t;
+}
+
+// ---- k_decide ------------------------------------------------------------------------------
+
+__device__ __forceinline__ void unproject_d(const double *R, const double *t, double x, double y,
+                                            double depth, uint32_t w, uint32_t h, double f,
+                                            double *o) {
+    // unproject, dense_legacy.rs:996-1011: R^T (camera_point - t).
+    const double c0 = (x - w * 0.5) / f * depth - t[0];
+    const double c1 = (y - h * 0.5) / f * depth - t[1];
+    const double c2 = depth - t[2];
+    o[0] = (R[0] * c0 + R[3] * c1) + R[6] * c2;
+    o[1] = (R[1] * c0 + R[4] * c1) + R[7] * c2;
+    o[2] = (R[2] * c0 + R[5] * c1) + R[8] * c2;
+}
+
+__device__ __forceinline__ bool project_d(const double *R, const double *t, const double *p,
+                                          uint32_t w, uint32_t h, double f, double *x, double *y,
+                                          double *z) {
+    // project, dense_legacy.rs:1013-1029.
+    const double c0 = ((R[0] * p[0] + R[1] * p[1]) + R[2] * p[2]) + t[0];
+    const double c1 = ((R[3] * p[0] + R[4] * p[1]) + R[5] * p[2]) + t[1];
+    const double c2 = ((R[6] * p[0] + R[7] * p[1]) + R[8] * p[2]) + t[2];
+    if (!isfinite(c0) || !isfinite(c1) || !isfinite(c2) || c2 <= 1.0e-4) return false;
+    *x = f * c0 / c2 + w * 0.5;
+    *y = f * c1 / c2 + h * 0.5;
+    *z = c2;
+    return true;
+}
+
+__global__ __launch_bounds__(kBlock) void k_decide(Args a) {
+    const SolveDev &sd = *a.solve;
+    const uint32_t n = *a.n_samples;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= n) return;
+    Top2 t{-1, 0, kInfF, -1, 0, kInfF};
+    for (int b = 0; b < a.n_hblocks; ++b) {
+        const Top2 p = a.partial[(size_t)b * a.max_samples + i];
+        top2_insert(t, p.s1, p.c1, p.h1);
+        top2_insert(t, p.s2, p.c2, p.h2);
+    }
+    Decision d{};
+    d.best_h = t.h1;
+    d.best_support = t.s1;
+    d.best_cost = t.c1;
+    d.second_h = -1;
+    d.second_cost = kInfF;
+    d.margin = __builtin_huge_val();
+    if (t.s1 <= 0) {
+        d.status = VT3D_SAMPLE_NO_CANDIDATE;
+    } else {
+        const bool has_second = t.s2 == t.s1;
+        if (has_second) {
+            d.second_h = t.h2;
+            d.second_cost = t.c2;
+            d.margin = (double)t.c2 - (double)t.c1;
+        }
+        if (t.c1 > sd.max_cost) {
+            d.status = VT3D_SAMPLE_REJECT_COST;
+        } else {
+            bool ambiguous;
+            if (sd.ambiguity_kind == VT3D_AMBIGUITY_MARGIN)
+                ambiguous = d.margin < sd.ambiguity;
+            else
+                ambiguous = has_second && (double)t.c1 > sd.ambiguity * (double)t.c2;
+            d.status = ambiguous ? VT3D_SAMPLE_REJECT_AMBIGUITY : 0;
+        }
+    }
+    if (d.status == 0) {
+        const uint32_t pk = a.samples[i];
+        const double x = (double)(pk & 0xffffu), y = (double)(pk >> 16);
+        d.depth = a.depth_p[d.best_h];
+        unproject_d(sd.Rr, sd.tr, x, y, d.depth, sd.ref.w, sd.ref.h, sd.focal, d.pos);
+        d.proj_ok = project_d(sd.Rr, sd.tr, d.pos, sd.ref.w, sd.ref.h, sd.focal, &d.xr, &d.yr,
+                              &d.dr)
+                        ? 1
+                        : 0;
+    }
+    a.decision[i] = d;
+}
+
+// ---- k_recip -------------------------------------------------------------------------------
+
+// One lane per (sample, source); blockIdx.y is the source, so every wave shares the source
+// and its reverse homographies. reciprocal_depth_observations (dense_legacy.rs:795-856):
+// direct cost at the re-projected reference point, projection into the source, the texture
+// gate there, a reverse sweep over the source's landmark envelope against the reference
+// image (estimate_single_view_depth, dense_legacy.rs:887-949), the 8% agreement test and the
+// unprojection of the source pixel at the recovered depth.
+template <int R, int COST, int ST>
+__global__ __launch_bounds__(kBlock) void k_recip(Args a) {
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    const SolveDev &sd = *a.solve;
+    const uint32_t n = *a.n_samples;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const int s = (int)blockIdx.y;
+    if (i >= n) return;
+    Recip out{{0.0, 0.0, 0.0}, 0, 0};
+    const Decision &d = a.decision[i];
+    const SourceDev &src = a.src[s];
+    const int D = sd.D;
+    const float maxc = sd.max_cost;
+    bool ok = d.status == 0 && d.proj_ok;
+    float rp[T];
+    if (ok) ok = sample_patch<R>(sd.ref.luma, sd.ref.w, sd.ref.h, d.xr, d.yr, rp);
+    if (ok) {
+        const RefStats st = ref_stats<COST, T>(rp);
+        double H[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) H[k] = fma(d.dr, src.A[k], src.B[k]);
+        const float c = patch_cost<R, COST, ST>(rp, st, H, d.xr, d.yr, src.tex.quad, src.tex.w,
+                                                src.tex.h);
+        ok = c <= maxc;
+    }
+    double xs = 0.0, ys = 0.0, zs = 0.0;
+    if (ok) ok = project_d(src.R, src.t, d.pos, src.tex.w, src.tex.h, sd.focal, &xs, &ys, &zs);
+    if (ok) ok = sample_patch<R>(src.tex.luma, src.tex.w, src.tex.h, xs, ys, rp);
+    if (ok) {
+        float mn = rp[0], mx = rp[0];
+#pragma unroll
+        for (int k = 1; k < T; ++k) {
+            mn = fminf(mn, rp[k]);
+            mx = fmaxf(mx, rp[k]);
+        }
+        ok = (mx - mn) >= sd.min_contrast;
+    }
+    if (ok) {
+        const RefStats st = ref_stats<COST, T>(rp);
+        float best_c = kInfF;
+        int best_h = -1;
+        const double *Hs = a.Hr + (size_t)s * D * 9;
+        for (int hyp = 0; hyp < D; ++hyp) {
+            const float c = patch_cost<R, COST, ST>(rp, st, Hs + (size_t)hyp * 9, xs, ys,
+                                                    sd.ref.quad, sd.ref.w, sd.ref.h);
+            // cost ascending, then depth (hypothesis) ascending: strict < keeps the first
+            if (c <= maxc && c < best_c) {
+                best_c = c;
+                best_h = hyp;
+            }
+        }
+        ok = best_h >= 0;
+        if (ok) {
+            const double dh = a.depth_r[(size_t)s * D + best_h];
+            // reciprocal_depth_agrees, dense_legacy.rs:951-957.
+            ok = isfinite(zs) && isfinite(dh) && zs > 0.0 && dh > 0.0 &&
+                 fabs(zs - dh) / fmax(fmax(zs, dh), 1.0e-9) <= sd.recip_rel;
+            if (ok) {
+                unproject_d(src.R, src.t, xs, ys, dh, src.tex.w, src.tex.h, sd.focal, out.pos);
+                ok = isfinite(out.pos[0]) && isfinite(out.pos[1]) && isfinite(out.pos[2]);
+            }
+        }
+    }
+    out.ok = ok ? 1 : 0;
+    a.recip[(size_t)s * a.max_samples + i] = out;
+}
+
+// ---- k_fuse --------------------------------------------------------------------------------
+
+// fuse_depth_observations (dense_legacy.rs:859-884) and the confidence and attributes of
+// dense_legacy.rs:387-410, in FP64; counters for DenseStats (dense_legacy.rs:363-385).
+__global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag) {
+    const SolveDev &sd = *a.solve;
+    const uint32_t n = *a.n_samples;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    uint32_t c[kCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < n) {
+        const Decision d = a.decision[i];
+        SampleOut o{};
+        o.status = d.status;
+        o.error = d.best_cost;
+        o.support = d.best_support;
+        if (d.status == 0) {
+            const double radius = fmax(fabs(d.depth) * sd.fusion_rel, 1.0e-4);
+            double sx = d.pos[0], sy = d.pos[1], sz = d.pos[2];
+            uint32_t obs = 1, rej = 0, cnt = 0, mask = 0;
+            for (int s = 0; s < sd.S; ++s) {
+                const Recip r = a.recip[(size_t)s * a.max_samples + i];
+                if (!r.ok) continue;
+                ++cnt;
+                mask |= 1u << s;
+                const double ex = r.pos[0] - d.pos[0], ey = r.pos[1] - d.pos[1],
+                             ez = r.pos[2] - d.pos[2];
+                const double dist = sqrt(ex * ex + ey * ey + ez * ez);
+                if (!(dist <= radius)) {
+                    ++rej;
+                    continue;
+                }
+                sx += r.pos[0];
+                sy += r.pos[1];
+                sz += r.pos[2];
+                ++obs;
+            }
+            o.recip_mask = mask;
+            o.recip_count = cnt;
+            if (cnt == 0) {
+                o.status = VT3D_SAMPLE_REJECT_RECIPROCAL;
+            } else {
+                o.fused_obs = obs;
+                o.fused_rejected = rej;
+                if (obs < (uint32_t)sd.min_obs) {
+                    o.status = VT3D_SAMPLE_REJECT_FUSION;
+                } else {
+                    o.status = VT3D_SAMPLE_ACCEPTED;
+                    const double S = (double)sd.S;
+                    const double support_conf = (double)d.best_support / S;
+                    const double recip_conf = fmin(fmax((double)(obs - 1) / S, 0.0), 1.0);
+                    const double err_conf = 1.0 / (1.0 + (double)d.best_cost / sd.err_scale);
+                    const double dm =
+                        isfinite(d.margin) ? fmin(fmax(d.margin / sd.margin_sat, 0.0), 1.0) : 1.0;
+                    const double conf =
+                        fmin(fmax(support_conf * recip_conf * err_conf * fmax(dm, 0.2), 0.05), 1.0);
+                    o.conf = (float)conf;
+                    o.recip_conf = (float)recip_conf;
+                    o.depth_margin = (float)dm;
+                    o.px = (float)(sx / (double)obs);
+                    o.py = (float)(sy / (double)obs);
+                    o.pz = (float)(sz / (double)obs);
+                }
+            }
+        }
+        a.out[i] = o;
+        const int st = o.status;
+        c[0] = st == VT3D_SAMPLE_ACCEPTED;
+        c[1] = st >= VT3D_SAMPLE_REJECT_RECIPROCAL;
+        c[2] = st == VT3D_SAMPLE_REJECT_RECIPROCAL;
+        c[3] = st >= VT3D_SAMPLE_REJECT_FUSION;
+        c[4] = st >= VT3D_SAMPLE_REJECT_FUSION ? 1 + o.recip_count : 0;
+        c[5] = st >= VT3D_SAMPLE_REJECT_FUSION ? o.fused_rejected : 0;
+        c[6] = st == VT3D_SAMPLE_REJECT_FUSION;
+        if (diag) {
+            const uint32_t pk = a.samples[i];
+            vt3d_sample_diag g;
+            g.x = pk & 0xffffu;
+            g.y = pk >> 16;
+            g.status = st;
+            g.best_hypothesis = d.best_h;
+            g.best_support = d.best_support;
+            g.second_hypothesis = d.second_h;
+            g.best_cost = d.best_cost;
+            g.second_cost = d.second_cost;
+            g.reciprocal_mask = o.recip_mask;
+            g.fused_observations = o.fused_obs;
+            diag[i] = g;
+        }
+    }
+    block_count(c, a.block_counts);
+}
+
+__global__ __launch_bounds__(kBlock) void k_compact_points(Args a, vt3d_point3 *points,
+                                                           vt3d_grid_site *sites,
+                                                           vt3d_point_attributes *attrs,
+                                                           float *errors, float *supports,
+                                                           float *fused_obs) {
+    const SolveDev &sd = *a.solve;
+    const uint32_t n = *a.n_samples;
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const bool flag = i < n && a.out[i].status == VT3D_SAMPLE_ACCEPTED;
+    const uint32_t r = block_rank(flag);
+    if (!flag) return;
+    const uint32_t dst = a.block_offsets[blockIdx.x * kCounters] + r;
+    const SampleOut o = a.out[i];
+    const uint32_t pk = a.samples[i];
+    const uint32_t x = pk & 0xffffu, y = pk >> 16;
+    const uint8_t *px = a.ref_rgba + ((size_t)y * a.ref_rgba_w + x) * 4;
+    vt3d_point3 p;
+    p.x = o.px;
+    p.y = o.py;
+    p.z = o.pz;
+    p.confidence = o.conf;
+    p.r = px[0];
+    p.g = px[1];
+    p.b = px[2];
+    p.pad = 0;
+    points[dst] = p;
+    sites[dst] = vt3d_grid_site{x, y};
+    attrs[dst] = vt3d_point_attributes{o.recip_conf, o.depth_margin};
+    errors[dst] = o.error;
+    supports[dst] = (float)o.support;
+    fused_obs[dst] = (float)o.fused_obs;
+    (void)sd;
+}
+
+// ---- launchers ------------------------------------------------------------------------------
+
+template <int ST>
+void launch_build_view_t(const uint8_t *rgba, uint32_t w, uint32_t h, uint8_t *luma, void *quad,
+                         hipStream_t s) {
+    const uint32_t n = w * h;
+    hipLaunchKernelGGL(k_build_view<ST>, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       rgba, w, h, luma, quad);
+}
+
+template <int R, int COST, int ST>
+void launch_sweep_t(const Args &a, int smax, uint32_t blocks, hipStream_t s) {
+    dim3 grid(blocks, a.n_hblocks);
+    if (smax <= 4)
+        hipLaunchKernelGGL((k_sweep<R, COST, ST, 4>), grid, dim3(kBlock), 0, s, a);
+    else if (smax <= 8)
+        hipLaunchKernelGGL((k_sweep<R, COST, ST, 8>), grid, dim3(kBlock), 0, s, a);
+    else if (smax <= 16)
+        hipLaunchKernelGGL((k_sweep<R, COST, ST, 16>), grid, dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_sweep<R, COST, ST, 32>), grid, dim3(kBlock), 0, s, a);
+}
+
+template <int R, int COST, int ST>
+void launch_recip_t(const Args &a, int S, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL((k_recip<R, COST, ST>), dim3(blocks, S), dim3(kBlock), 0, s, a);
+}
+
+template <int COST, int ST>
+bool dispatch_r(int radius, const Args &a, int smax, int S, uint32_t blocks, bool recip,
+                hipStream_t s) {
+#define VT3D_CASE(RR)                                                           \
+    case RR:                                                                    \
+        if (recip) launch_recip_t<RR, COST, ST>(a, S, blocks, s);               \
+        else launch_sweep_t<RR, COST, ST>(a, smax, blocks, s);                  \
+        return true;
+    if constexpr (COST == VT3D_COST_ZSAD) {
+        switch (radius) {
+            VT3D_CASE(1)
+            VT3D_CASE(2)
+            VT3D_CASE(3)
+        default: return false;
+        }
+    } else {
+        switch (radius) {
+            VT3D_CASE(1)
+            VT3D_CASE(2)
+            VT3D_CASE(3)
+            VT3D_CASE(4)
+            VT3D_CASE(5)
+        default: return false;
+        }
+    }
+#undef VT3D_CASE
+}
+
+template <int ST>
+bool dispatch_cost(int cost, int radius, const Args &a, int smax, int S, uint32_t blocks,
+                   bool recip, hipStream_t s) {
+    if (cost == VT3D_COST_ZSAD) return dispatch_r<VT3D_COST_ZSAD, ST>(radius, a, smax, S, blocks, recip, s);
+    return dispatch_r<VT3D_COST_ZNCC, ST>(radius, a, smax, S, blocks, recip, s);
+}
+
+bool launch_sweep_or_recip(int storage, int cost, int radius, const Args &a, int smax, int S,
+                           uint32_t blocks, bool recip, hipStream_t s) {
+    switch (storage) {
+    case VT3D_STORAGE_F32: return dispatch_cost<VT3D_STORAGE_F32>(cost, radius, a, smax, S, blocks, recip, s);
+    case VT3D_STORAGE_F16: return dispatch_cost<VT3D_STORAGE_F16>(cost, radius, a, smax, S, blocks, recip, s);
+    case VT3D_STORAGE_BF16: return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, S, blocks, recip, s);
+    default: return false;
+    }
+}
+
+bool launch_build_view(int storage, const uint8_t *rgba, uint32_t w, uint32_t h, uint8_t *luma,
+                       void *quad, hipStream_t s) {
+    switch (storage) {
+    case VT3D_STORAGE_F32: launch_build_view_t<VT3D_STORAGE_F32>(rgba, w, h, luma, quad, s); return true;
+    case VT3D_STORAGE_F16: launch_build_view_t<VT3D_STORAGE_F16>(rgba, w, h, luma, quad, s); return true;
+    case VT3D_STORAGE_BF16: launch_build_view_t<VT3D_STORAGE_BF16>(rgba, w, h, luma, quad, s); return true;
+    default: return false;
+    }
+}
+
+size_t texel_bytes(int storage) {
+    return storage == VT3D_STORAGE_F32 ? 16 : 8;
+}
+
+void launch_gate(const Args &a, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_gate, dim3(blocks), dim3(kBlock), 0, s, a);
+}
+void launch_scan(const Args &a, uint32_t *n_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a, n_out);
+}
+void launch_compact_sites(const Args &a, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_sites, dim3(blocks), dim3(kBlock), 0, s, a);
+}
+void launch_decide(const Args &a, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_decide, dim3(blocks), dim3(kBlock), 0, s, a);
+}
+void launch_fuse(const Args &a, vt3d_sample_diag *diag, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_fuse, dim3(blocks), dim3(kBlock), 0, s, a, diag);
+}
+void launch_compact_points(const Args &a, vt3d_point3 *points, vt3d_grid_site *sites,
+                           vt3d_point_attributes *attrs, float *errors, float *supports,
+                           float *fused_obs, uint32_t blocks, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_points, dim3(blocks), dim3(kBlock), 0, s, a, points, sites, attrs,
+                       errors, supports, fused_obs);
+}
+
+}  // namespace vt3d

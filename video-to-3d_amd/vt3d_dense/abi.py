@@ -1,0 +1,244 @@
+"""ctypes mirror of include/vt3d_dense.h (the C ABI of the dense plane sweep).
+
+The structs here are byte-for-byte the header's; `bind` declares the argument and return
+types of every exported function on a loaded library. The oracle library shares these
+structs (it implements the same data contract), so the helpers that marshal inputs and
+unpack results work for both.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import dataclasses
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+VT3D_OK = 0
+VT3D_NO_PATCH = 1
+VT3D_E_INVALID = -1
+VT3D_E_DEVICE = -2
+VT3D_E_NO_DEVICE = -3
+
+COST_ZSAD, COST_ZNCC = 0, 1
+AMBIGUITY_MARGIN, AMBIGUITY_RATIO = 0, 1
+STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+
+SAMPLE_NO_CANDIDATE = 1
+SAMPLE_REJECT_COST = 2
+SAMPLE_REJECT_AMBIGUITY = 3
+SAMPLE_REJECT_RECIPROCAL = 4
+SAMPLE_REJECT_FUSION = 5
+SAMPLE_ACCEPTED = 6
+
+
+class Frame(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.POINTER(C.c_uint8))]
+
+
+class Camera(C.Structure):
+    _fields_ = [("frame_index", C.c_uint64), ("rotation_colmajor", C.c_double * 9),
+                ("translation", C.c_double * 3)]
+
+
+class SweepParams(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("cost_kind", C.c_int32),
+                ("patch_radius", C.c_int32), ("depth_hypotheses", C.c_int32),
+                ("max_source_views", C.c_int32), ("grid_stride", C.c_int32),
+                ("ambiguity_kind", C.c_int32), ("min_fusion_observations", C.c_int32),
+                ("surface_completion_passes", C.c_int32), ("storage", C.c_int32),
+                ("emit_samples", C.c_int32), ("reserved0", C.c_int32),
+                ("min_reference_contrast", C.c_double), ("max_photometric_error", C.c_double),
+                ("ambiguity_threshold", C.c_double),
+                ("max_reciprocal_relative_depth_error", C.c_double),
+                ("max_fusion_relative_position_error", C.c_double),
+                ("error_confidence_scale", C.c_double), ("depth_margin_saturation", C.c_double),
+                ("depth_min_override", C.c_double), ("depth_max_override", C.c_double)]
+
+
+class Point3(C.Structure):
+    _fields_ = [("x", C.c_float), ("y", C.c_float), ("z", C.c_float), ("confidence", C.c_float),
+                ("r", C.c_uint8), ("g", C.c_uint8), ("b", C.c_uint8), ("pad", C.c_uint8)]
+
+
+class GridSite(C.Structure):
+    _fields_ = [("x", C.c_uint32), ("y", C.c_uint32)]
+
+
+class PointAttributes(C.Structure):
+    _fields_ = [("reciprocal_consistency", C.c_float), ("depth_margin", C.c_float)]
+
+
+class DenseStats(C.Structure):
+    _fields_ = [("attempted", C.c_int32), ("has_reference_frame", C.c_int32),
+                ("reference_frame", C.c_uint64), ("source_views", C.c_uint64),
+                ("source_frames", C.c_uint64 * 32), ("sampled_pixels", C.c_uint64),
+                ("depth_hypotheses", C.c_uint64), ("accepted_points", C.c_uint64),
+                ("surface_completion_proposals", C.c_uint64),
+                ("surface_completed_points", C.c_uint64),
+                ("surface_completion_rejected_texture", C.c_uint64),
+                ("surface_completion_rejected_cross_view", C.c_uint64),
+                ("surface_completion_rejected_reciprocal", C.c_uint64),
+                ("surface_completion_rejected_fusion", C.c_uint64),
+                ("surface_completion_rejected_footprint", C.c_uint64),
+                ("grid_stride", C.c_uint64), ("grid_border", C.c_uint32), ("pad0", C.c_uint32),
+                ("reciprocal_checked_points", C.c_uint64),
+                ("reciprocal_rejected_points", C.c_uint64),
+                ("reciprocal_consistent_points", C.c_uint64),
+                ("fusion_input_observations", C.c_uint64),
+                ("fusion_rejected_observations", C.c_uint64),
+                ("fusion_rejected_points", C.c_uint64),
+                ("has_median_fusion_observations", C.c_int32),
+                ("median_fusion_observations", C.c_float),
+                ("has_median_supporting_views", C.c_int32),
+                ("median_supporting_views", C.c_float),
+                ("has_median_photometric_error", C.c_int32),
+                ("median_photometric_error", C.c_float),
+                ("has_search_depth", C.c_int32), ("search_min_depth", C.c_float),
+                ("search_max_depth", C.c_float), ("pad1", C.c_int32)]
+
+
+class SampleDiag(C.Structure):
+    _fields_ = [("x", C.c_uint32), ("y", C.c_uint32), ("status", C.c_int32),
+                ("best_hypothesis", C.c_int32), ("best_support", C.c_int32),
+                ("second_hypothesis", C.c_int32), ("best_cost", C.c_float),
+                ("second_cost", C.c_float), ("reciprocal_mask", C.c_uint32),
+                ("fused_observations", C.c_uint32)]
+
+
+class DenseResult(C.Structure):
+    _fields_ = [("stats", DenseStats), ("skip_code", C.c_int32), ("skip_reason", C.c_char * 256),
+                ("n_points", C.c_size_t), ("primary_points", C.c_size_t),
+                ("points", C.POINTER(Point3)), ("grid_sites", C.POINTER(GridSite)),
+                ("attributes", C.POINTER(PointAttributes)), ("n_samples", C.c_size_t),
+                ("samples", C.POINTER(SampleDiag))]
+
+
+class Timing(C.Structure):
+    _fields_ = [("total_ms", C.c_double), ("upload_ms", C.c_double), ("gate_ms", C.c_double),
+                ("sweep_ms", C.c_double), ("reciprocal_ms", C.c_double), ("fuse_ms", C.c_double),
+                ("completion_ms", C.c_double), ("download_ms", C.c_double),
+                ("units", C.c_uint64)]
+
+
+POLL_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p)
+
+POINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("confidence", "<f4"),
+                        ("r", "u1"), ("g", "u1"), ("b", "u1"), ("pad", "u1")])
+SITE_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4")])
+ATTR_DTYPE = np.dtype([("reciprocal_consistency", "<f4"), ("depth_margin", "<f4")])
+SAMPLE_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("status", "<i4"), ("best_hypothesis", "<i4"),
+                         ("best_support", "<i4"), ("second_hypothesis", "<i4"),
+                         ("best_cost", "<f4"), ("second_cost", "<f4"),
+                         ("reciprocal_mask", "<u4"), ("fused_observations", "<u4")])
+
+assert C.sizeof(Point3) == POINT_DTYPE.itemsize
+assert C.sizeof(SampleDiag) == SAMPLE_DTYPE.itemsize
+
+
+@dataclasses.dataclass
+class DenseAnalysis:
+    """DenseAnalysis (crates/video-to-3d-core/src/dense_legacy.rs:61-68) as numpy arrays."""
+    stats: Dict[str, object]
+    points: np.ndarray        # POINT_DTYPE
+    grid_sites: np.ndarray    # SITE_DTYPE
+    point_attributes: np.ndarray  # ATTR_DTYPE
+    primary_points: int
+    samples: Optional[np.ndarray]  # SAMPLE_DTYPE, texture-gated samples in row-major order
+    skip_code: int
+
+
+def _array(ptr, n: int, dtype: np.dtype) -> np.ndarray:
+    if n == 0 or not ptr:
+        return np.zeros(0, dtype)
+    buf = C.string_at(ptr, n * dtype.itemsize)
+    return np.frombuffer(buf, dtype=dtype).copy()
+
+
+def unpack_result(res: DenseResult) -> DenseAnalysis:
+    st = res.stats
+    stats: Dict[str, object] = {}
+    for name, _ in DenseStats._fields_:
+        if name.startswith("pad") or name.startswith("has_"):
+            continue
+        stats[name] = getattr(st, name)
+    stats["attempted"] = bool(st.attempted)
+    stats["source_frames"] = [int(st.source_frames[i]) for i in range(min(int(st.source_views), 32))]
+    stats["reference_frame"] = int(st.reference_frame) if st.has_reference_frame else None
+    for opt in ("median_fusion_observations", "median_supporting_views", "median_photometric_error"):
+        stats[opt] = float(getattr(st, opt)) if getattr(st, "has_" + opt) else None
+    if st.has_search_depth:
+        stats["search_min_depth"] = float(st.search_min_depth)
+        stats["search_max_depth"] = float(st.search_max_depth)
+    else:
+        stats["search_min_depth"] = stats["search_max_depth"] = None
+    reason = res.skip_reason.decode()
+    stats["skip_reason"] = reason if res.skip_code else None
+    n = int(res.n_points)
+    samples = _array(res.samples, int(res.n_samples), SAMPLE_DTYPE) if res.samples else None
+    return DenseAnalysis(stats, _array(res.points, n, POINT_DTYPE),
+                         _array(res.grid_sites, n, SITE_DTYPE),
+                         _array(res.attributes, n, ATTR_DTYPE), int(res.primary_points), samples,
+                         int(res.skip_code))
+
+
+class Inputs:
+    """Frames, cameras and landmarks marshalled for one call (keeps the buffers alive)."""
+
+    def __init__(self, frames_rgba: Sequence[np.ndarray], cameras: Sequence, sparse: np.ndarray):
+        self._rgba = [np.ascontiguousarray(f, dtype=np.uint8) for f in frames_rgba]
+        self.frames = (Frame * max(1, len(self._rgba)))()
+        for i, f in enumerate(self._rgba):
+            h, w = f.shape[:2]
+            self.frames[i] = Frame(w, h, f.ctypes.data_as(C.POINTER(C.c_uint8)))
+        self.n_frames = len(self._rgba)
+        self.cameras = (Camera * max(1, len(cameras)))()
+        for i, cam in enumerate(cameras):
+            if isinstance(cam, tuple):
+                idx, R, t = cam
+            else:
+                idx, R, t = i, cam.R, cam.t
+            self.cameras[i].frame_index = int(idx)
+            Rc = np.asarray(R, dtype=np.float64).reshape(3, 3).T.reshape(-1)  # column-major
+            for k in range(9):
+                self.cameras[i].rotation_colmajor[k] = float(Rc[k])
+            for k in range(3):
+                self.cameras[i].translation[k] = float(np.asarray(t, dtype=np.float64)[k])
+        self.n_cameras = len(cameras)
+        self.sparse = np.ascontiguousarray(np.asarray(sparse, dtype=np.float64).reshape(-1, 3))
+        self.sparse_ptr = self.sparse.ctypes.data_as(C.POINTER(C.c_double))
+        self.n_sparse = self.sparse.shape[0]
+
+
+def make_params(lib_reference, lib_zncc, settings: Optional[dict] = None, **overrides) -> SweepParams:
+    """vt3d_sweep_params from a settings dict ({"preset": "reference"|"zncc", ...})."""
+    s = dict(settings or {})
+    s.update(overrides)
+    p = SweepParams()
+    if s.get("preset", "reference") == "zncc":
+        lib_zncc(C.byref(p), int(s.get("patch_radius", 3)), int(s.get("depth_hypotheses", 64)),
+                 int(s.get("max_source_views", 4)))
+    else:
+        lib_reference(C.byref(p))
+    mapping = {"patch_radius": "patch_radius", "depth_hypotheses": "depth_hypotheses",
+               "max_source_views": "max_source_views", "grid_stride": "grid_stride",
+               "surface_completion_passes": "surface_completion_passes",
+               "emit_samples": "emit_samples", "min_fusion_observations": "min_fusion_observations",
+               "depth_min": "depth_min_override", "depth_max": "depth_max_override",
+               "max_photometric_error": "max_photometric_error",
+               "ambiguity_threshold": "ambiguity_threshold",
+               "min_reference_contrast": "min_reference_contrast"}
+    for key, field in mapping.items():
+        if key in s:
+            setattr(p, field, type(getattr(p, field))(s[key]))
+    if "storage" in s:
+        p.storage = STORAGE[s["storage"]] if isinstance(s["storage"], str) else int(s["storage"])
+    if "cost_kind" in s:
+        p.cost_kind = int(s["cost_kind"])
+    return p
+
+
+def bind_common(lib, prefix: str) -> None:
+    """Declares the entry points shared by the product and the oracle libraries."""
+    pfx = prefix
+    getattr(lib, f"{pfx}_result_free").argtypes = [C.POINTER(DenseResult)]
+    getattr(lib, f"{pfx}_result_free").restype = None
