@@ -225,6 +225,12 @@ const char *vt3d_skip_reason(int32_t skip_code);
 /* grid_stride (dense.rs:215-218). */
 uint32_t vt3d_grid_stride(uint32_t width, uint32_t height, int32_t fine_grid);
 int32_t vt3d_abi_version(void);
+/* The plane-induced homography the kernels use: maps pixel (u, v, 1) of camera a (image
+ * wa x ha) on the fronto-parallel plane at depth `depth` to homogeneous pixel coordinates of
+ * camera b, i.e. unproject (dense_legacy.rs:996-1011) followed by project
+ * (dense_legacy.rs:1013-1029); row-major H[9]. Host-only, no device needed. */
+void vt3d_plane_homography(const vt3d_camera *a, uint32_t wa, uint32_t ha, const vt3d_camera *b,
+                           uint32_t wb, uint32_t hb, double focal, double depth, double *H);
 
 /* ---- one-shot entry points (host buffers in, host result out) ------------------------ */
 

@@ -22,6 +22,10 @@
 
 namespace {
 
+// Row band of the grid that plane_sweep evaluates (count 0: every row).
+int g_band_first = 0;
+int g_band_count = 0;
+
 constexpr double kInf = std::numeric_limits<double>::infinity();
 constexpr size_t kMinVisibleSparsePoints = 8;        // dense_legacy.rs:8
 constexpr size_t kMinSurfaceNeighbors = 3;           // dense_legacy.rs:17
@@ -904,8 +908,14 @@ vt3d_dense_result *plane_sweep(const vt3d_frame *frames, size_t n_frames,
 
     std::vector<uint32_t> rows;
     for (uint32_t y = border; y < h - border; y += (uint32_t)stride) rows.push_back(y);
+    // Optional band of grid rows (bounded CPU-baseline samples of a large workload).
+    std::vector<char> in_band(rows.size(), 1);
+    if (g_band_count > 0)
+        for (size_t ri = 0; ri < rows.size(); ++ri)
+            in_band[ri] = ri >= (size_t)g_band_first && ri < (size_t)g_band_first + g_band_count;
     std::vector<std::vector<SampleOut>> row_out(rows.size());
     auto run_row = [&](size_t ri) {
+        if (!in_band[ri]) return;
         uint32_t y = rows[ri];
         for (uint32_t x = border; x < w - border; x += (uint32_t)stride) {
             SampleOut o;
@@ -1163,6 +1173,11 @@ int32_t vt3d_oracle_reference_patch(const vt3d_frame *frames, size_t n_frames,
         r->stats.source_frames[i] = cameras[order[r->stats.source_frames[i]]].frame_index;
     *out = r;
     return VT3D_OK;
+}
+
+void vt3d_oracle_set_row_band(int32_t first, int32_t count) {
+    g_band_first = first < 0 ? 0 : first;
+    g_band_count = count < 0 ? 0 : count;
 }
 
 void vt3d_oracle_result_free(vt3d_dense_result *r) {

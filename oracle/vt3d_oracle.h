@@ -48,6 +48,10 @@ int32_t vt3d_oracle_reference_patch(const vt3d_frame *frames, size_t n_frames,
                                     void *poll_user, int32_t threads, vt3d_dense_result **out,
                                     vt3d_oracle_slack **slack_out);
 
+/* Restricts plane_sweep to grid rows [first, first + count) (count 0: all rows), for
+ * bounded timing samples. Process-wide; not thread-safe. */
+void vt3d_oracle_set_row_band(int32_t first, int32_t count);
+
 void vt3d_oracle_result_free(vt3d_dense_result *result);
 void vt3d_oracle_free(void *p);
 

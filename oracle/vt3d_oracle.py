@@ -54,6 +54,8 @@ def library() -> C.CDLL:
                                                          C.c_void_p, C.c_int32,
                                                          P(P(abi.DenseResult)), P(C.c_void_p)]
     lib.vt3d_oracle_reference_patch.restype = C.c_int32
+    lib.vt3d_oracle_set_row_band.argtypes = [C.c_int32, C.c_int32]
+    lib.vt3d_oracle_set_row_band.restype = None
     lib.vt3d_oracle_result_free.argtypes = [P(abi.DenseResult)]
     lib.vt3d_oracle_free.argtypes = [C.c_void_p]
     lib.vt3d_oracle_to_luma.argtypes = [P(C.c_uint8), C.c_size_t, P(C.c_uint8)]
@@ -184,3 +186,8 @@ def u8ptr(a: np.ndarray):
 
 def f64ptr(a: np.ndarray):
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def set_row_band(first: int = 0, count: int = 0) -> None:
+    """Restricts the oracle sweep to grid rows [first, first + count); count 0 = all rows."""
+    library().vt3d_oracle_set_row_band(int(first), int(count))

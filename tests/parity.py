@@ -1,0 +1,75 @@
+"""Comparison of a GPU solve with the FP64 oracle on the same inputs.
+
+A texture-gated sample must reach the same outcome (status) and choose the same hypothesis
+index on both sides, unless the oracle's decision slack shows that the sample sat within
+FP32 rounding of a decision: a photometric gate, a best/second tie, the ambiguity gate, the
+reciprocal texture gate or reverse-sweep tie (cost units, `tau_cost`), a patch tap on the
+bilinear-bounds edge (`tau_px` source pixels), or the reciprocal-agreement / fusion-radius
+tests (`tau_rel`). Fused points of samples that agree on hypothesis and reciprocal set must
+match within 1e-4 relative error.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+# Tolerances. ZSAD costs are intensity differences (0..255): FP32 bilinear and sums carry
+# about 1e-5 absolute error. ZNCC costs are in [0, 2] with about 1e-6 error.
+TAU_COST = {0: 1.0e-3, 1: 1.0e-4}
+TAU_PX = 1.0e-3
+TAU_REL = 1.0e-5
+POINT_REL = 1.0e-4
+
+
+def compare(gpu, oracle, slack, cost_kind: int, max_unexplained: int = 0):
+    """Returns a report dict; raises AssertionError on a parity violation."""
+    gs, os_ = gpu.samples, oracle.samples
+    assert gs is not None and os_ is not None, "both sides must emit samples"
+    assert gpu.stats["sampled_pixels"] == oracle.stats["sampled_pixels"]
+    assert len(gs) == len(os_)
+    assert np.array_equal(gs["x"], os_["x"]) and np.array_equal(gs["y"], os_["y"])
+    agree = (gs["status"] == os_["status"]) & (gs["best_hypothesis"] == os_["best_hypothesis"])
+    near = ((slack["cost"] < TAU_COST[cost_kind]) | (slack["px"] < TAU_PX)
+            | (slack["rel"] < TAU_REL))
+    unexplained = ~agree & ~near
+    report = {"samples": int(len(gs)), "agree": float(agree.mean()) if len(gs) else 1.0,
+              "near_ties": int(near.sum()), "disagree": int((~agree).sum()),
+              "unexplained": int(unexplained.sum())}
+    assert report["unexplained"] <= max_unexplained, (report, _describe(gs, os_, slack,
+                                                                        np.nonzero(unexplained)[0]))
+    # Points: accepted on both sides with the same hypothesis and reciprocal set.
+    same = (agree & (gs["status"] == 6) & (gs["reciprocal_mask"] == os_["reciprocal_mask"]))
+    gp = _points_by_site(gpu)
+    op = _points_by_site(oracle)
+    worst = 0.0
+    for i in np.nonzero(same)[0]:
+        key = (int(gs["x"][i]), int(gs["y"][i]))
+        a, b = gp[key], op[key]
+        va = np.array([a["x"], a["y"], a["z"]], dtype=np.float64)
+        vb = np.array([b["x"], b["y"], b["z"]], dtype=np.float64)
+        rel = np.linalg.norm(va - vb) / max(np.linalg.norm(vb), 1e-12)
+        worst = max(worst, rel)
+    report["point_rel_max"] = worst
+    assert worst <= POINT_REL, report
+    return report
+
+
+def _points_by_site(res):
+    return {(int(s["x"]), int(s["y"])): p for s, p in zip(res.grid_sites, res.points)}
+
+
+def _describe(gs, os_, slack, idx):
+    rows = []
+    for i in idx[:6]:
+        rows.append(f"i={i} ({gs['x'][i]},{gs['y'][i]}) gpu st={gs['status'][i]} "
+                    f"h={gs['best_hypothesis'][i]} c={gs['best_cost'][i]:.6g} | oracle "
+                    f"st={os_['status'][i]} h={os_['best_hypothesis'][i]} "
+                    f"c={os_['best_cost'][i]:.6g} slack={tuple(slack[i])}")
+    return "\n".join(rows)
+
+
+def stats_equal(a: dict, b: dict, keys=None):
+    keys = keys or ["attempted", "reference_frame", "source_views", "source_frames",
+                    "sampled_pixels", "depth_hypotheses", "grid_stride", "grid_border",
+                    "skip_reason"]
+    for k in keys:
+        assert a[k] == b[k], (k, a[k], b[k])

@@ -824,6 +824,14 @@ void vt3d_sweep_params_zncc(vt3d_sweep_params *p, int32_t radius, int32_t hyps, 
 
 const char *vt3d_skip_reason(int32_t code) { return skip_string(code); }
 
+void vt3d_plane_homography(const vt3d_camera *a, uint32_t wa, uint32_t ha, const vt3d_camera *b,
+                           uint32_t wb, uint32_t hb, double focal, double depth, double *H) {
+    if (!a || !b || !H) return;
+    double A[9], B[9];
+    plane_homography_parts(to_cam(*a), wa, ha, to_cam(*b), wb, hb, focal, A, B);
+    for (int k = 0; k < 9; ++k) H[k] = depth * A[k] + B[k];
+}
+
 uint32_t vt3d_grid_stride(uint32_t w, uint32_t h, int32_t fine) {
     return grid_stride_rule(w, h, fine != 0);
 }

@@ -56,6 +56,10 @@ def _bind(lib: C.CDLL) -> None:
     lib.vt3d_grid_stride.restype = C.c_uint32
     lib.vt3d_abi_version.argtypes = []
     lib.vt3d_abi_version.restype = C.c_int32
+    lib.vt3d_plane_homography.argtypes = [P(abi.Camera), C.c_uint32, C.c_uint32, P(abi.Camera),
+                                          C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                          P(C.c_double)]
+    lib.vt3d_plane_homography.restype = None
     lib.vt3d_last_error.argtypes = []
     lib.vt3d_last_error.restype = C.c_char_p
     common = [P(abi.Frame), C.c_size_t, P(abi.Camera), C.c_size_t, P(C.c_double), C.c_size_t,
@@ -89,7 +93,7 @@ EXPORTED_SYMBOLS = (
     "vt3d_grid_stride", "vt3d_abi_version", "vt3d_dense_plane_sweep",
     "vt3d_dense_reference_patch", "vt3d_dense_result_free", "vt3d_session_create",
     "vt3d_session_destroy", "vt3d_session_upload_frames", "vt3d_session_solve",
-    "vt3d_session_last_timing", "vt3d_last_error",
+    "vt3d_session_last_timing", "vt3d_last_error", "vt3d_plane_homography",
 )
 
 
