@@ -63,6 +63,11 @@ typedef int32_t (*vt3d_poll_fn)(void *user);
 enum vt3d_cost_kind { VT3D_COST_ZSAD = 0, VT3D_COST_ZNCC = 1 };
 enum vt3d_ambiguity_kind { VT3D_AMBIGUITY_MARGIN = 0, VT3D_AMBIGUITY_RATIO = 1 };
 enum vt3d_storage { VT3D_STORAGE_F32 = 0, VT3D_STORAGE_F16 = 1, VT3D_STORAGE_BF16 = 2 };
+/* Primary-sweep kernel: DIRECT warps every tap of every patch (one lane per sample);
+ * TILED warps each reference pixel of a workgroup tile once per (hypothesis, source) into
+ * LDS and sums the overlapping patches there; AUTO picks TILED when patches overlap
+ * (grid stride < patch size). Results are the same up to FP32 rounding. */
+enum vt3d_sweep_kernel { VT3D_KERNEL_AUTO = 0, VT3D_KERNEL_DIRECT = 1, VT3D_KERNEL_TILED = 2 };
 
 /* Sweep parameters. vt3d_sweep_params_reference() gives the dense_legacy.rs:6-21 constants
  * (3x3 zero-mean SAD, 24 hypotheses, 4 sources, additive ambiguity margin 1.0);
@@ -79,7 +84,7 @@ typedef struct vt3d_sweep_params {
     int32_t surface_completion_passes; /* SURFACE_COMPLETION_PASSES, dense_legacy.rs:16 */
     int32_t storage;                   /* vt3d_storage: image storage on the device */
     int32_t emit_samples;              /* nonzero: fill vt3d_dense_result.samples */
-    int32_t reserved0;
+    int32_t sweep_kernel;              /* vt3d_sweep_kernel: primary-sweep kernel choice */
     double min_reference_contrast;     /* dense_legacy.rs:10 */
     double max_photometric_error;      /* dense_legacy.rs:11 (cost units) */
     double ambiguity_threshold;        /* margin (dense_legacy.rs:12) or ratio */

@@ -36,8 +36,12 @@ def compare(gpu, oracle, slack, cost_kind: int, max_unexplained: int = 0):
               "unexplained": int(unexplained.sum())}
     assert report["unexplained"] <= max_unexplained, (report, _describe(gs, os_, slack,
                                                                         np.nonzero(unexplained)[0]))
-    # Points: accepted on both sides with the same hypothesis and reciprocal set.
-    same = (agree & (gs["status"] == 6) & (gs["reciprocal_mask"] == os_["reciprocal_mask"]))
+    # Points: accepted on both sides with the same hypothesis and reciprocal set, and no near
+    # tie (a reverse-sweep tie moves a reciprocal observation without changing any index
+    # reported here).
+    same = (agree & ~near & (gs["status"] == 6)
+            & (gs["reciprocal_mask"] == os_["reciprocal_mask"]))
+    report["points_compared"] = int(same.sum())
     gp = _points_by_site(gpu)
     op = _points_by_site(oracle)
     worst = 0.0

@@ -51,24 +51,26 @@ def test_plane_fixture_matches_oracle(product, oracle):
         assert g.stats[k] == o.stats[k], k
 
 
+@pytest.mark.parametrize("kernel", ["direct", "tiled"])
 @pytest.mark.parametrize("preset", ["reference", "zncc"])
-def test_c1_matches_oracle(product, oracle, preset):
+def test_c1_matches_oracle(product, oracle, preset, kernel):
     c1 = scenes.config_c1()
     st = dict(c1.settings) if preset == "zncc" else {"preset": "reference"}
     st.update(surface_completion_passes=0, emit_samples=1)
-    g = _gpu(product, c1, st)
+    g = _gpu(product, c1, st, kernel=kernel)
     o, sl = _oracle(oracle, c1, st)
     parity.stats_equal(g.stats, o.stats)
     rep = parity.compare(g, o, sl, st.get("preset") == "zncc" and 1 or 0)
-    print("c1", preset, rep)
+    print("c1", preset, kernel, rep)
     assert rep["agree"] >= 0.999
     _check_counters(g.stats)
 
 
+@pytest.mark.parametrize("kernel", ["direct", "tiled"])
 @pytest.mark.parametrize("kind,radius,hyps,sources", [
     ("zncc", 2, 64, 4), ("zncc", 4, 48, 4), ("zncc", 5, 32, 4), ("zncc", 1, 96, 4),
     ("zsad", 2, 40, 4), ("zsad", 3, 32, 4)])
-def test_patch_sizes_match_oracle(product, oracle, kind, radius, hyps, sources):
+def test_patch_sizes_match_oracle(product, oracle, kind, radius, hyps, sources, kernel):
     c1 = scenes.config_c1()
     st = dict(c1.settings, patch_radius=radius, depth_hypotheses=hyps, max_source_views=sources,
               surface_completion_passes=0, emit_samples=1)
@@ -76,20 +78,21 @@ def test_patch_sizes_match_oracle(product, oracle, kind, radius, hyps, sources):
         st = {"preset": "reference", "patch_radius": radius, "depth_hypotheses": hyps,
               "max_source_views": sources, "grid_stride": 4, "surface_completion_passes": 0,
               "emit_samples": 1}
-    g = _gpu(product, c1, st)
+    g = _gpu(product, c1, st, kernel=kernel)
     o, sl = _oracle(oracle, c1, st)
     parity.stats_equal(g.stats, o.stats)
     rep = parity.compare(g, o, sl, 1 if kind == "zncc" else 0)
-    print(kind, radius, rep)
+    print(kind, radius, kernel, rep)
     assert rep["agree"] >= 0.995
 
 
-def test_c2_full_size_band_matches_oracle(product, oracle):
+@pytest.mark.parametrize("kernel", ["direct", "tiled"])
+def test_c2_full_size_band_matches_oracle(product, oracle, kernel):
     """640x360, 8 sources, 128 hypotheses: the full GPU solve against the oracle on a band of
     grid rows (the oracle would take minutes for every row)."""
     c2 = scenes.config_c2()
     st = dict(c2.settings, surface_completion_passes=0, emit_samples=1)
-    g = _gpu(product, c2, st)
+    g = _gpu(product, c2, st, kernel=kernel)
     oracle.set_row_band(80, 3)
     try:
         o, sl = _oracle(oracle, c2, st)
@@ -104,7 +107,7 @@ def test_c2_full_size_band_matches_oracle(product, oracle):
     gb.points, gb.grid_sites = g.points[pk], g.grid_sites[pk]
     gb.stats = dict(g.stats, sampled_pixels=int(keep.sum()))
     rep = parity.compare(gb, o, sl, abi.COST_ZNCC)
-    print("c2 band", rep)
+    print("c2 band", kernel, rep)
     assert rep["samples"] > 500 and rep["agree"] >= 0.999
     _check_counters(g.stats)
 
@@ -122,6 +125,12 @@ def test_c2_is_deterministic_and_storage_independent(product):
         r = _gpu(product, c2, st, storage=storage)
         assert r.points.tobytes() == base.points.tobytes(), storage
         assert r.samples.tobytes() == base.samples.tobytes(), storage
+    # the two primary-sweep kernels reach the same decisions (up to FP32 rounding)
+    direct = _gpu(product, c2, st, kernel="direct")
+    tiled = _gpu(product, c2, st, kernel="tiled")
+    same = ((direct.samples["status"] == tiled.samples["status"])
+            & (direct.samples["best_hypothesis"] == tiled.samples["best_hypothesis"]))
+    assert same.mean() >= 0.999
     pts = base.points
     assert np.all((pts["confidence"] >= 0.05) & (pts["confidence"] <= 1.0))
     at = base.point_attributes

@@ -22,7 +22,8 @@
 namespace vt3d {
 
 bool launch_sweep_or_recip(int storage, int cost, int radius, const Args &a, int smax, int S,
-                           uint32_t blocks, bool recip, hipStream_t s);
+                           uint32_t blocks, bool recip, const TileLaunch *tl, hipStream_t s);
+size_t tile_lds_bytes(int tile_w, int stride, int radius);
 bool launch_build_view(int storage, const uint8_t *rgba, uint32_t w, uint32_t h, uint8_t *luma,
                        void *quad, hipStream_t s);
 size_t texel_bytes(int storage);
@@ -93,7 +94,7 @@ struct vt3d_session {
     int storage = VT3D_STORAGE_F32;
     std::vector<std::unique_ptr<vt3d::DevView>> views;  // uploaded frames by frame index
     // per-solve buffers
-    vt3d::DevBuf solve, src, Hp, Hr, depth_p, depth_r, site_flag, samples, n_samples, partial,
+    vt3d::DevBuf solve, src, Hp, Hr, depth_p, depth_r, site_flag, samples, site_map, n_samples, partial,
         decision, recip, out, block_counts, block_offsets, totals, points, sites, attrs, errors,
         supports, fobs, diag;
     hipEvent_t ev[8] = {};
@@ -265,7 +266,8 @@ bool params_ok(const vt3d_sweep_params &p) {
            p.min_fusion_observations >= 1 && p.surface_completion_passes >= 0 &&
            (p.storage == VT3D_STORAGE_F32 || p.storage == VT3D_STORAGE_F16 ||
             p.storage == VT3D_STORAGE_BF16) &&
-           (p.ambiguity_kind == VT3D_AMBIGUITY_MARGIN || p.ambiguity_kind == VT3D_AMBIGUITY_RATIO);
+           (p.ambiguity_kind == VT3D_AMBIGUITY_MARGIN || p.ambiguity_kind == VT3D_AMBIGUITY_RATIO) &&
+           p.sweep_kernel >= VT3D_KERNEL_AUTO && p.sweep_kernel <= VT3D_KERNEL_TILED;
 }
 
 struct Control {
@@ -294,8 +296,9 @@ struct FrameSource {
 };
 
 int upload_view(vt3d_session *s, DevView &v, const vt3d_frame &f, int storage) {
-    if (!f.rgba || f.width == 0 || f.height == 0 || f.width > 65535 || f.height > 65535) {
-        set_error("frame has no pixels or exceeds 65535 px");
+    if (!f.rgba || f.width < 2 || f.height < 2 || f.width > 65535 || f.height > 65535 ||
+        (size_t)f.width * f.height > (1u << 28)) {
+        set_error("frame has no pixels, is under 2 px, or exceeds 65535 px or 2^28 pixels");
         return VT3D_E_INVALID;
     }
     const size_t n = (size_t)f.width * f.height;
@@ -527,6 +530,7 @@ int plane_sweep(vt3d_session *s, FrameSource &frames, const vt3d_camera *cams_in
         (rc = s->Hp.ensure(Hp.size() * 8)) || (rc = s->Hr.ensure(Hr.size() * 8)) ||
         (rc = s->depth_p.ensure(dp.size() * 8)) || (rc = s->depth_r.ensure(dr.size() * 8)) ||
         (rc = s->site_flag.ensure(nmax)) || (rc = s->samples.ensure(4ull * nmax)) ||
+        (rc = s->site_map.ensure(4ull * nmax)) ||
         (rc = s->n_samples.ensure(16)) ||
         (rc = s->partial.ensure(sizeof(Top2) * (size_t)nmax * n_hb)) ||
         (rc = s->decision.ensure(sizeof(Decision) * (size_t)nmax)) ||
@@ -560,6 +564,7 @@ int plane_sweep(vt3d_session *s, FrameSource &frames, const vt3d_camera *cams_in
     a.ref_rgba_w = w;
     a.site_flag = s->site_flag.as<uint8_t>();
     a.samples = s->samples.as<uint32_t>();
+    a.sample_of_site = s->site_map.as<int>();
     a.n_samples = s->n_samples.as<uint32_t>();
     a.max_samples = nmax;
     a.partial = s->partial.as<Top2>();
@@ -580,14 +585,32 @@ int plane_sweep(vt3d_session *s, FrameSource &frames, const vt3d_camera *cams_in
     launch_compact_sites(a, blocks, st);
     VT3D_HIP(hipGetLastError());
     VT3D_HIP(hipEventRecord(s->ev[3], st));
-    if (!launch_sweep_or_recip(p.storage, p.cost_kind, R, a, smax, S, blocks, false, st)) {
+    // Tiled primary sweep when patches of neighbouring sites overlap (stride < P) and the
+    // warped region fits the LDS budget; the direct kernel otherwise or when asked for.
+    TileLaunch tl{};
+    bool tiled = false;
+    {
+        const int P = 2 * R + 1;
+        const bool want = p.sweep_kernel == VT3D_KERNEL_TILED ||
+                          (p.sweep_kernel == VT3D_KERNEL_AUTO && (int)stride < P);
+        tl.tile_w = 32;
+        tl.lds_bytes = tile_lds_bytes(tl.tile_w, (int)stride, R);
+        if (want && tl.lds_bytes <= 64 * 1024) {
+            tiled = true;
+            tl.tiles_x = (int)((nx + tl.tile_w - 1) / tl.tile_w);
+            const uint32_t ty = (uint32_t)(kBlock / tl.tile_w);
+            tl.tiles = (uint32_t)tl.tiles_x * ((ny + ty - 1) / ty);
+        }
+    }
+    if (!launch_sweep_or_recip(p.storage, p.cost_kind, R, a, smax, S, blocks, false,
+                               tiled ? &tl : nullptr, st)) {
         set_error("no kernel for this cost / radius / storage");
         return VT3D_E_INVALID;
     }
     VT3D_HIP(hipGetLastError());
     VT3D_HIP(hipEventRecord(s->ev[4], st));
     launch_decide(a, blocks, st);
-    launch_sweep_or_recip(p.storage, p.cost_kind, R, a, smax, S, blocks, true, st);
+    launch_sweep_or_recip(p.storage, p.cost_kind, R, a, smax, S, blocks, true, nullptr, st);
     VT3D_HIP(hipGetLastError());
     VT3D_HIP(hipEventRecord(s->ev[5], st));
     launch_fuse(a, p.emit_samples ? s->diag.as<vt3d_sample_diag>() : nullptr, blocks, st);

@@ -110,6 +110,7 @@ struct Args {
     // gate / sample list
     uint8_t *site_flag;     // [grid_nx * grid_ny]
     uint32_t *samples;      // [grid sites] packed x | y << 16
+    int *sample_of_site;    // [grid sites] sample index, -1 where the texture gate failed
     uint32_t *n_samples;    // device counter (total of the gate scan)
     uint32_t max_samples;
     // sweep
@@ -123,6 +124,14 @@ struct Args {
     uint32_t *block_offsets;// [blocks][kCounters]
     uint32_t *totals;       // [kCounters]
     uint32_t n_blocks;
+};
+
+// Launch shape of the tiled primary sweep.
+struct TileLaunch {
+    int tile_w;        // grid sites per tile row; kBlock / tile_w rows
+    int tiles_x;       // tiles per grid row
+    uint32_t tiles;    // tiles in the grid
+    size_t lds_bytes;  // two buffers of the warped region
 };
 
 }  // namespace vt3d

@@ -26,17 +26,23 @@ namespace vt3d {
 constexpr float kInfF = __builtin_huge_valf();
 
 // ---- texel storage ----------------------------------------------------------------------
+//
+// Every pixel (x, y) of a view stores the bilinear coefficients of the cell it anchors:
+//   a = I[x,y], b = I[x+1,y] - a, c = I[x,y+1] - a, d = I[x+1,y+1] - I[x+1,y] - I[x,y+1] + a
+// so the sample at (x + tx, y + ty) is a + tx b + ty c + tx ty d -- one gather and four FMA
+// per tap. Luma is an integer 0..255, so a..d are integers in [-510, 510] and FP16 holds them
+// exactly. BF16 (8 significant bits) holds integers only up to 256 exactly, so BF16 textures
+// store the four raw neighbours (exact) and form the coefficients after the load.
 
 template <int ST> struct Store;
 
 template <> struct Store<VT3D_STORAGE_F32> {
-    static constexpr size_t kBytes = 16;
-    __device__ static __forceinline__ float4 load(const void *q, uint32_t i) {
+    __device__ static __forceinline__ float4 coef(const void *q, uint32_t i) {
         return reinterpret_cast<const float4 *>(q)[i];
     }
     __device__ static __forceinline__ void put(void *q, uint32_t i, float a, float b, float c,
                                                float d) {
-        reinterpret_cast<float4 *>(q)[i] = make_float4(a, b, c, d);
+        reinterpret_cast<float4 *>(q)[i] = make_float4(a, b - a, c - a, d - b - c + a);
     }
 };
 
@@ -51,26 +57,24 @@ __device__ __forceinline__ uint32_t half_bits(float v) {
 }
 
 template <> struct Store<VT3D_STORAGE_F16> {
-    static constexpr size_t kBytes = 8;
-    __device__ static __forceinline__ float4 load(const void *q, uint32_t i) {
+    __device__ static __forceinline__ float4 coef(const void *q, uint32_t i) {
         uint2 u = reinterpret_cast<const uint2 *>(q)[i];
         return make_float4(half_lo(u.x), half_hi(u.x), half_lo(u.y), half_hi(u.y));
     }
     __device__ static __forceinline__ void put(void *q, uint32_t i, float a, float b, float c,
                                                float d) {
         reinterpret_cast<uint2 *>(q)[i] =
-            make_uint2(half_bits(a) | (half_bits(b) << 16), half_bits(c) | (half_bits(d) << 16));
+            make_uint2(half_bits(a) | (half_bits(b - a) << 16),
+                       half_bits(c - a) | (half_bits(d - b - c + a) << 16));
     }
 };
 
-// Luma is an integer 0..255, so BF16 (8 significant bits) stores it exactly; its bits are
-// the top half of the FP32 pattern.
 template <> struct Store<VT3D_STORAGE_BF16> {
-    static constexpr size_t kBytes = 8;
-    __device__ static __forceinline__ float4 load(const void *q, uint32_t i) {
+    __device__ static __forceinline__ float4 coef(const void *q, uint32_t i) {
         uint2 u = reinterpret_cast<const uint2 *>(q)[i];
-        return make_float4(__uint_as_float(u.x << 16), __uint_as_float(u.x & 0xffff0000u),
-                           __uint_as_float(u.y << 16), __uint_as_float(u.y & 0xffff0000u));
+        const float a = __uint_as_float(u.x << 16), b = __uint_as_float(u.x & 0xffff0000u);
+        const float c = __uint_as_float(u.y << 16), d = __uint_as_float(u.y & 0xffff0000u);
+        return make_float4(a, b - a, c - a, d - b - c + a);
     }
     __device__ static __forceinline__ void put(void *q, uint32_t i, float a, float b, float c,
                                                float d) {
@@ -79,6 +83,11 @@ template <> struct Store<VT3D_STORAGE_BF16> {
             make_uint2(bits(a) | (bits(b) << 16), bits(c) | (bits(d) << 16));
     }
 };
+
+// Bilinear value from a coefficient texel.
+__device__ __forceinline__ float bicoef(float4 k, float tx, float ty) {
+    return fmaf(tx * ty, k.w, fmaf(ty, k.z, fmaf(tx, k.y, k.x)));
+}
 
 __device__ __forceinline__ float bilerp(float4 q, float tx, float ty) {
     const float top = fmaf(tx, q.y - q.x, q.x);
@@ -112,7 +121,15 @@ __global__ __launch_bounds__(kBlock) void k_build_view(const uint8_t *__restrict
 
 // ---- patch statistics and cost ------------------------------------------------------------
 
+// Occupancy cap of the sweep kernels: at most 512 / VT3D_SWEEP_WAVES VGPRs per lane, so the
+// compiler keeps fewer gathers in flight per wave and the SIMD holds more waves instead.
+#ifndef VT3D_SWEEP_WAVES
+#define VT3D_SWEEP_WAVES 4
+#endif
+#define VT3D_WAVES __attribute__((amdgpu_waves_per_eu(VT3D_SWEEP_WAVES)))
+
 struct RefStats {
+    float mean;     // mean of the raw reference values (the ZNCC accumulation pivot)
     float sum_hat;  // ZNCC: sum of normalised deviations (0 up to rounding)
     bool flat;      // ZNCC: no variance, every cost is 1
 };
@@ -124,7 +141,7 @@ __device__ __forceinline__ RefStats ref_stats(float (&rp)[T]) {
 #pragma unroll
     for (int k = 0; k < T; ++k) s += rp[k];
     const float mean = s / (float)T;
-    RefStats st{0.f, false};
+    RefStats st{mean, 0.f, false};
     if constexpr (COST == VT3D_COST_ZSAD) {
 #pragma unroll
         for (int k = 0; k < T; ++k) rp[k] -= mean;
@@ -156,7 +173,7 @@ __device__ __forceinline__ bool sample_patch(const uint8_t *__restrict__ l, uint
                                              uint32_t h, double cx, double cy,
                                              float (&out)[(2 * R + 1) * (2 * R + 1)]) {
     constexpr int P = 2 * R + 1;
-    if (!(cx >= -1.0e8 && cx <= 1.0e8 && cy >= -1.0e8 && cy <= 1.0e8)) return false;
+    if (!(cx >= -4096.0 && cx <= 70000.0 && cy >= -4096.0 && cy <= 70000.0)) return false;
     const double fx = floor(cx), fy = floor(cy);
     const int ix = (int)fx, iy = (int)fy;
     const float tx = (float)(cx - fx), ty = (float)(cy - fy);
@@ -188,7 +205,10 @@ struct Warp {
     bool ok;
 };
 
-__device__ __forceinline__ Warp make_warp(const double *__restrict__ H, double cx, double cy) {
+// sw, sh: source size. A centre farther than 4096 px outside the source cannot have a tap
+// inside it for any patch these kernels support, and the bound keeps texel indices in int32.
+__device__ __forceinline__ Warp make_warp(const double *__restrict__ H, double cx, double cy,
+                                          uint32_t sw, uint32_t sh) {
     const double nc = fma(H[0], cx, fma(H[1], cy, H[2]));
     const double mc = fma(H[3], cx, fma(H[4], cy, H[5]));
     const double dc = fma(H[6], cx, fma(H[7], cy, H[8]));
@@ -197,12 +217,13 @@ __device__ __forceinline__ Warp make_warp(const double *__restrict__ H, double c
     w.ok = dc > 1.0e-4;
     const double rd = 1.0 / (w.ok ? dc : 1.0);
     const double xc = nc * rd, yc = mc * rd;
-    w.ok = w.ok && fabs(xc) < 1.0e8 && fabs(yc) < 1.0e8;
+    w.ok = w.ok && xc > -4096.0 && xc < (double)sw + 4096.0 && yc > -4096.0 &&
+           yc < (double)sh + 4096.0;
     const double fx = w.ok ? floor(xc) : 0.0, fy = w.ok ? floor(yc) : 0.0;
     w.ixc = (int)fx;
     w.iyc = (int)fy;
-    w.fxc = (float)(xc - fx);
-    w.fyc = (float)(yc - fy);
+    w.fxc = w.ok ? (float)(xc - fx) : 0.f;
+    w.fyc = w.ok ? (float)(yc - fy) : 0.f;
     w.g = (float)(H[6] * rd);
     w.e = (float)(H[7] * rd);
     w.ax = (float)((H[0] - xc * H[6]) * rd);
@@ -213,54 +234,10 @@ __device__ __forceinline__ Warp make_warp(const double *__restrict__ H, double c
     return w;
 }
 
-// patch_error (dense_legacy.rs:1031-1095) for one (sample, hypothesis, source): every tap of
-// the reference patch is warped into the source texture and sampled bilinearly; any tap
-// outside [0, w-1) x [0, h-1) or with source depth <= 1e-4 makes the source not count
-// (+inf). ZSAD: mean |(r - r_mean) - (s - s_mean)|; ZNCC: 1 - ZNCC.
-template <int R, int COST, int ST>
-__device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
-                                            const RefStats &st, const double *__restrict__ H,
-                                            double cx, double cy, const void *__restrict__ tex,
-                                            uint32_t sw, uint32_t sh) {
-    constexpr int P = 2 * R + 1;
-    constexpr int T = P * P;
-    const Warp w = make_warp(H, cx, cy);
-    bool ok = w.ok;
-    const uint32_t wm2 = sw - 2u, hm2 = sh - 2u, nlast = sw * sh - 1u;
-    float s1 = 0.f, s2 = 0.f, srs = 0.f, pivot = 0.f;
-    float sv[COST == VT3D_COST_ZSAD ? T : 1];
-#pragma unroll
-    for (int dy = -R; dy <= R; ++dy) {
-        const float fdy = (float)dy;
-        const float row_den = fmaf(w.e, fdy, 1.f);
-        const float row_x = w.bx * fdy;
-        const float row_y = w.by * fdy;
-#pragma unroll
-        for (int dx = -R; dx <= R; ++dx) {
-            const int k = (dy + R) * P + (dx + R);
-            const float fdx = (float)dx;
-            const float den = fmaf(w.g, fdx, row_den);
-            const float rinv = __builtin_amdgcn_rcpf(den);
-            const float qx = fmaf(fmaf(w.ax, fdx, row_x), rinv, w.fxc);
-            const float qy = fmaf(fmaf(w.ay, fdx, row_y), rinv, w.fyc);
-            const float flx = floorf(qx), fly = floorf(qy);
-            const int ix = w.ixc + (int)flx, iy = w.iyc + (int)fly;
-            ok = ok && ((uint32_t)ix <= wm2) && ((uint32_t)iy <= hm2) && (w.dc * den > 1.0e-4f);
-            const uint32_t idx = min((uint32_t)iy * sw + (uint32_t)ix, nlast);
-            const float v = bilerp(Store<ST>::load(tex, idx), qx - flx, qy - fly);
-            if constexpr (COST == VT3D_COST_ZSAD) {
-                sv[k] = v;
-                s1 += v;
-            } else {
-                if (k == 0) pivot = v;
-                const float d = v - pivot;
-                s1 += d;
-                s2 = fmaf(d, d, s2);
-                srs = fmaf(rp[k], d, srs);
-            }
-        }
-    }
-    if (!ok) return kInfF;
+// Result of the ZNCC / ZSAD reduction of one patch.
+template <int COST, int T>
+__device__ __forceinline__ float finish_cost(const float (&rp)[T], const RefStats &st, float s1,
+                                             float s2, float srs, const float *sv) {
     if constexpr (COST == VT3D_COST_ZSAD) {
         const float sm = s1 / (float)T;
         float acc = 0.f;
@@ -274,6 +251,128 @@ __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 *
         const float cov = fmaf(-m, st.sum_hat, srs);
         return 1.f - cov * rsqrtf(var);
     }
+}
+
+// patch_error (dense_legacy.rs:1031-1095) for one (sample, hypothesis, source): every tap of
+// the reference patch is warped into the source texture and sampled bilinearly; a tap outside
+// [0, w-1) x [0, h-1) or with source depth <= 1e-4 makes the source not count (+inf).
+// ZSAD: mean |(r - r_mean) - (s - s_mean)|; ZNCC: 1 - ZNCC.
+//
+// Validity is decided on the four corner taps before any texel is read. Over a patch the
+// source depth is affine in (dx, dy), so its minimum is at a corner; with that depth positive
+// the warp is linear-fractional, so the extremes of the source x and y are at corners too.
+// Interior taps therefore lie inside the corners' box; their texel index is still clamped so
+// that FP32 rounding at a box edge can never address outside the texture.
+template <int R, int COST, int ST>
+__device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
+                                            const RefStats &st, const double *__restrict__ H,
+                                            double cx, double cy, const void *__restrict__ tex,
+                                            uint32_t sw, uint32_t sh) {
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    const Warp w = make_warp(H, cx, cy, sw, sh);
+    bool ok = w.ok;
+    const float fr = (float)R;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float fdx = (c & 1) ? fr : -fr, fdy = (c & 2) ? fr : -fr;
+        const float den = fmaf(w.g, fdx, fmaf(w.e, fdy, 1.f));
+        const float rinv = __builtin_amdgcn_rcpf(den);
+        const float qx = fmaf(fmaf(w.ax, fdx, w.bx * fdy), rinv, w.fxc);
+        const float qy = fmaf(fmaf(w.ay, fdx, w.by * fdy), rinv, w.fyc);
+        // in [0, w-1) relative to the centre's integer cell
+        ok = ok && den > 0.f && w.dc * den > 1.0e-4f && qx >= (float)(-w.ixc) &&
+             qx < (float)((int)sw - 1 - w.ixc) && qy >= (float)(-w.iyc) &&
+             qy < (float)((int)sh - 1 - w.iyc);
+    }
+    if (!ok) return kInfF;
+    const uint32_t nlast = sw * sh - 1u;
+    const uint32_t base = (uint32_t)(w.iyc * (int)sw + w.ixc);
+    const float pivot = st.mean;
+    float s1 = 0.f, s2 = 0.f, srs = 0.f;
+    float sv[COST == VT3D_COST_ZSAD ? T : 1];
+#pragma unroll
+    for (int dy = -R; dy <= R; ++dy) {
+        const float fdy = (float)dy;
+        const float row_den = fmaf(w.e, fdy, 1.f);
+        const float row_x = w.bx * fdy;
+        const float row_y = w.by * fdy;
+#pragma unroll
+        for (int dx = -R; dx <= R; ++dx) {
+            const int k = (dy + R) * P + (dx + R);
+            const float fdx = (float)dx;
+            const float rinv = __builtin_amdgcn_rcpf(fmaf(w.g, fdx, row_den));
+            const float qx = fmaf(fmaf(w.ax, fdx, row_x), rinv, w.fxc);
+            const float qy = fmaf(fmaf(w.ay, fdx, row_y), rinv, w.fyc);
+            const int ixo = (int)floorf(qx), iyo = (int)floorf(qy);
+            const float tx = __builtin_amdgcn_fractf(qx), ty = __builtin_amdgcn_fractf(qy);
+            const uint32_t idx =
+                min(base + (uint32_t)__mul24(iyo, (int)sw) + (uint32_t)ixo, nlast);
+            const float v = bicoef(Store<ST>::coef(tex, idx), tx, ty);
+            if constexpr (COST == VT3D_COST_ZSAD) {
+                sv[k] = v;
+                s1 += v;
+            } else {
+                const float d = v - pivot;
+                s1 += d;
+                s2 = fmaf(d, d, s2);
+                srs = fmaf(rp[k], d, srs);
+            }
+        }
+    }
+    return finish_cost<COST, T>(rp, st, s1, s2, srs, sv);
+}
+
+// Bilinear sample of a source texture at the FP64 warp of reference pixel (u, v) under the
+// homography H, exactly as patch_error evaluates one tap; NaN when the tap is invalid.
+template <int ST>
+__device__ __forceinline__ float warp_sample(const double *__restrict__ H, double u, double v,
+                                             const void *__restrict__ tex, uint32_t sw,
+                                             uint32_t sh) {
+    const double n = fma(H[0], u, fma(H[1], v, H[2]));
+    const double m = fma(H[3], u, fma(H[4], v, H[5]));
+    const double z = fma(H[6], u, fma(H[7], v, H[8]));
+    const float nan = __builtin_nanf("");
+    if (!(z > 1.0e-4)) return nan;
+    const double r = 1.0 / z;
+    const double xs = n * r, ys = m * r;
+    if (!(xs >= 0.0 && ys >= 0.0 && xs < (double)(sw - 1) && ys < (double)(sh - 1))) return nan;
+    const double fx = floor(xs), fy = floor(ys);
+    const uint32_t idx = (uint32_t)fy * sw + (uint32_t)fx;
+    return bicoef(Store<ST>::coef(tex, idx), (float)(xs - fx), (float)(ys - fy));
+}
+
+// Patch cost from a tile of warped source values in LDS (row pitch `pitch` floats); NaN in
+// any tap (an invalid tap) makes the source not count.
+template <int R, int COST>
+__device__ __forceinline__ float lds_patch_cost(const float *__restrict__ L, int pitch,
+                                                const float (&rp)[(2 * R + 1) * (2 * R + 1)],
+                                                const RefStats &st) {
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    const float pivot = st.mean;
+    float s1 = 0.f, s2 = 0.f, srs = 0.f;
+    float sv[COST == VT3D_COST_ZSAD ? T : 1];
+#pragma unroll
+    for (int dy = 0; dy < P; ++dy) {
+        const float *row = L + dy * pitch;
+#pragma unroll
+        for (int dx = 0; dx < P; ++dx) {
+            const int k = dy * P + dx;
+            const float v = row[dx];
+            if constexpr (COST == VT3D_COST_ZSAD) {
+                sv[k] = v;
+                s1 += v;
+            } else {
+                const float d = v - pivot;
+                s1 += d;
+                s2 = fmaf(d, d, s2);
+                srs = fmaf(rp[k], d, srs);
+            }
+        }
+    }
+    if (__builtin_isnan(s1)) return kInfF;
+    return finish_cost<COST, T>(rp, st, s1, s2, srs, sv);
 }
 
 // ---- candidate order -----------------------------------------------------------------------
@@ -406,10 +505,13 @@ __global__ __launch_bounds__(kBlock) void k_compact_sites(Args a) {
     const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
     const bool flag = j < nsites && a.site_flag[j] != 0;
     const uint32_t r = block_rank(flag);
+    if (j < nsites) a.sample_of_site[j] = -1;
     if (flag) {
         const uint32_t x = sd.border + (j % sd.grid_nx) * sd.stride;
         const uint32_t y = sd.border + (j / sd.grid_nx) * sd.stride;
-        a.samples[a.block_offsets[blockIdx.x * kCounters] + r] = x | (y << 16);
+        const uint32_t i = a.block_offsets[blockIdx.x * kCounters] + r;
+        a.samples[i] = x | (y << 16);
+        a.sample_of_site[j] = (int)i;
     }
 }
 
@@ -432,7 +534,7 @@ __device__ __forceinline__ void sorted_insert(float (&l)[N], float c) {
 // sources that pass, error = quantile(sorted errors, 0.5) (index n >> 1, dense_legacy.rs:
 // 1168-1174), then the running top-2 in candidate order.
 template <int R, int COST, int ST, int SMAX>
-__global__ __launch_bounds__(kBlock) void k_sweep(Args a) {
+__global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep(Args a) {
     constexpr int P = 2 * R + 1;
     constexpr int T = P * P;
     const SolveDev &sd = *a.solve;
@@ -482,6 +584,93 @@ __global__ __launch_bounds__(kBlock) void k_sweep(Args a) {
         }
     }
     a.partial[(size_t)blockIdx.y * a.max_samples + i] = t;
+}
+
+// Tiled primary sweep. A workgroup owns a TX x TY tile of grid sites (TX * TY = kBlock);
+// for each (hypothesis, source) its threads first warp every reference pixel the tile's
+// patches touch (a ((TX-1) s + P) x ((TY-1) s + P) region, FP64 homography per pixel, exactly
+// as patch_error warps a tap) into LDS, then each lane sums its own patch from LDS. Patches
+// of neighbouring sites overlap when the grid stride s is below the patch size P, so each
+// warp and gather serves up to (P / s)^2 taps. Double-buffered LDS: one barrier per step.
+template <int R, int COST, int ST, int SMAX>
+__global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep_tile(Args a, int tile_w,
+                                                                 int tiles_x) {
+    extern __shared__ float lds[];
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    const SolveDev &sd = *a.solve;
+    const int TX = tile_w, TY = kBlock / tile_w;
+    const int stride = (int)sd.stride;
+    const int RW = (TX - 1) * stride + P, RH = (TY - 1) * stride + P;
+    const int npx = RW * RH;
+    const int bx = (int)blockIdx.x % tiles_x, by = (int)blockIdx.x / tiles_x;
+    const int lx = (int)threadIdx.x % TX, ly = (int)threadIdx.x / TX;
+    const int gx = bx * TX + lx, gy = by * TY + ly;
+    const bool in_grid = gx < (int)sd.grid_nx && gy < (int)sd.grid_ny;
+    const int sidx = in_grid ? a.sample_of_site[gy * (int)sd.grid_nx + gx] : -1;
+    const bool active = sidx >= 0;
+    float rp[T];
+    {
+        const int x = (int)sd.border + gx * stride, y = (int)sd.border + gy * stride;
+        const uint8_t *l = sd.ref.luma;
+        const uint32_t w = sd.ref.w;
+#pragma unroll
+        for (int dy = -R; dy <= R; ++dy)
+#pragma unroll
+            for (int dx = -R; dx <= R; ++dx)
+                rp[(dy + R) * P + (dx + R)] =
+                    active ? (float)l[(size_t)(y + dy) * w + (x + dx)] : 0.f;
+    }
+    const RefStats st = ref_stats<COST, T>(rp);
+    // Region origin (reference pixels) and this thread's walk over the region.
+    const int X0 = (int)sd.border + bx * TX * stride - R;
+    const int Y0 = (int)sd.border + by * TY * stride - R;
+    const int py0 = (int)threadIdx.x / RW, px0 = (int)threadIdx.x - py0 * RW;
+    const int step_y = kBlock / RW, step_x = kBlock - step_y * RW;
+    const int S = sd.S, D = sd.D;
+    const int h0 = (int)blockIdx.y * a.hblock;
+    const int h1 = min(h0 + a.hblock, D);
+    const float maxc = sd.max_cost;
+    const float *tile = lds + (ly * stride) * RW + lx * stride;
+    Top2 t{-1, 0, kInfF, -1, 0, kInfF};
+    int buf = 0;
+    for (int hyp = h0; hyp < h1; ++hyp) {
+        float l[SMAX];
+#pragma unroll
+        for (int k = 0; k < SMAX; ++k) l[k] = kInfF;
+        for (int s = 0; s < S; ++s) {
+            const SourceDev &src = a.src[s];
+            const double *H = a.Hp + ((size_t)s * D + hyp) * 9;
+            float *L = lds + buf * npx;
+            for (int px = px0, py = py0; py < RH;) {
+                L[py * RW + px] = warp_sample<ST>(H, (double)(X0 + px), (double)(Y0 + py),
+                                                  src.tex.quad, src.tex.w, src.tex.h);
+                px += step_x;
+                py += step_y;
+                if (px >= RW) {
+                    px -= RW;
+                    ++py;
+                }
+            }
+            __syncthreads();
+            float c = kInfF;
+            if (active) c = lds_patch_cost<R, COST>(tile + buf * npx, RW, rp, st);
+            c = c <= maxc ? c : kInfF;
+            sorted_insert<SMAX>(l, c);
+            buf ^= 1;
+        }
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < SMAX; ++k) cnt += l[k] < kInfF ? 1 : 0;
+        if (cnt > 0) {
+            const int mid = cnt >> 1;
+            float med = l[0];
+#pragma unroll
+            for (int k = 1; k < SMAX; ++k) med = k == mid ? l[k] : med;
+            top2_insert(t, cnt, med, hyp);
+        }
+    }
+    if (active) a.partial[(size_t)blockIdx.y * a.max_samples + sidx] = t;
 }
 
 // ---- k_decide ------------------------------------------------------------------------------
@@ -572,7 +761,7 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
 // image (estimate_single_view_depth, dense_legacy.rs:887-949), the 8% agreement test and the
 // unprojection of the source pixel at the recovered depth.
 template <int R, int COST, int ST>
-__global__ __launch_bounds__(kBlock) void k_recip(Args a) {
+__global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     constexpr int P = 2 * R + 1;
     constexpr int T = P * P;
     const SolveDev &sd = *a.solve;
@@ -775,17 +964,24 @@ void launch_build_view_t(const uint8_t *rgba, uint32_t w, uint32_t h, uint8_t *l
                        rgba, w, h, luma, quad);
 }
 
+template <int R, int COST, int ST, int SMAX>
+void launch_sweep_s(const Args &a, uint32_t blocks, const TileLaunch *tl, hipStream_t s) {
+    if (tl) {
+        hipLaunchKernelGGL((k_sweep_tile<R, COST, ST, SMAX>), dim3(tl->tiles, a.n_hblocks),
+                           dim3(kBlock), tl->lds_bytes, s, a, tl->tile_w, tl->tiles_x);
+    } else {
+        hipLaunchKernelGGL((k_sweep<R, COST, ST, SMAX>), dim3(blocks, a.n_hblocks), dim3(kBlock),
+                           0, s, a);
+    }
+}
+
 template <int R, int COST, int ST>
-void launch_sweep_t(const Args &a, int smax, uint32_t blocks, hipStream_t s) {
-    dim3 grid(blocks, a.n_hblocks);
-    if (smax <= 4)
-        hipLaunchKernelGGL((k_sweep<R, COST, ST, 4>), grid, dim3(kBlock), 0, s, a);
-    else if (smax <= 8)
-        hipLaunchKernelGGL((k_sweep<R, COST, ST, 8>), grid, dim3(kBlock), 0, s, a);
-    else if (smax <= 16)
-        hipLaunchKernelGGL((k_sweep<R, COST, ST, 16>), grid, dim3(kBlock), 0, s, a);
-    else
-        hipLaunchKernelGGL((k_sweep<R, COST, ST, 32>), grid, dim3(kBlock), 0, s, a);
+void launch_sweep_t(const Args &a, int smax, uint32_t blocks, const TileLaunch *tl,
+                    hipStream_t s) {
+    if (smax <= 4) launch_sweep_s<R, COST, ST, 4>(a, blocks, tl, s);
+    else if (smax <= 8) launch_sweep_s<R, COST, ST, 8>(a, blocks, tl, s);
+    else if (smax <= 16) launch_sweep_s<R, COST, ST, 16>(a, blocks, tl, s);
+    else launch_sweep_s<R, COST, ST, 32>(a, blocks, tl, s);
 }
 
 template <int R, int COST, int ST>
@@ -795,11 +991,11 @@ void launch_recip_t(const Args &a, int S, uint32_t blocks, hipStream_t s) {
 
 template <int COST, int ST>
 bool dispatch_r(int radius, const Args &a, int smax, int S, uint32_t blocks, bool recip,
-                hipStream_t s) {
+                const TileLaunch *tl, hipStream_t s) {
 #define VT3D_CASE(RR)                                                           \
     case RR:                                                                    \
         if (recip) launch_recip_t<RR, COST, ST>(a, S, blocks, s);               \
-        else launch_sweep_t<RR, COST, ST>(a, smax, blocks, s);                  \
+        else launch_sweep_t<RR, COST, ST>(a, smax, blocks, tl, s);              \
         return true;
     if constexpr (COST == VT3D_COST_ZSAD) {
         switch (radius) {
@@ -823,17 +1019,21 @@ bool dispatch_r(int radius, const Args &a, int smax, int S, uint32_t blocks, boo
 
 template <int ST>
 bool dispatch_cost(int cost, int radius, const Args &a, int smax, int S, uint32_t blocks,
-                   bool recip, hipStream_t s) {
-    if (cost == VT3D_COST_ZSAD) return dispatch_r<VT3D_COST_ZSAD, ST>(radius, a, smax, S, blocks, recip, s);
-    return dispatch_r<VT3D_COST_ZNCC, ST>(radius, a, smax, S, blocks, recip, s);
+                   bool recip, const TileLaunch *tl, hipStream_t s) {
+    if (cost == VT3D_COST_ZSAD)
+        return dispatch_r<VT3D_COST_ZSAD, ST>(radius, a, smax, S, blocks, recip, tl, s);
+    return dispatch_r<VT3D_COST_ZNCC, ST>(radius, a, smax, S, blocks, recip, tl, s);
 }
 
 bool launch_sweep_or_recip(int storage, int cost, int radius, const Args &a, int smax, int S,
-                           uint32_t blocks, bool recip, hipStream_t s) {
+                           uint32_t blocks, bool recip, const TileLaunch *tl, hipStream_t s) {
     switch (storage) {
-    case VT3D_STORAGE_F32: return dispatch_cost<VT3D_STORAGE_F32>(cost, radius, a, smax, S, blocks, recip, s);
-    case VT3D_STORAGE_F16: return dispatch_cost<VT3D_STORAGE_F16>(cost, radius, a, smax, S, blocks, recip, s);
-    case VT3D_STORAGE_BF16: return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, S, blocks, recip, s);
+    case VT3D_STORAGE_F32:
+        return dispatch_cost<VT3D_STORAGE_F32>(cost, radius, a, smax, S, blocks, recip, tl, s);
+    case VT3D_STORAGE_F16:
+        return dispatch_cost<VT3D_STORAGE_F16>(cost, radius, a, smax, S, blocks, recip, tl, s);
+    case VT3D_STORAGE_BF16:
+        return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, S, blocks, recip, tl, s);
     default: return false;
     }
 }
@@ -850,6 +1050,13 @@ bool launch_build_view(int storage, const uint8_t *rgba, uint32_t w, uint32_t h,
 
 size_t texel_bytes(int storage) {
     return storage == VT3D_STORAGE_F32 ? 16 : 8;
+}
+
+// LDS bytes of the tiled sweep for a tile width (sites per tile row) and grid stride.
+size_t tile_lds_bytes(int tile_w, int stride, int radius) {
+    const int P = 2 * radius + 1;
+    const int TY = kBlock / tile_w;
+    return 2ull * 4ull * (size_t)((tile_w - 1) * stride + P) * (size_t)((TY - 1) * stride + P);
 }
 
 void launch_gate(const Args &a, uint32_t blocks, hipStream_t s) {

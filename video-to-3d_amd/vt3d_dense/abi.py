@@ -22,6 +22,7 @@ VT3D_E_NO_DEVICE = -3
 COST_ZSAD, COST_ZNCC = 0, 1
 AMBIGUITY_MARGIN, AMBIGUITY_RATIO = 0, 1
 STORAGE = {"f32": 0, "f16": 1, "bf16": 2}
+KERNEL = {"auto": 0, "direct": 1, "tiled": 2}
 
 SAMPLE_NO_CANDIDATE = 1
 SAMPLE_REJECT_COST = 2
@@ -46,7 +47,7 @@ class SweepParams(C.Structure):
                 ("max_source_views", C.c_int32), ("grid_stride", C.c_int32),
                 ("ambiguity_kind", C.c_int32), ("min_fusion_observations", C.c_int32),
                 ("surface_completion_passes", C.c_int32), ("storage", C.c_int32),
-                ("emit_samples", C.c_int32), ("reserved0", C.c_int32),
+                ("emit_samples", C.c_int32), ("sweep_kernel", C.c_int32),
                 ("min_reference_contrast", C.c_double), ("max_photometric_error", C.c_double),
                 ("ambiguity_threshold", C.c_double),
                 ("max_reciprocal_relative_depth_error", C.c_double),
@@ -234,6 +235,8 @@ def make_params(lib_reference, lib_zncc, settings: Optional[dict] = None, **over
         p.storage = STORAGE[s["storage"]] if isinstance(s["storage"], str) else int(s["storage"])
     if "cost_kind" in s:
         p.cost_kind = int(s["cost_kind"])
+    if "kernel" in s:
+        p.sweep_kernel = KERNEL[s["kernel"]] if isinstance(s["kernel"], str) else int(s["kernel"])
     return p
 
 
