@@ -172,6 +172,9 @@ typedef struct vt3d_sample_diag {
     float second_cost;
     uint32_t reciprocal_mask;    /* bit s: source s gave a reciprocal observation */
     uint32_t fused_observations; /* 0 unless fusion ran */
+    uint32_t reciprocal_signature; /* FNV-1a over (s, reverse-sweep hypothesis) of every source
+                                      whose reverse sweep found a depth, in source order */
+    uint32_t pad;
 } vt3d_sample_diag;
 
 /* Skip codes; vt3d_skip_reason() returns the exact reference string. */

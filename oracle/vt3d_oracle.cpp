@@ -108,16 +108,15 @@ Params to_params(const vt3d_sweep_params &p) {
     return q;
 }
 
+// cost / px / rel: slack of decisions that set a sample's status or chosen hypothesis.
+// pos: slack of reverse-sweep ties that only move a reciprocal observation (and with it the
+// fused point) by one hypothesis step.
 struct Slack {
-    double cost = kInf, px = kInf, rel = kInf;
+    double cost = kInf, px = kInf, rel = kInf, pos = kInf;
     void c(double v) { cost = std::min(cost, std::fabs(v)); }
     void p(double v) { px = std::min(px, std::fabs(v)); }
     void r(double v) { rel = std::min(rel, std::fabs(v)); }
-    void merge(const Slack &o) {
-        cost = std::min(cost, o.cost);
-        px = std::min(px, o.px);
-        rel = std::min(rel, o.rel);
-    }
+    void q(double v) { pos = std::min(pos, std::fabs(v)); }
 };
 
 // unproject, dense_legacy.rs:996-1011.
@@ -350,32 +349,64 @@ struct SourceView {
 // estimate_single_view_depth, dense_legacy.rs:887-949.
 bool single_view_depth(const Cam &rc, const Cam &sc, const uint8_t *rl, const uint8_t *sl,
                        uint32_t rw, uint32_t rh, uint32_t sw, uint32_t sh, double x, double y,
-                       double mn, double mx, double f, const Params &p, double *out, Slack *sk) {
+                       double mn, double mx, double f, const Params &p, double *out, Slack *sk,
+                       double *runner_up = nullptr, double *gap = nullptr, int *best_h = nullptr) {
     double contrast = patch_contrast(rl, rw, rh, x, y, p.radius);
     if (sk) sk->c(contrast - p.min_contrast);
     if (contrast < p.min_contrast) return false;
     struct C {
         double depth, error;
+        int h;
     };
     std::vector<C> cands;
     cands.reserve(p.D);
+    // Diagnostics: a valid patch's gate matters if its cost is near the winner's; the
+    // winner's validity matters too. A patch turning valid is not tracked (its cost would
+    // be unknown); such flips sit at bilinear edges the primary checks already cover.
+    std::vector<C> all;
+    std::vector<double> edge;
     for (int hyp = 0; hyp < p.D; ++hyp) {
         double depth = hypothesis_depth(hyp, p.D, mn, mx);
         double e;
-        if (!patch_error(rc, sc, rl, sl, rw, rh, sw, sh, x, y, depth, f, p.cost_kind, p.radius,
-                         &e, sk))
-            continue;
-        if (sk) sk->c(e - p.max_err);
-        if (e <= p.max_err) cands.push_back({depth, e});
+        Slack local;
+        bool valid = patch_error(rc, sc, rl, sl, rw, rh, sw, sh, x, y, depth, f, p.cost_kind,
+                                 p.radius, &e, sk ? &local : nullptr);
+        if (!valid) continue;
+        if (sk) {
+            all.push_back({depth, e, hyp});
+            edge.push_back(local.px);
+        }
+        if (e <= p.max_err) cands.push_back({depth, e, hyp});
     }
     std::stable_sort(cands.begin(), cands.end(), [](const C &a, const C &b) {
         if (a.error != b.error) return a.error < b.error;
         return a.depth < b.depth;
     });
+    if (sk) {
+        const double best = cands.empty() ? kInf : cands[0].error;
+        for (size_t i = 0; i < all.size(); ++i) {
+            // gate flips of patches at or below the runner-up can change the argmin
+            if (cands.size() < 2 || all[i].error <= cands[1].error) sk->c(all[i].error - p.max_err);
+            if (all[i].error == best) sk->p(edge[i]);
+        }
+    }
     if (cands.empty()) return false;
-    if (sk && cands.size() > 1) sk->c(cands[1].error - cands[0].error);
+    if (runner_up && gap && cands.size() > 1) {
+        *runner_up = cands[1].depth;
+        *gap = cands[1].error - cands[0].error;
+    }
     *out = cands[0].depth;
+    if (best_h) *best_h = cands[0].h;
     return true;
+}
+
+// FNV-1a step over one 32-bit word (diagnostic signatures).
+inline uint32_t fnv1a(uint32_t h, uint32_t v) {
+    for (int b = 0; b < 4; ++b) {
+        h ^= (v >> (8 * b)) & 0xffu;
+        h *= 16777619u;
+    }
+    return h;
 }
 
 // reciprocal_depth_agrees, dense_legacy.rs:951-957.
@@ -393,9 +424,11 @@ bool reciprocal_agrees(double expected, double estimated, double max_rel, Slack 
 // that contributed a position.
 std::vector<V3> reciprocal_observations(V3 position, const Cam &rc, uint32_t rw, uint32_t rh,
                                         const uint8_t *rl, const std::vector<SourceView> &srcs,
-                                        double f, const Params &p, uint32_t *mask, Slack *sk) {
+                                        double f, const Params &p, uint32_t *mask, Slack *sk,
+                                        uint32_t *signature = nullptr) {
     std::vector<V3> out;
     *mask = 0;
+    if (signature) *signature = 2166136261u;
     double xr, yr, dr;
     if (!project(rc, position, rw, rh, f, &xr, &yr, &dr)) return out;
     for (size_t s = 0; s < srcs.size(); ++s) {
@@ -408,11 +441,21 @@ std::vector<V3> reciprocal_observations(V3 position, const Cam &rc, uint32_t rw,
         if (direct > p.max_err) continue;
         double xs, ys, zs;
         if (!project(src.cam, position, src.w, src.h, f, &xs, &ys, &zs)) continue;
-        double recip;
+        double recip, runner_up = 0.0, gap = kInf;
+        int rh_best = -1;
         if (!single_view_depth(src.cam, rc, src.luma.data(), rl, src.w, src.h, rw, rh, xs, ys,
-                               src.min_depth, src.max_depth, f, p, &recip, sk))
+                               src.min_depth, src.max_depth, f, p, &recip, sk, &runner_up, &gap,
+                               &rh_best))
             continue;
-        if (!reciprocal_agrees(zs, recip, p.recip_rel, sk)) continue;
+        if (signature) *signature = fnv1a(fnv1a(*signature, (uint32_t)s), (uint32_t)rh_best);
+        const bool agrees = reciprocal_agrees(zs, recip, p.recip_rel, sk);
+        if (sk && std::isfinite(gap)) {
+            // A reverse-sweep tie decides the outcome only if the runner-up's depth would
+            // flip the agreement test; otherwise it only moves the observation.
+            if (reciprocal_agrees(zs, runner_up, p.recip_rel, nullptr) != agrees) sk->c(gap);
+            else sk->q(gap);
+        }
+        if (!agrees) continue;
         V3 pos = unproject(src.cam, xs, ys, recip, src.w, src.h, f);
         if (!finite3(pos)) continue;
         out.push_back(pos);
@@ -467,7 +510,7 @@ struct SampleOut {
     int status;
     int best_h, best_support, second_h;
     double best_err, second_err;
-    uint32_t recip_mask;
+    uint32_t recip_mask, recip_sig;
     size_t recip_count, fused_obs, fused_rejected;
     vt3d_point3 point;
     vt3d_point_attributes attr;
@@ -517,6 +560,7 @@ struct Sweep {
         o->best_support = 0;
         o->best_err = o->second_err = kInf;
         o->recip_mask = 0;
+        o->recip_sig = 0;
         o->recip_count = o->fused_obs = o->fused_rejected = 0;
         struct Cand {
             double depth;
@@ -529,18 +573,29 @@ struct Sweep {
         cands.reserve(p.D);
         std::vector<double> errs;
         errs.reserve(srcs.size());
+        // Diagnostics: every (hypothesis, source) patch's validity, cost and edge distance.
+        struct PatchInfo {
+            bool valid;
+            double cost, px;
+        };
+        std::vector<PatchInfo> info(p.diag ? (size_t)p.D * srcs.size() : 0);
+        std::vector<size_t> support_of(p.diag ? (size_t)p.D : 0, 0);
         for (int hyp = 0; hyp < p.D; ++hyp) {
             double depth = hypothesis_depth(hyp, p.D, search_min, search_max);
             V3 position = unproject(ref, x, y, depth, w, h, f);
             errs.clear();
-            for (const SourceView &src : srcs) {
-                double e;
-                if (!patch_error(ref, src.cam, ref_luma.data(), src.luma.data(), w, h, src.w,
-                                 src.h, x, y, depth, f, p.cost_kind, p.radius, &e, skp))
-                    continue;
-                if (skp) skp->c(e - p.max_err);
+            for (size_t si = 0; si < srcs.size(); ++si) {
+                const SourceView &src = srcs[si];
+                double e = kInf;
+                Slack local;
+                bool valid = patch_error(ref, src.cam, ref_luma.data(), src.luma.data(), w, h,
+                                         src.w, src.h, x, y, depth, f, p.cost_kind, p.radius, &e,
+                                         skp ? &local : nullptr);
+                if (skp) info[(size_t)hyp * srcs.size() + si] = {valid, e, local.px};
+                if (!valid) continue;
                 if (e <= p.max_err) errs.push_back(e);
             }
+            if (skp) support_of[hyp] = errs.size();
             if (errs.empty()) continue;
             sort_f64(errs);
             cands.push_back({depth, position, errs.size(), quantile(errs.data(), errs.size(), 0.5),
@@ -552,6 +607,30 @@ struct Sweep {
             if (a.error != b.error) return a.error < b.error;
             return a.depth < b.depth;
         });
+        if (skp && !cands.empty()) {
+            // Only flips that can reorder the top two count: a patch's photometric gate or
+            // validity can move its hypothesis' support by one, which matters for the winner,
+            // the comparable second, and any hypothesis within one of the winner's support.
+            const size_t sup_b = cands[0].support;
+            const int h_second = cands.size() > 1 ? cands[1].h : -1;
+            for (int hyp = 0; hyp < p.D; ++hyp) {
+                if (!(hyp == cands[0].h || hyp == h_second || support_of[hyp] + 1 >= sup_b))
+                    continue;
+                for (size_t si = 0; si < srcs.size(); ++si) {
+                    const PatchInfo &pi = info[(size_t)hyp * srcs.size() + si];
+                    if (pi.valid) skp->c(pi.cost - p.max_err);
+                    skp->p(pi.px);
+                }
+            }
+            // Order among same-support candidates: the winner/second and second/third gaps.
+            for (size_t i = 0; i + 1 < cands.size() && i < 2; ++i)
+                if (cands[i + 1].support == sup_b) skp->c(cands[i + 1].error - cands[i].error);
+        } else if (skp) {
+            for (const PatchInfo &pi : info) {
+                if (pi.valid) skp->c(pi.cost - p.max_err);
+                skp->p(pi.px);
+            }
+        }
         o->slack = sk;
         if (cands.empty()) {
             o->status = VT3D_SAMPLE_NO_CANDIDATE;
@@ -571,7 +650,6 @@ struct Sweep {
         if (second) {
             o->second_h = second->h;
             o->second_err = second->error;
-            if (skp) skp->c(second->error - best.error);
         }
         if (best.error > p.max_err) {
             o->status = VT3D_SAMPLE_REJECT_COST;
@@ -594,7 +672,7 @@ struct Sweep {
         }
         uint32_t mask = 0;
         std::vector<V3> recip = reciprocal_observations(best.position, ref, w, h, ref_luma.data(),
-                                                        srcs, f, p, &mask, skp);
+                                                        srcs, f, p, &mask, skp, &o->recip_sig);
         o->recip_mask = mask;
         if (recip.empty()) {
             o->status = VT3D_SAMPLE_REJECT_RECIPROCAL;
@@ -958,8 +1036,10 @@ vt3d_dense_result *plane_sweep(const vt3d_frame *frames, size_t n_frames,
                 d.second_cost = (float)o.second_err;
                 d.reciprocal_mask = o.recip_mask;
                 d.fused_observations = (uint32_t)o.fused_obs;
+                d.reciprocal_signature = o.recip_sig;
+                d.pad = 0;
                 diags.push_back(d);
-                slacks.push_back({o.slack.cost, o.slack.px, o.slack.rel});
+                slacks.push_back({o.slack.cost, o.slack.px, o.slack.rel, o.slack.pos});
             }
             if (o.status >= VT3D_SAMPLE_REJECT_RECIPROCAL) ++checked;
             if (o.status == VT3D_SAMPLE_REJECT_RECIPROCAL) ++rejected;

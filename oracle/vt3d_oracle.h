@@ -20,13 +20,16 @@ extern "C" {
 #endif
 
 /* Per-sample decision slack: how far each decision that shaped the sample's outcome was
- * from flipping. cost: cost units (photometric gate, best/second ties, ambiguity,
- * reciprocal texture gate, reverse-sweep ties). px: source pixels (bilinear bounds of a
- * patch). rel: relative units (reciprocal agreement, fusion radius). */
+ * from flipping. cost: cost units (photometric gates and best/second ties that can reorder
+ * the top two candidates, ambiguity, reciprocal texture gate, reverse-sweep ties that flip
+ * the agreement test). px: source pixels (bilinear bounds of a patch). rel: relative units
+ * (reciprocal agreement, fusion radius). pos: cost units, reverse-sweep ties that only move
+ * a reciprocal observation by one hypothesis step (and with it the fused point). */
 typedef struct vt3d_oracle_slack {
     double cost;
     double px;
     double rel;
+    double pos;
 } vt3d_oracle_slack;
 
 /* Same contract as vt3d_dense_plane_sweep. threads > 1 runs grid rows on OpenMP threads

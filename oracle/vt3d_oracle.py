@@ -20,7 +20,7 @@ sys.path.insert(0, os.path.join(_REPO, "video-to-3d_amd"))
 from vt3d_dense import abi  # noqa: E402  (shared data contract)
 
 LIBRARY_PATH = os.path.join(_HERE, "libvt3d_oracle.so")
-SLACK_DTYPE = np.dtype([("cost", "<f8"), ("px", "<f8"), ("rel", "<f8")])
+SLACK_DTYPE = np.dtype([("cost", "<f8"), ("px", "<f8"), ("rel", "<f8"), ("pos", "<f8")])
 
 _lib = None
 

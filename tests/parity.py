@@ -15,6 +15,9 @@ import numpy as np
 # Tolerances. ZSAD costs are intensity differences (0..255): FP32 bilinear and sums carry
 # about 1e-5 absolute error. ZNCC costs are in [0, 2] with about 1e-6 error.
 TAU_COST = {0: 1.0e-3, 1: 1.0e-4}
+# Reverse-sweep ties that only move a reciprocal observation: the FP32 cost error must
+# exceed the gap to flip them, so a tighter bound (a few times the cost error) applies.
+TAU_POS = {0: 2.0e-4, 1: 2.0e-5}
 TAU_PX = 1.0e-3
 TAU_REL = 1.0e-5
 POINT_REL = 1.0e-4
@@ -40,7 +43,16 @@ def compare(gpu, oracle, slack, cost_kind: int, max_unexplained: int = 0):
     # tie (a reverse-sweep tie moves a reciprocal observation without changing any index
     # reported here).
     same = (agree & ~near & (gs["status"] == 6)
-            & (gs["reciprocal_mask"] == os_["reciprocal_mask"]))
+            & (gs["reciprocal_mask"] == os_["reciprocal_mask"])
+            & (gs["reciprocal_signature"] == os_["reciprocal_signature"]))
+    # Reverse-sweep ties below the cost error may pick a neighbouring hypothesis; away from
+    # such ties the reverse choices must agree.
+    sig_same = gs["reciprocal_signature"] == os_["reciprocal_signature"]
+    clear = agree & ~near & (slack["pos"] >= TAU_POS[cost_kind]) & (gs["status"] >= 4)
+    report["reverse_agree_clear"] = float(sig_same[clear].mean()) if clear.any() else 1.0
+    report["reverse_agree_all"] = float(sig_same[agree & (gs["status"] >= 4)].mean()) \
+        if (agree & (gs["status"] >= 4)).any() else 1.0
+    assert report["reverse_agree_clear"] == 1.0, report
     report["points_compared"] = int(same.sum())
     gp = _points_by_site(gpu)
     op = _points_by_site(oracle)

@@ -78,13 +78,14 @@ struct Decision {
 struct Recip {
     double pos[3];
     int ok;
-    int pad;
+    int rh;  // reverse-sweep hypothesis, -1 if the reverse sweep did not run or found none
 };
 
 // Final record of one sample, consumed by the order-preserving compaction.
 struct SampleOut {
     int status;
     uint32_t recip_mask;
+    uint32_t recip_sig;
     uint32_t fused_obs;
     uint32_t recip_count;
     uint32_t fused_rejected;
@@ -94,7 +95,6 @@ struct SampleOut {
     float px, py, pz;
     float error;
     int support;
-    int pad;
 };
 
 // Kernel arguments shared by the per-solve kernels.

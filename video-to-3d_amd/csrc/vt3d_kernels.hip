@@ -769,7 +769,7 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const int s = (int)blockIdx.y;
     if (i >= n) return;
-    Recip out{{0.0, 0.0, 0.0}, 0, 0};
+    Recip out{{0.0, 0.0, 0.0}, 0, -1};
     const Decision &d = a.decision[i];
     const SourceDev &src = a.src[s];
     const int D = sd.D;
@@ -813,6 +813,7 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
             }
         }
         ok = best_h >= 0;
+        out.rh = best_h;
         if (ok) {
             const double dh = a.depth_r[(size_t)s * D + best_h];
             // reciprocal_depth_agrees, dense_legacy.rs:951-957.
@@ -830,6 +831,16 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
 
 // ---- k_fuse --------------------------------------------------------------------------------
 
+// FNV-1a step over one 32-bit word (diagnostic signatures).
+__device__ __forceinline__ uint32_t fnv1a(uint32_t h, uint32_t v) {
+#pragma unroll
+    for (int b = 0; b < 4; ++b) {
+        h ^= (v >> (8 * b)) & 0xffu;
+        h *= 16777619u;
+    }
+    return h;
+}
+
 // fuse_depth_observations (dense_legacy.rs:859-884) and the confidence and attributes of
 // dense_legacy.rs:387-410, in FP64; counters for DenseStats (dense_legacy.rs:363-385).
 __global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag) {
@@ -846,9 +857,10 @@ __global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag)
         if (d.status == 0) {
             const double radius = fmax(fabs(d.depth) * sd.fusion_rel, 1.0e-4);
             double sx = d.pos[0], sy = d.pos[1], sz = d.pos[2];
-            uint32_t obs = 1, rej = 0, cnt = 0, mask = 0;
+            uint32_t obs = 1, rej = 0, cnt = 0, mask = 0, sig = 2166136261u;
             for (int s = 0; s < sd.S; ++s) {
                 const Recip r = a.recip[(size_t)s * a.max_samples + i];
+                if (r.rh >= 0) sig = fnv1a(fnv1a(sig, (uint32_t)s), (uint32_t)r.rh);
                 if (!r.ok) continue;
                 ++cnt;
                 mask |= 1u << s;
@@ -865,6 +877,7 @@ __global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag)
                 ++obs;
             }
             o.recip_mask = mask;
+            o.recip_sig = sig;
             o.recip_count = cnt;
             if (cnt == 0) {
                 o.status = VT3D_SAMPLE_REJECT_RECIPROCAL;
@@ -914,6 +927,8 @@ __global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag)
             g.second_cost = d.second_cost;
             g.reciprocal_mask = o.recip_mask;
             g.fused_observations = o.fused_obs;
+            g.reciprocal_signature = o.recip_sig;
+            g.pad = 0;
             diag[i] = g;
         }
     }

@@ -103,7 +103,8 @@ class SampleDiag(C.Structure):
                 ("best_hypothesis", C.c_int32), ("best_support", C.c_int32),
                 ("second_hypothesis", C.c_int32), ("best_cost", C.c_float),
                 ("second_cost", C.c_float), ("reciprocal_mask", C.c_uint32),
-                ("fused_observations", C.c_uint32)]
+                ("fused_observations", C.c_uint32), ("reciprocal_signature", C.c_uint32),
+                ("pad", C.c_uint32)]
 
 
 class DenseResult(C.Structure):
@@ -130,7 +131,8 @@ ATTR_DTYPE = np.dtype([("reciprocal_consistency", "<f4"), ("depth_margin", "<f4"
 SAMPLE_DTYPE = np.dtype([("x", "<u4"), ("y", "<u4"), ("status", "<i4"), ("best_hypothesis", "<i4"),
                          ("best_support", "<i4"), ("second_hypothesis", "<i4"),
                          ("best_cost", "<f4"), ("second_cost", "<f4"),
-                         ("reciprocal_mask", "<u4"), ("fused_observations", "<u4")])
+                         ("reciprocal_mask", "<u4"), ("fused_observations", "<u4"),
+                         ("reciprocal_signature", "<u4"), ("pad", "<u4")])
 
 assert C.sizeof(Point3) == POINT_DTYPE.itemsize
 assert C.sizeof(SampleDiag) == SAMPLE_DTYPE.itemsize
