@@ -1349,6 +1349,10 @@ int32_t vt3d_oracle_single_view_depth(const vt3d_camera *rc, const vt3d_camera *
                : 0;
 }
 
+double vt3d_oracle_hypothesis_depth(int32_t h, int32_t D, double mn, double mx) {
+    return hypothesis_depth(h, D, mn, mx);
+}
+
 int32_t vt3d_oracle_reciprocal_agrees(double expected, double estimated, double max_rel) {
     return reciprocal_agrees(expected, estimated, max_rel, nullptr) ? 1 : 0;
 }

@@ -105,6 +105,8 @@ int32_t vt3d_oracle_single_view_depth(const vt3d_camera *reference_camera,
                                       double y, double min_depth, double max_depth,
                                       double focal, const vt3d_sweep_params *params,
                                       double *depth);
+/* Inverse-depth hypothesis schedule (dense_legacy.rs:285-293). */
+double vt3d_oracle_hypothesis_depth(int32_t h, int32_t D, double min_depth, double max_depth);
 /* reciprocal_depth_agrees (dense_legacy.rs:951-957) with the given threshold. */
 int32_t vt3d_oracle_reciprocal_agrees(double expected, double estimated, double max_rel);
 /* fuse_depth_observations (dense_legacy.rs:859-884). reciprocal: n x 3. */

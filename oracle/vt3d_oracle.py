@@ -90,6 +90,8 @@ def library() -> C.CDLL:
                                                   C.c_double, C.c_double, C.c_double,
                                                   P(abi.SweepParams), P(C.c_double)]
     lib.vt3d_oracle_single_view_depth.restype = C.c_int32
+    lib.vt3d_oracle_hypothesis_depth.argtypes = [C.c_int32, C.c_int32, C.c_double, C.c_double]
+    lib.vt3d_oracle_hypothesis_depth.restype = C.c_double
     lib.vt3d_oracle_reciprocal_agrees.argtypes = [C.c_double, C.c_double, C.c_double]
     lib.vt3d_oracle_reciprocal_agrees.restype = C.c_int32
     lib.vt3d_oracle_fuse.argtypes = [P(C.c_double), P(C.c_double), C.c_size_t, C.c_double,

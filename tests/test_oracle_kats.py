@@ -430,3 +430,75 @@ def test_reference_patch_remaps_frames(oracle):
     none, _ = oracle.reference_patch(c1.frames, c1.cameras, c1.sparse[:5], c1.focal, 2,
                                      sweep=oracle.params(st))
     assert none is None
+
+
+# ---- hypothesis schedule (dense_legacy.rs:285-293) ---------------------------------------------
+
+@pytest.mark.parametrize("D", [1, 2, 24, 128])
+def test_hypothesis_schedule_is_linear_in_inverse_depth(oracle, D):
+    lib = oracle.library()
+    mn, mx = 0.5, 8.0
+    d = np.array([lib.vt3d_oracle_hypothesis_depth(h, D, mn, mx) for h in range(D)])
+    assert d[0] == pytest.approx(mn, rel=1e-15)
+    if D == 1:
+        return
+    assert d[-1] == pytest.approx(mx, rel=1e-14)
+    assert np.all(np.diff(d) > 0), "depth increases with the hypothesis index"
+    inv = 1.0 / d
+    assert np.allclose(np.diff(inv, 2), 0.0, atol=1e-12), "inverse depth is affine in h"
+
+
+# ---- winner-take-all and ambiguity, brute force (dense_legacy.rs:297-362) -----------------------
+
+def _round_half_away(v):
+    return int(math.floor(v + 0.5))
+
+
+def test_wta_and_ambiguity_match_a_brute_force_sort(oracle):
+    """Candidates rebuilt from per-patch costs and sorted with Python's stable sort by the
+    reference's key (support desc, error asc, depth asc) give the oracle's choices."""
+    lib = oracle.library()
+    pf = scenes.plane_fixture()
+    W, H, f = 68, 48, pf.focal
+    st = {"preset": "reference", "surface_completion_passes": 0, "emit_samples": 1}
+    res, _ = oracle.plane_sweep(pf.frames, pf.cameras, pf.sparse, f, sweep=oracle.params(st))
+    cams = [cam(c.R, c.t, i) for i, c in enumerate(pf.cameras)]
+    mn, mx = C.c_double(), C.c_double()
+    assert lib.vt3d_oracle_depth_search_bounds(C.byref(cams[0]), oracle.f64ptr(pf.sparse),
+                                               len(pf.sparse), W, H, f, C.byref(mn), C.byref(mx))
+    D = 24
+    depths = [lib.vt3d_oracle_hypothesis_depth(h, D, mn.value, mx.value) for h in range(D)]
+    lumas = [np.ascontiguousarray(fr) for fr in pf.frames]
+    checked = 0
+    for smp in res.samples[::3]:
+        x, y = float(smp["x"]), float(smp["y"])
+        cands = []
+        for h, d in enumerate(depths):
+            errs = []
+            for si in (1, 2):
+                e = C.c_double()
+                if lib.vt3d_oracle_patch_error(C.byref(cams[0]), C.byref(cams[si]),
+                                               oracle.u8ptr(lumas[0]), oracle.u8ptr(lumas[si]),
+                                               W, H, W, H, x, y, d, f, 0, 1, C.byref(e)):
+                    if e.value <= 18.0:
+                        errs.append(e.value)
+            if errs:
+                errs.sort()
+                cands.append((len(errs), errs[_round_half_away((len(errs) - 1) * 0.5)], d, h))
+        cands.sort(key=lambda c: (-c[0], c[1], c[2]))
+        if not cands:
+            assert smp["status"] == 1
+            continue
+        best = cands[0]
+        assert smp["best_hypothesis"] == best[3]
+        assert smp["best_support"] == best[0]
+        assert smp["best_cost"] == pytest.approx(best[1], abs=1e-6)
+        second = next((c for c in cands[1:] if c[0] == best[0]), None)
+        assert smp["second_hypothesis"] == (second[3] if second else -1)
+        margin = second[1] - best[1] if second else math.inf
+        if margin < 1.0:
+            assert smp["status"] == 3
+        else:
+            assert smp["status"] >= 4
+        checked += 1
+    assert checked > 20
