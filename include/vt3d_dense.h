@@ -285,6 +285,48 @@ int32_t vt3d_session_solve(vt3d_session *session, const vt3d_camera *cameras,
                            void *poll_user, vt3d_dense_result **out);
 int32_t vt3d_session_last_timing(const vt3d_session *session, vt3d_timing *timing);
 
+/* Launch-shape tuning of a session (vt3d_dense/autotune.py searches it; zero fields keep the
+ * defaults). tile_width: grid sites per tile row of the tiled sweep (16, 32 or 64; default
+ * 32). target_waves: hypothesis blocks are sized so that a launch holds at least this many
+ * waves (default 4096, four per SIMD of a 256-CU MI355X). */
+typedef struct vt3d_tuning {
+    int32_t tile_width;
+    int32_t target_waves;
+} vt3d_tuning;
+int32_t vt3d_session_set_tuning(vt3d_session *session, const vt3d_tuning *tuning);
+
+/* ---- batches: many solves in one pipeline ------------------------------------------------ */
+/* No counterpart in the reference (which solves one reference at a time on the caller's
+ * thread). Uneven solves -- different frame sizes, sources, hypotheses and patch sizes --
+ * share each stage's launch; with use_graph the batch's launches are captured in a HIP graph
+ * on the first run and replayed afterwards. */
+
+typedef struct vt3d_request {
+    const vt3d_camera *cameras;
+    size_t n_cameras;
+    const double *sparse_xyz;
+    size_t n_sparse;
+    const uint64_t *frame_map;       /* request frame index -> session frame index */
+    size_t n_frames;
+    double focal;
+    int64_t reference_camera_index;  /* >= 0: dense.rs:674-747 remap; < 0: dense_legacy.rs:149 */
+    int32_t fine_grid;
+    int32_t pad;
+    const vt3d_sweep_params *params; /* NULL: vt3d_sweep_params_reference() */
+} vt3d_request;
+
+typedef struct vt3d_batch vt3d_batch;
+
+/* Host selection, FP64 tables and device layout of a batch on the session's frames. The
+ * session must outlive the batch; batches of one session run one at a time. */
+int32_t vt3d_batch_create(vt3d_session *session, const vt3d_request *requests, size_t n,
+                          int32_t use_graph, vt3d_batch **out);
+/* Runs the batch. results[i] receives request i's result (free each with
+ * vt3d_dense_result_free), or NULL where the reference remap returns None. */
+int32_t vt3d_batch_run(vt3d_batch *batch, vt3d_dense_result **results);
+int32_t vt3d_batch_last_timing(const vt3d_batch *batch, vt3d_timing *timing);
+void vt3d_batch_destroy(vt3d_batch *batch);
+
 /* Last device or validation error message of the calling thread ("" if none). */
 const char *vt3d_last_error(void);
 

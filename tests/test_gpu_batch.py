@@ -1,0 +1,111 @@
+"""Batches: uneven solves in one kernel pipeline (and its captured HIP graph) give the same
+bits as each solve run alone."""
+import numpy as np
+import pytest
+
+from vt3d_dense import scenes
+
+pytestmark = pytest.mark.gpu
+
+
+def _same(a, b, what):
+    assert (a is None) == (b is None), what
+    if a is None:
+        return
+    assert a.skip_code == b.skip_code, what
+    assert a.stats == b.stats, what
+    assert a.points.tobytes() == b.points.tobytes(), what
+    assert a.grid_sites.tobytes() == b.grid_sites.tobytes(), what
+    assert a.point_attributes.tobytes() == b.point_attributes.tobytes(), what
+    if a.samples is not None or b.samples is not None:
+        assert a.samples.tobytes() == b.samples.tobytes(), what
+
+
+def _mix(product):
+    """Six batched-config keyframes (three frame sizes, 8 sources, 96 hypotheses, 7x7 ZNCC),
+    the plain CPU-size config (4 sources, 64 hypotheses), the 2-source ZSAD plane fixture on
+    the plain-sweep entry, a remap that returns None and a skipped solve."""
+    c3 = scenes.config_c3(count=6)
+    c1 = scenes.config_c1()
+    pf = scenes.plane_fixture()
+    jobs = []
+    for p in c3 + [c1]:
+        jobs.append(dict(frames=p.frames, cameras=p.cameras, sparse_points=p.sparse,
+                         focal=p.focal, reference_camera_index=p.reference,
+                         settings=dict(p.settings)))
+    ref = {"preset": "reference"}
+    jobs.append(dict(frames=pf.frames, cameras=pf.cameras, sparse_points=pf.sparse,
+                     focal=pf.focal, reference_camera_index=-1, settings=ref))
+    jobs.append(dict(frames=c1.frames, cameras=c1.cameras, sparse_points=c1.sparse[:5],
+                     focal=c1.focal, reference_camera_index=c1.reference,
+                     settings=dict(c1.settings)))
+    jobs.append(dict(frames=pf.frames, cameras=pf.cameras[:1], sparse_points=pf.sparse,
+                     focal=pf.focal, reference_camera_index=-1, settings=ref))
+    for j in jobs:
+        j["settings"].update(surface_completion_passes=0, emit_samples=1, storage="f16")
+        j["sweep"] = product.params(j["settings"])
+    return jobs
+
+
+def _alone(product, j):
+    if j["reference_camera_index"] >= 0:
+        return product.reconstruct_reference_patch(j["frames"], j["cameras"], j["sparse_points"],
+                                                   j["focal"], j["reference_camera_index"],
+                                                   sweep=j["sweep"])
+    return product.estimate_depth_points_with_control(j["frames"], j["cameras"],
+                                                      j["sparse_points"], j["focal"],
+                                                      sweep=j["sweep"])
+
+
+@pytest.fixture(scope="module")
+def mix(product):
+    jobs = _mix(product)
+    alone = [_alone(product, j) for j in jobs]
+    return jobs, alone
+
+
+def _batch(product, jobs, use_graph, tuning=None):
+    frames, maps = product.pack_frames([j["frames"] for j in jobs])
+    sess = product.Session(0)
+    sess.upload(frames, "f16")
+    if tuning:
+        sess.set_tuning(**tuning)
+    reqs = [dict(j, frame_map=m) for j, m in zip(jobs, maps)]
+    return sess, sess.batch(reqs, use_graph=use_graph)
+
+
+@pytest.mark.parametrize("tuning", [None, {"tile_width": 16, "target_waves": 16384},
+                                    {"tile_width": 64, "target_waves": 1}])
+def test_batch_matches_each_solve_alone(product, mix, tuning):
+    jobs, alone = mix
+    sess, b = _batch(product, jobs, False, tuning)
+    try:
+        got = b.run()
+        assert len(got) == len(jobs)
+        for i, (g, a) in enumerate(zip(got, alone)):
+            _same(g, a, f"request {i}")
+        assert got[-2] is None and got[-1].skip_code == 2
+        assert sum(len(g.points) for g in got[:7]) > 1000
+        t = b.timing()
+        assert t.units == sum(g.stats["sampled_pixels"] * g.stats["depth_hypotheses"]
+                              for g in got if g is not None)
+        assert t.sweep_ms > 0 and t.total_ms > 0
+    finally:
+        b.close()
+        sess.close()
+
+
+def test_graph_replays_match_eager(product, mix):
+    jobs, alone = mix
+    sess, b = _batch(product, jobs, True)
+    try:
+        for replay in range(3):
+            got = b.run()
+            for i, (g, a) in enumerate(zip(got, alone)):
+                _same(g, a, f"replay {replay} request {i}")
+            # eager work of the caller between replays
+            one = _alone(product, jobs[0])
+            _same(one, alone[0], "eager between replays")
+    finally:
+        b.close()
+        sess.close()

@@ -2,9 +2,10 @@
 //
 // The host side does what is small and sequential in the reference
 // (crates/video-to-3d-core/src/dense_legacy.rs:157-256: skip checks, reference choice,
-// landmark envelope, source selection), composes the FP64 homography tables, and drives the
-// kernel pipeline of vt3d_kernels.hip on one HIP stream per calling thread. Everything per
-// grid sample runs on the GPU; there is no CPU path for it.
+// landmark envelope, source selection), composes the FP64 homography tables, lays a batch of
+// solves out in device memory and drives the kernel pipeline of vt3d_kernels.hip on one HIP
+// stream. Everything per grid sample runs on the GPU; there is no CPU path for it. A single
+// solve is a batch of one.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -13,28 +14,26 @@
 #include <limits>
 #include <memory>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "vt3d_internal.h"
 
-#include "../../include/vt3d_dense.h"
-
 namespace vt3d {
 
-bool launch_sweep_or_recip(int storage, int cost, int radius, const Args &a, int smax, int S,
-                           uint32_t blocks, bool recip, const TileLaunch *tl, hipStream_t s);
-size_t tile_lds_bytes(int tile_w, int stride, int radius);
+bool launch_group(int storage, int cost, int radius, int smax, bool recip, bool tiled,
+                  size_t lds, const Args &a, hipStream_t s);
 bool launch_build_view(int storage, const uint8_t *rgba, uint32_t w, uint32_t h, uint8_t *luma,
                        void *quad, hipStream_t s);
 size_t texel_bytes(int storage);
-void launch_gate(const Args &a, uint32_t blocks, hipStream_t s);
-void launch_scan(const Args &a, uint32_t *n_out, hipStream_t s);
-void launch_compact_sites(const Args &a, uint32_t blocks, hipStream_t s);
-void launch_decide(const Args &a, uint32_t blocks, hipStream_t s);
-void launch_fuse(const Args &a, vt3d_sample_diag *diag, uint32_t blocks, hipStream_t s);
-void launch_compact_points(const Args &a, vt3d_point3 *points, vt3d_grid_site *sites,
-                           vt3d_point_attributes *attrs, float *errors, float *supports,
-                           float *fused_obs, uint32_t blocks, hipStream_t s);
+size_t tile_lds_bytes(int tile_w, int stride, int radius);
+void launch_gate(const Args &a, hipStream_t s);
+void launch_scan(const Args &a, int gate, hipStream_t s);
+void launch_problem_offsets(const Args &a, int which, hipStream_t s);
+void launch_compact_sites(const Args &a, hipStream_t s);
+void launch_decide(const Args &a, hipStream_t s);
+void launch_fuse(const Args &a, vt3d_sample_diag *diag, hipStream_t s);
+void launch_compact_points(const Args &a, const Outputs &o, hipStream_t s);
 
 namespace {
 
@@ -52,11 +51,15 @@ void set_error(const std::string &msg) { g_error = msg; }
     } while (0)
 
 constexpr size_t kMinVisibleSparsePoints = 8;  // dense_legacy.rs:8
+constexpr size_t kMaxTileLds = 64 * 1024;      // LDS budget of the tiled sweep's region
 
 // Grow-only device buffer.
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
     int ensure(size_t bytes) {
         if (bytes <= cap) return VT3D_OK;
         if (p) (void)hipFree(p);
@@ -78,11 +81,48 @@ struct DevBuf {
     }
 };
 
+// Grow-only pinned host buffer.
+struct HostBuf {
+    void *p = nullptr;
+    size_t cap = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf &) = delete;
+    HostBuf &operator=(const HostBuf &) = delete;
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return VT3D_OK;
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        cap = 0;
+        const size_t want = std::max<size_t>(bytes, 256);
+        hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
+        if (e != hipSuccess) {
+            set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e));
+            p = nullptr;
+            return VT3D_E_DEVICE;
+        }
+        cap = want;
+        return VT3D_OK;
+    }
+    template <class T> T *as() const { return reinterpret_cast<T *>(p); }
+    ~HostBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+};
+
 struct DevView {
     DevBuf rgba, luma, quad;
     uint32_t w = 0, h = 0;
     int storage = -1;
     bool valid = false;
+};
+
+// Device buffers of one batch pipeline.
+struct Workspace {
+    DevBuf prob, src, Hp, Hr, depth_p, depth_r, block_problem;
+    DevBuf site_flag, samples, site_map, partial, decision, recip, out;
+    DevBuf block_counts, block_offsets, meta;
+    DevBuf points, sites, attrs, errors, supports, fobs, diag;
+    HostBuf meta_host;
 };
 
 }  // namespace
@@ -93,12 +133,10 @@ struct vt3d_session {
     hipStream_t stream = nullptr;
     int storage = VT3D_STORAGE_F32;
     std::vector<std::unique_ptr<vt3d::DevView>> views;  // uploaded frames by frame index
-    // per-solve buffers
-    vt3d::DevBuf solve, src, Hp, Hr, depth_p, depth_r, site_flag, samples, site_map, n_samples, partial,
-        decision, recip, out, block_counts, block_offsets, totals, points, sites, attrs, errors,
-        supports, fobs, diag;
+    vt3d::Workspace ws;                                  // one-shot and session solves
     hipEvent_t ev[8] = {};
     vt3d_timing timing{};
+    vt3d_tuning tuning{32, 4096};
 };
 
 namespace vt3d {
@@ -185,7 +223,6 @@ double hyp_depth(int h, int D, double mn, double mx) {
     return 1.0 / ((1.0 / mn) * (1.0 - t) + (1.0 / mx) * t);
 }
 
-// 3x3 row-major helpers for homography composition.
 void mat_mul(const double *A, const double *B, double *C) {
     for (int r = 0; r < 3; ++r)
         for (int c = 0; c < 3; ++c)
@@ -281,7 +318,7 @@ struct Control {
     }
 };
 
-// Frame access for one solve: dims of every frame, and device views for those swept.
+// Frame access of one solve: dims of every frame, and device views for those swept.
 struct FrameSource {
     const vt3d_frame *host;       // host frames (one-shot API), or null
     vt3d_session *session;        // device-resident frames (session API)
@@ -326,19 +363,49 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     return ms;
 }
 
-// estimate_depth_points_with_control (dense_legacy.rs:149-470) on the GPU. `frames` gives
-// dims (and host pixels in the one-shot API); camera frame indices index `frames`.
-int plane_sweep(vt3d_session *s, FrameSource &frames, const vt3d_camera *cams_in, size_t n_cams,
-                const double *sparse_xyz, size_t n_sparse, double f, bool fine_grid,
-                const vt3d_sweep_params &p, Control &ctl, vt3d_dense_result **out) {
-    *out = nullptr;
-    s->timing = vt3d_timing{};
-    if (frames.count() == 0) return (*out = skipped(VT3D_SKIP_NO_FRAMES)) ? VT3D_OK : VT3D_E_INVALID;
-    if (n_cams < 2) return (*out = skipped(VT3D_SKIP_FEW_CAMERAS)) ? VT3D_OK : VT3D_E_INVALID;
-    if (n_sparse < kMinVisibleSparsePoints)
-        return (*out = skipped(VT3D_SKIP_FEW_LANDMARKS)) ? VT3D_OK : VT3D_E_INVALID;
+// One solve after host selection: what the device needs and what the result reports.
+struct Prepared {
+    int state = 0;                    // 0: device work; 1: finished on the host (skip / None)
+    int none = 0;                     // 1: the reference remap returns None
+    vt3d_dense_result *host_result = nullptr;
+    vt3d_sweep_params p{};
+    Cam ref{};
+    uint32_t w = 0, h = 0;
+    double focal = 0.0;
+    std::vector<Cam> src_cams;
+    std::vector<double> src_min, src_max;
+    std::vector<uint64_t> src_frames_out;  // reported source frame indices
+    uint64_t ref_frame_out = 0;
+    DevView *ref_view = nullptr;
+    std::vector<DevView *> src_views;
+    std::vector<std::unique_ptr<DevView>> owned;  // one-shot uploads
+    int S = 0, D = 0, R = 0;
+    uint32_t stride = 0, border = 0, nx = 0, ny = 0;
+    double sweep_min = 0.0, sweep_max = 0.0;
+    Prepared() = default;
+    Prepared(const Prepared &) = delete;
+    Prepared &operator=(const Prepared &) = delete;
+    ~Prepared() { vt3d_dense_result_free(host_result); }
+};
+
+int finish_host(Prepared &pr, int code) {
+    pr.state = 1;
+    pr.host_result = skipped(code);
+    return pr.host_result ? VT3D_OK : VT3D_E_DEVICE;
+}
+
+// Host part of estimate_depth_points_with_control (dense_legacy.rs:149-282): skip checks,
+// reference, envelope, sources, grid; the device views of the swept frames.
+int prepare_sweep(vt3d_session *s, FrameSource &frames, const vt3d_camera *cams_in, size_t n_cams,
+                  const double *sparse_xyz, size_t n_sparse, double f, bool fine_grid,
+                  const vt3d_sweep_params &p, Control &ctl, Prepared &pr) {
+    pr.p = p;
+    pr.focal = f;
+    if (frames.count() == 0) return finish_host(pr, VT3D_SKIP_NO_FRAMES);
+    if (n_cams < 2) return finish_host(pr, VT3D_SKIP_FEW_CAMERAS);
+    if (n_sparse < kMinVisibleSparsePoints) return finish_host(pr, VT3D_SKIP_FEW_LANDMARKS);
     const uint32_t w = frames.w(0), h = frames.h(0);
-    if (w < 16 || h < 16) return (*out = skipped(VT3D_SKIP_SMALL_FRAMES)) ? VT3D_OK : VT3D_E_INVALID;
+    if (w < 16 || h < 16) return finish_host(pr, VT3D_SKIP_SMALL_FRAMES);
 
     std::vector<Cam> cams(n_cams);
     for (size_t i = 0; i < n_cams; ++i) cams[i] = to_cam(cams_in[i]);
@@ -363,17 +430,16 @@ int plane_sweep(vt3d_session *s, FrameSource &frames, const vt3d_camera *cams_in
             ref_depths = std::move(d);
         }
     }
-    if (ref_i < 0) return (*out = skipped(VT3D_SKIP_NO_REFERENCE)) ? VT3D_OK : VT3D_E_INVALID;
+    if (ref_i < 0) return finish_host(pr, VT3D_SKIP_NO_REFERENCE);
     const Cam ref = cams[ref_i];
     std::stable_sort(ref_depths.begin(), ref_depths.end());
     const double lower = quantile_sorted(ref_depths, 0.10), upper = quantile_sorted(ref_depths, 0.90);
     if (!std::isfinite(lower) || !std::isfinite(upper) || lower <= 0.0 || upper <= 0.0)
-        return (*out = skipped(VT3D_SKIP_NO_ENVELOPE)) ? VT3D_OK : VT3D_E_INVALID;
+        return finish_host(pr, VT3D_SKIP_NO_ENVELOPE);
     const double search_min = std::max(lower * 0.8, 1.0e-4);
     const double search_max = std::max(upper * 1.25, search_min * 1.2);
     const double median_depth = quantile_sorted(ref_depths, 0.50);
-    if (ref.frame_index >= frames.count())
-        return (*out = skipped(VT3D_SKIP_NO_REFERENCE_FRAME)) ? VT3D_OK : VT3D_E_INVALID;
+    if (ref.frame_index >= frames.count()) return finish_host(pr, VT3D_SKIP_NO_REFERENCE_FRAME);
     if (frames.w(ref.frame_index) != w || frames.h(ref.frame_index) != h) {
         // The reference sizes its grid from frames[0] and reads the reference frame's pixels
         // (dense_legacy.rs:169-170, 238); differing sizes make Rust panic.
@@ -416,40 +482,43 @@ int plane_sweep(vt3d_session *s, FrameSource &frames, const vt3d_camera *cams_in
         return cams[a.cam].frame_index < cams[b.cam].frame_index;
     });
     if (picks.size() > (size_t)p.max_source_views) picks.resize((size_t)p.max_source_views);
-    if (picks.empty()) return (*out = skipped(VT3D_SKIP_NO_SOURCES)) ? VT3D_OK : VT3D_E_INVALID;
-    const int S = (int)picks.size();
-    const int D = p.depth_hypotheses;
-    const int R = p.patch_radius;
+    if (picks.empty()) return finish_host(pr, VT3D_SKIP_NO_SOURCES);
 
-    double sweep_min = search_min, sweep_max = search_max;
+    pr.ref = ref;
+    pr.w = w;
+    pr.h = h;
+    pr.S = (int)picks.size();
+    pr.D = p.depth_hypotheses;
+    pr.R = p.patch_radius;
+    pr.sweep_min = search_min;
+    pr.sweep_max = search_max;
     if (p.depth_min_override > 0.0 && p.depth_max_override > p.depth_min_override) {
-        sweep_min = p.depth_min_override;
-        sweep_max = p.depth_max_override;
+        pr.sweep_min = p.depth_min_override;
+        pr.sweep_max = p.depth_max_override;
     }
-    const uint32_t stride = p.grid_stride > 0 ? (uint32_t)p.grid_stride : grid_stride_rule(w, h, fine_grid);
-    const uint32_t border = (uint32_t)(R + 2);
-    const uint32_t nx = w > 2 * border ? (w - 2 * border + stride - 1) / stride : 0;
-    const uint32_t ny = h > 2 * border ? (h - 2 * border + stride - 1) / stride : 0;
+    pr.stride = p.grid_stride > 0 ? (uint32_t)p.grid_stride : grid_stride_rule(w, h, fine_grid);
+    pr.border = (uint32_t)(pr.R + 2);
+    pr.nx = w > 2 * pr.border ? (w - 2 * pr.border + pr.stride - 1) / pr.stride : 0;
+    pr.ny = h > 2 * pr.border ? (h - 2 * pr.border + pr.stride - 1) / pr.stride : 0;
+    pr.ref_frame_out = ref.frame_index;
+    for (const Pick &pk : picks) {
+        pr.src_cams.push_back(cams[pk.cam]);
+        pr.src_min.push_back(pk.mn);
+        pr.src_max.push_back(pk.mx);
+        pr.src_frames_out.push_back(cams[pk.cam].frame_index);
+    }
 
     // Cancellation is polled once per grid row, in row order, before the rows are swept
     // (dense_legacy.rs:272-275); a cancel at any row returns the canceled skip.
-    for (uint32_t r = 0; r < ny; ++r)
-        if (ctl.is_canceled()) return (*out = skipped(VT3D_SKIP_CANCELED)) ? VT3D_OK : VT3D_E_INVALID;
-
-    VT3D_HIP(hipSetDevice(s->device));
-    hipStream_t st = s->stream;
-    for (int k = 0; k < 8; ++k)
-        if (!s->ev[k]) VT3D_HIP(hipEventCreate(&s->ev[k]));
-    VT3D_HIP(hipEventRecord(s->ev[0], st));
+    for (uint32_t r = 0; r < pr.ny; ++r)
+        if (ctl.is_canceled()) return finish_host(pr, VT3D_SKIP_CANCELED);
 
     // Device views of the reference and the sources.
-    std::vector<DevView *> views(1 + S);
-    std::vector<std::unique_ptr<DevView>> temp;
     auto view_of = [&](uint64_t fi, DevView **outv) -> int {
         if (frames.host) {
-            temp.emplace_back(new DevView());
-            int rc = upload_view(s, *temp.back(), frames.host[frames.map[fi]], p.storage);
-            *outv = temp.back().get();
+            pr.owned.emplace_back(new DevView());
+            int rc = upload_view(s, *pr.owned.back(), frames.host[frames.map[fi]], p.storage);
+            *outv = pr.owned.back().get();
             return rc;
         }
         DevView *v = s->views[frames.map[fi]].get();
@@ -461,273 +530,29 @@ int plane_sweep(vt3d_session *s, FrameSource &frames, const vt3d_camera *cams_in
         return VT3D_OK;
     };
     int rc;
-    if ((rc = view_of(ref.frame_index, &views[0]))) return rc;
-    for (int k = 0; k < S; ++k)
-        if ((rc = view_of(cams[picks[k].cam].frame_index, &views[1 + k]))) return rc;
-    VT3D_HIP(hipEventRecord(s->ev[1], st));
-
-    // Host tables (FP64).
-    SolveDev sd{};
-    sd.ref = ViewTex{views[0]->quad.p, views[0]->luma.as<uint8_t>(), w, h};
-    std::memcpy(sd.Rr, ref.R, sizeof(sd.Rr));
-    std::memcpy(sd.tr, ref.t, sizeof(sd.tr));
-    sd.focal = f;
-    sd.S = S;
-    sd.D = D;
-    sd.radius = R;
-    sd.cost_kind = p.cost_kind;
-    sd.ambiguity_kind = p.ambiguity_kind;
-    sd.min_obs = p.min_fusion_observations;
-    sd.max_cost = (float)p.max_photometric_error;
-    sd.min_contrast = (float)p.min_reference_contrast;
-    sd.ambiguity = p.ambiguity_threshold;
-    sd.recip_rel = p.max_reciprocal_relative_depth_error;
-    sd.fusion_rel = p.max_fusion_relative_position_error;
-    sd.err_scale = p.error_confidence_scale;
-    sd.margin_sat = p.depth_margin_saturation;
-    sd.border = border;
-    sd.stride = stride;
-    sd.grid_nx = nx;
-    sd.grid_ny = ny;
-    sd.emit_samples = p.emit_samples;
-    std::vector<SourceDev> srcs(S);
-    std::vector<double> Hp((size_t)S * D * 9), Hr((size_t)S * D * 9), dp(D), dr((size_t)S * D);
-    for (int hh = 0; hh < D; ++hh) dp[hh] = hyp_depth(hh, D, sweep_min, sweep_max);
-    for (int k = 0; k < S; ++k) {
-        const Cam &c = cams[picks[k].cam];
-        const DevView *v = views[1 + k];
-        SourceDev &sdv = srcs[k];
-        sdv.tex = ViewTex{v->quad.p, v->luma.as<uint8_t>(), v->w, v->h};
-        std::memcpy(sdv.R, c.R, sizeof(sdv.R));
-        std::memcpy(sdv.t, c.t, sizeof(sdv.t));
-        plane_homography_parts(ref, w, h, c, v->w, v->h, f, sdv.A, sdv.B);
-        sdv.min_depth = picks[k].mn;
-        sdv.max_depth = picks[k].mx;
-        for (int hh = 0; hh < D; ++hh) {
-            double *H = &Hp[((size_t)k * D + hh) * 9];
-            for (int e = 0; e < 9; ++e) H[e] = dp[hh] * sdv.A[e] + sdv.B[e];
-        }
-        double Ar[9], Br[9];
-        plane_homography_parts(c, v->w, v->h, ref, w, h, f, Ar, Br);
-        for (int hh = 0; hh < D; ++hh) {
-            const double d = hyp_depth(hh, D, picks[k].mn, picks[k].mx);
-            dr[(size_t)k * D + hh] = d;
-            double *H = &Hr[((size_t)k * D + hh) * 9];
-            for (int e = 0; e < 9; ++e) H[e] = d * Ar[e] + Br[e];
-        }
-    }
-
-    const uint32_t nsites = nx * ny;
-    const uint32_t nmax = std::max<uint32_t>(nsites, 1);
-    const uint32_t blocks = (nmax + kBlock - 1) / kBlock;
-    // Hypothesis blocks so that a launch holds at least ~4 waves per SIMD of 256 CUs.
-    const uint32_t waves = (nmax + 63) / 64;
-    int n_hb = (int)std::min<uint32_t>((uint32_t)D, std::max<uint32_t>(1u, (4096u + waves - 1) / waves));
-    const int hblock = (D + n_hb - 1) / n_hb;
-    n_hb = (D + hblock - 1) / hblock;
-
-    if ((rc = s->solve.ensure(sizeof(SolveDev))) || (rc = s->src.ensure(sizeof(SourceDev) * S)) ||
-        (rc = s->Hp.ensure(Hp.size() * 8)) || (rc = s->Hr.ensure(Hr.size() * 8)) ||
-        (rc = s->depth_p.ensure(dp.size() * 8)) || (rc = s->depth_r.ensure(dr.size() * 8)) ||
-        (rc = s->site_flag.ensure(nmax)) || (rc = s->samples.ensure(4ull * nmax)) ||
-        (rc = s->site_map.ensure(4ull * nmax)) ||
-        (rc = s->n_samples.ensure(16)) ||
-        (rc = s->partial.ensure(sizeof(Top2) * (size_t)nmax * n_hb)) ||
-        (rc = s->decision.ensure(sizeof(Decision) * (size_t)nmax)) ||
-        (rc = s->recip.ensure(sizeof(Recip) * (size_t)nmax * S)) ||
-        (rc = s->out.ensure(sizeof(SampleOut) * (size_t)nmax)) ||
-        (rc = s->block_counts.ensure(4ull * kCounters * blocks)) ||
-        (rc = s->block_offsets.ensure(4ull * kCounters * blocks)) ||
-        (rc = s->totals.ensure(4ull * kCounters)) ||
-        (rc = s->points.ensure(sizeof(vt3d_point3) * (size_t)nmax)) ||
-        (rc = s->sites.ensure(sizeof(vt3d_grid_site) * (size_t)nmax)) ||
-        (rc = s->attrs.ensure(sizeof(vt3d_point_attributes) * (size_t)nmax)) ||
-        (rc = s->errors.ensure(4ull * nmax)) || (rc = s->supports.ensure(4ull * nmax)) ||
-        (rc = s->fobs.ensure(4ull * nmax)) ||
-        (p.emit_samples && (rc = s->diag.ensure(sizeof(vt3d_sample_diag) * (size_t)nmax))))
-        return rc;
-    VT3D_HIP(hipMemcpyAsync(s->solve.p, &sd, sizeof(sd), hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(s->src.p, srcs.data(), sizeof(SourceDev) * S, hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(s->Hp.p, Hp.data(), Hp.size() * 8, hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(s->Hr.p, Hr.data(), Hr.size() * 8, hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(s->depth_p.p, dp.data(), dp.size() * 8, hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(s->depth_r.p, dr.data(), dr.size() * 8, hipMemcpyHostToDevice, st));
-
-    Args a{};
-    a.solve = s->solve.as<SolveDev>();
-    a.src = s->src.as<SourceDev>();
-    a.Hp = s->Hp.as<double>();
-    a.Hr = s->Hr.as<double>();
-    a.depth_p = s->depth_p.as<double>();
-    a.depth_r = s->depth_r.as<double>();
-    a.ref_rgba = views[0]->rgba.as<uint8_t>();
-    a.ref_rgba_w = w;
-    a.site_flag = s->site_flag.as<uint8_t>();
-    a.samples = s->samples.as<uint32_t>();
-    a.sample_of_site = s->site_map.as<int>();
-    a.n_samples = s->n_samples.as<uint32_t>();
-    a.max_samples = nmax;
-    a.partial = s->partial.as<Top2>();
-    a.hblock = hblock;
-    a.n_hblocks = n_hb;
-    a.decision = s->decision.as<Decision>();
-    a.recip = s->recip.as<Recip>();
-    a.out = s->out.as<SampleOut>();
-    a.block_counts = s->block_counts.as<uint32_t>();
-    a.block_offsets = s->block_offsets.as<uint32_t>();
-    a.totals = s->totals.as<uint32_t>();
-    a.n_blocks = blocks;
-
-    const int smax = S <= 4 ? 4 : S <= 8 ? 8 : S <= 16 ? 16 : 32;
-    VT3D_HIP(hipEventRecord(s->ev[2], st));
-    launch_gate(a, blocks, st);
-    launch_scan(a, a.n_samples, st);
-    launch_compact_sites(a, blocks, st);
-    VT3D_HIP(hipGetLastError());
-    VT3D_HIP(hipEventRecord(s->ev[3], st));
-    // Tiled primary sweep when patches of neighbouring sites overlap (stride < P) and the
-    // warped region fits the LDS budget; the direct kernel otherwise or when asked for.
-    TileLaunch tl{};
-    bool tiled = false;
-    {
-        const int P = 2 * R + 1;
-        const bool want = p.sweep_kernel == VT3D_KERNEL_TILED ||
-                          (p.sweep_kernel == VT3D_KERNEL_AUTO && (int)stride < P);
-        tl.tile_w = 32;
-        tl.lds_bytes = tile_lds_bytes(tl.tile_w, (int)stride, R);
-        if (want && tl.lds_bytes <= 64 * 1024) {
-            tiled = true;
-            tl.tiles_x = (int)((nx + tl.tile_w - 1) / tl.tile_w);
-            const uint32_t ty = (uint32_t)(kBlock / tl.tile_w);
-            tl.tiles = (uint32_t)tl.tiles_x * ((ny + ty - 1) / ty);
-        }
-    }
-    if (!launch_sweep_or_recip(p.storage, p.cost_kind, R, a, smax, S, blocks, false,
-                               tiled ? &tl : nullptr, st)) {
-        set_error("no kernel for this cost / radius / storage");
-        return VT3D_E_INVALID;
-    }
-    VT3D_HIP(hipGetLastError());
-    VT3D_HIP(hipEventRecord(s->ev[4], st));
-    launch_decide(a, blocks, st);
-    launch_sweep_or_recip(p.storage, p.cost_kind, R, a, smax, S, blocks, true, nullptr, st);
-    VT3D_HIP(hipGetLastError());
-    VT3D_HIP(hipEventRecord(s->ev[5], st));
-    launch_fuse(a, p.emit_samples ? s->diag.as<vt3d_sample_diag>() : nullptr, blocks, st);
-    launch_scan(a, nullptr, st);
-    launch_compact_points(a, s->points.as<vt3d_point3>(), s->sites.as<vt3d_grid_site>(),
-                          s->attrs.as<vt3d_point_attributes>(), s->errors.as<float>(),
-                          s->supports.as<float>(), s->fobs.as<float>(), blocks, st);
-    VT3D_HIP(hipGetLastError());
-    VT3D_HIP(hipEventRecord(s->ev[6], st));
-
-    uint32_t totals[kCounters + 1];
-    VT3D_HIP(hipMemcpyAsync(totals, s->totals.p, 4 * kCounters, hipMemcpyDeviceToHost, st));
-    VT3D_HIP(hipMemcpyAsync(&totals[kCounters], s->n_samples.p, 4, hipMemcpyDeviceToHost, st));
-    VT3D_HIP(hipStreamSynchronize(st));
-    const uint32_t n_acc = totals[0];
-    const uint32_t n_smp = totals[kCounters];
-
-    vt3d_dense_result *r = new_result();
-    if (!r) return VT3D_E_DEVICE;
-    std::unique_ptr<vt3d_dense_result, void (*)(vt3d_dense_result *)> guard(r, vt3d_dense_result_free);
-    std::vector<float> errs(n_acc), sups(n_acc), fob(n_acc);
-    if (n_acc) {
-        r->points = (vt3d_point3 *)std::malloc(sizeof(vt3d_point3) * n_acc);
-        r->grid_sites = (vt3d_grid_site *)std::malloc(sizeof(vt3d_grid_site) * n_acc);
-        r->attributes = (vt3d_point_attributes *)std::malloc(sizeof(vt3d_point_attributes) * n_acc);
-        if (!r->points || !r->grid_sites || !r->attributes) return VT3D_E_DEVICE;
-        VT3D_HIP(hipMemcpyAsync(r->points, s->points.p, sizeof(vt3d_point3) * n_acc, hipMemcpyDeviceToHost, st));
-        VT3D_HIP(hipMemcpyAsync(r->grid_sites, s->sites.p, sizeof(vt3d_grid_site) * n_acc, hipMemcpyDeviceToHost, st));
-        VT3D_HIP(hipMemcpyAsync(r->attributes, s->attrs.p, sizeof(vt3d_point_attributes) * n_acc, hipMemcpyDeviceToHost, st));
-        VT3D_HIP(hipMemcpyAsync(errs.data(), s->errors.p, 4ull * n_acc, hipMemcpyDeviceToHost, st));
-        VT3D_HIP(hipMemcpyAsync(sups.data(), s->supports.p, 4ull * n_acc, hipMemcpyDeviceToHost, st));
-        VT3D_HIP(hipMemcpyAsync(fob.data(), s->fobs.p, 4ull * n_acc, hipMemcpyDeviceToHost, st));
-    }
-    if (p.emit_samples && n_smp) {
-        r->samples = (vt3d_sample_diag *)std::malloc(sizeof(vt3d_sample_diag) * n_smp);
-        if (!r->samples) return VT3D_E_DEVICE;
-        VT3D_HIP(hipMemcpyAsync(r->samples, s->diag.p, sizeof(vt3d_sample_diag) * n_smp, hipMemcpyDeviceToHost, st));
-        r->n_samples = n_smp;
-    }
-    VT3D_HIP(hipEventRecord(s->ev[7], st));
-    VT3D_HIP(hipStreamSynchronize(st));
-
-    vt3d_timing &tm = s->timing;
-    tm.upload_ms = elapsed(s->ev[0], s->ev[1]);
-    tm.gate_ms = elapsed(s->ev[2], s->ev[3]);
-    tm.sweep_ms = elapsed(s->ev[3], s->ev[4]);
-    tm.reciprocal_ms = elapsed(s->ev[4], s->ev[5]);
-    tm.fuse_ms = elapsed(s->ev[5], s->ev[6]);
-    tm.download_ms = elapsed(s->ev[6], s->ev[7]);
-    tm.total_ms = elapsed(s->ev[0], s->ev[7]);
-    tm.units = (uint64_t)n_smp * (uint64_t)D;
-
-    // Medians (median_option, dense_legacy.rs:1176-1182): sorted value at index
-    // round((n - 1) / 2), i.e. n / 2.
-    auto median = [](std::vector<float> &v, float *m) {
-        if (v.empty()) return false;
-        std::nth_element(v.begin(), v.begin() + v.size() / 2, v.end());
-        *m = v[v.size() / 2];
-        return true;
-    };
-    vt3d_dense_stats &sx = r->stats;
-    sx.attempted = 1;
-    sx.has_reference_frame = 1;
-    sx.reference_frame = ref.frame_index;
-    sx.source_views = (uint64_t)S;
-    for (int k = 0; k < S && k < 32; ++k) sx.source_frames[k] = cams[picks[k].cam].frame_index;
-    sx.sampled_pixels = n_smp;
-    sx.depth_hypotheses = (uint64_t)D;
-    sx.accepted_points = n_acc;
-    sx.grid_stride = stride;
-    sx.grid_border = border;
-    sx.reciprocal_checked_points = totals[1];
-    sx.reciprocal_rejected_points = totals[2];
-    sx.reciprocal_consistent_points = totals[3];
-    sx.fusion_input_observations = totals[4];
-    sx.fusion_rejected_observations = totals[5];
-    sx.fusion_rejected_points = totals[6];
-    sx.has_median_fusion_observations = median(fob, &sx.median_fusion_observations);
-    sx.has_median_supporting_views = median(sups, &sx.median_supporting_views);
-    sx.has_median_photometric_error = median(errs, &sx.median_photometric_error);
-    sx.has_search_depth = 1;
-    sx.search_min_depth = (float)sweep_min;
-    sx.search_max_depth = (float)sweep_max;
-    r->n_points = n_acc;
-    r->primary_points = n_acc;
-    *out = guard.release();
+    if ((rc = view_of(ref.frame_index, &pr.ref_view))) return rc;
+    pr.src_views.resize(pr.S);
+    for (int k = 0; k < pr.S; ++k)
+        if ((rc = view_of(pr.src_cams[k].frame_index, &pr.src_views[k]))) return rc;
+    pr.state = 0;
     return VT3D_OK;
-}
-
-// One session per (thread, device) for the one-shot entry points. They live until the
-// process ends: freeing device memory from thread-exit destructors can run after the HIP
-// runtime has shut down.
-vt3d_session *thread_session(int *rc) {
-    thread_local std::vector<vt3d_session *> per_device;
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
-    if ((size_t)dev >= per_device.size()) per_device.resize((size_t)dev + 1, nullptr);
-    if (!per_device[dev]) {
-        vt3d_session *n = nullptr;
-        *rc = vt3d_session_create(dev, &n);
-        if (*rc) return nullptr;
-        per_device[dev] = n;
-    }
-    *rc = VT3D_OK;
-    return per_device[dev];
 }
 
 // reconstruct_reference_patch (dense.rs:674-747): remap so that the reference is index 0
 // and the others follow nearest frame first; keep the landmarks visible in the reference.
-int reference_patch(vt3d_session *s, FrameSource &base, const vt3d_camera *cameras,
-                    size_t n_cameras, const double *sparse_xyz, size_t n_sparse, double focal,
-                    size_t ref_idx, bool fine_grid, const vt3d_sweep_params &p, Control &ctl,
-                    vt3d_dense_result **out) {
-    *out = nullptr;
-    if (ref_idx >= n_cameras) return VT3D_NO_PATCH;
+int prepare_reference_patch(vt3d_session *s, FrameSource &base, const vt3d_camera *cameras,
+                            size_t n_cameras, const double *sparse_xyz, size_t n_sparse,
+                            double focal, size_t ref_idx, bool fine_grid,
+                            const vt3d_sweep_params &p, Control &ctl, Prepared &pr) {
+    pr.p = p;
+    auto none = [&]() {
+        pr.state = 1;
+        pr.none = 1;
+        return VT3D_OK;
+    };
+    if (ref_idx >= n_cameras) return none();
     const vt3d_camera &rcam = cameras[ref_idx];
-    if (rcam.frame_index >= base.count()) return VT3D_NO_PATCH;
+    if (rcam.frame_index >= base.count()) return none();
     const uint32_t rw = base.w(rcam.frame_index), rh = base.h(rcam.frame_index);
     const Cam rc = to_cam(rcam);
     std::vector<double> vis;
@@ -740,7 +565,7 @@ int reference_patch(vt3d_session *s, FrameSource &base, const vt3d_camera *camer
             vis.push_back(q.z);
         }
     }
-    if (vis.size() / 3 < kMinVisibleSparsePoints) return VT3D_NO_PATCH;
+    if (vis.size() / 3 < kMinVisibleSparsePoints) return none();
     std::vector<size_t> idx;
     for (size_t i = 0; i < n_cameras; ++i)
         if (i != ref_idx && cameras[i].frame_index < base.count()) idx.push_back(i);
@@ -756,35 +581,548 @@ int reference_patch(vt3d_session *s, FrameSource &base, const vt3d_camera *camer
     FrameSource remapped{base.host, base.session, {}};
     std::vector<vt3d_camera> rcams;
     for (size_t k = 0; k < order.size(); ++k) {
-        if (ctl.is_canceled()) return VT3D_NO_PATCH;
+        if (ctl.is_canceled()) return none();
         remapped.map.push_back(base.map[cameras[order[k]].frame_index]);
         vt3d_camera c = cameras[order[k]];
         c.frame_index = k;
         rcams.push_back(c);
     }
-    int rc2 = plane_sweep(s, remapped, rcams.data(), rcams.size(), vis.data(), vis.size() / 3,
-                          focal, fine_grid, p, ctl, out);
+    int rc2 = prepare_sweep(s, remapped, rcams.data(), rcams.size(), vis.data(), vis.size() / 3,
+                            focal, fine_grid, p, ctl, pr);
     if (rc2 != VT3D_OK) return rc2;
-    vt3d_dense_result *r = *out;
-    r->stats.has_reference_frame = 1;
-    r->stats.reference_frame = rcam.frame_index;
-    for (size_t i = 0; i < r->stats.source_views && i < 32; ++i)
-        r->stats.source_frames[i] = cameras[order[r->stats.source_frames[i]]].frame_index;
+    // Report the caller's frame indices (dense.rs:734-745).
+    if (pr.state == 1 && pr.host_result) {
+        pr.host_result->stats.has_reference_frame = 1;
+        pr.host_result->stats.reference_frame = rcam.frame_index;
+    } else {
+        pr.ref_frame_out = rcam.frame_index;
+        for (uint64_t &f : pr.src_frames_out) f = cameras[order[f]].frame_index;
+    }
     return VT3D_OK;
 }
 
-int check_common(const vt3d_camera *cameras, size_t n_cameras, const double *sparse_xyz,
-                 size_t n_sparse, const vt3d_sweep_params *params, vt3d_dense_result **out,
-                 vt3d_sweep_params *resolved) {
-    if (!out) {
-        set_error("out is null");
-        return VT3D_E_INVALID;
+// ---- batch pipeline ---------------------------------------------------------------------------
+
+struct Group {
+    uint32_t p0, p1, items;
+    int storage, cost, radius, smax, tiled;
+    size_t lds;
+};
+
+int smax_of(int S) { return S <= 4 ? 4 : S <= 8 ? 8 : S <= 16 ? 16 : 32; }
+
+// A batch laid out on the device: problem order, tables, groups and buffer sizes.
+struct Plan {
+    std::vector<Prepared *> dev;       // problems with device work, in device order
+    std::vector<uint32_t> slot_of;     // request -> device problem index (or UINT32_MAX)
+    std::vector<ProblemDev> prob;
+    std::vector<SourceDev> srcs;
+    std::vector<double> Hp, Hr, dp, dr;
+    std::vector<uint32_t> block_problem;
+    std::vector<Group> sweep_groups, recip_groups;
+    uint32_t slots = 0, n_blocks = 0;
+    uint64_t partial_slots = 0, recip_slots = 0;
+    bool emit = false;
+};
+
+bool wants_tiled(const Prepared &pr, int tile_w) {
+    const int P = 2 * pr.R + 1;
+    const bool want = pr.p.sweep_kernel == VT3D_KERNEL_TILED ||
+                      (pr.p.sweep_kernel == VT3D_KERNEL_AUTO && (int)pr.stride < P);
+    return want && tile_lds_bytes(tile_w, (int)pr.stride, pr.R) <= kMaxTileLds;
+}
+
+// Orders the device problems by kernel instantiation and lays out slots, tables and items.
+int build_plan(const std::vector<Prepared *> &reqs, const vt3d_tuning &tuning, Plan &pl) {
+    pl = Plan{};
+    pl.slot_of.assign(reqs.size(), UINT32_MAX);
+    const int tile_w = tuning.tile_width > 0 ? tuning.tile_width : 32;
+    std::vector<uint32_t> order;
+    for (uint32_t i = 0; i < reqs.size(); ++i)
+        if (reqs[i]->state == 0) order.push_back(i);
+    // Problems sharing a kernel instantiation (storage, cost, radius, source bucket, tiled)
+    // are consecutive, so each stage launches once per instantiation.
+    auto key = [&](uint32_t i) {
+        const Prepared &pr = *reqs[i];
+        return std::make_tuple(pr.p.storage, pr.p.cost_kind, pr.R, smax_of(pr.S),
+                               wants_tiled(pr, tile_w) ? 1 : 0);
+    };
+    std::stable_sort(order.begin(), order.end(),
+                     [&](uint32_t a, uint32_t b) { return key(a) < key(b); });
+    // Hypothesis blocks: enough work items that the sweep holds target_waves waves.
+    uint64_t total_blocks = 0;
+    for (uint32_t i : order) {
+        const Prepared &pr = *reqs[i];
+        total_blocks += std::max<uint32_t>(1, (pr.nx * pr.ny + kBlock - 1) / kBlock);
     }
-    *out = nullptr;
-    if ((n_cameras && !cameras) || (n_sparse && !sparse_xyz)) {
-        set_error("null input array with nonzero length");
-        return VT3D_E_INVALID;
+    const uint64_t target = tuning.target_waves > 0 ? (uint64_t)tuning.target_waves : 4096;
+    const uint64_t want_hb =
+        total_blocks ? std::max<uint64_t>(1, (target + 4 * total_blocks - 1) / (4 * total_blocks))
+                     : 1;
+
+    for (uint32_t i : order) {
+        Prepared &pr = *reqs[i];
+        const uint32_t k = (uint32_t)pl.prob.size();
+        pl.slot_of[i] = k;
+        pl.dev.push_back(&pr);
+        ProblemDev pd{};
+        pd.ref = ViewTex{pr.ref_view->quad.p, pr.ref_view->luma.as<uint8_t>(), pr.w, pr.h};
+        pd.ref_rgba = pr.ref_view->rgba.as<uint8_t>();
+        std::memcpy(pd.Rr, pr.ref.R, sizeof(pd.Rr));
+        std::memcpy(pd.tr, pr.ref.t, sizeof(pd.tr));
+        pd.focal = pr.focal;
+        pd.S = pr.S;
+        pd.D = pr.D;
+        pd.radius = pr.R;
+        pd.cost_kind = pr.p.cost_kind;
+        pd.ambiguity_kind = pr.p.ambiguity_kind;
+        pd.min_obs = pr.p.min_fusion_observations;
+        pd.max_cost = (float)pr.p.max_photometric_error;
+        pd.min_contrast = (float)pr.p.min_reference_contrast;
+        pd.ambiguity = pr.p.ambiguity_threshold;
+        pd.recip_rel = pr.p.max_reciprocal_relative_depth_error;
+        pd.fusion_rel = pr.p.max_fusion_relative_position_error;
+        pd.err_scale = pr.p.error_confidence_scale;
+        pd.margin_sat = pr.p.depth_margin_saturation;
+        pd.border = pr.border;
+        pd.stride = pr.stride;
+        pd.grid_nx = pr.nx;
+        pd.grid_ny = pr.ny;
+        const uint32_t nsites = pr.nx * pr.ny;
+        pd.cap = std::max<uint32_t>(1, (nsites + kBlock - 1) / kBlock) * kBlock;
+        pd.base = pl.slots;
+        pd.block0 = pd.base / kBlock;
+        pl.slots += pd.cap;
+        const int n_hb0 = (int)std::min<uint64_t>((uint64_t)pr.D, want_hb);
+        pd.hblock = (pr.D + n_hb0 - 1) / n_hb0;
+        pd.n_hb = (pr.D + pd.hblock - 1) / pd.hblock;
+        pd.partial_base = pl.partial_slots;
+        pl.partial_slots += (uint64_t)pd.n_hb * pd.cap;
+        pd.recip_base = pl.recip_slots;
+        pl.recip_slots += (uint64_t)pr.S * pd.cap;
+        pd.tiled = wants_tiled(pr, tile_w) ? 1 : 0;
+        pd.tile_w = tile_w;
+        pd.tiles_x = (int)((pr.nx + tile_w - 1) / tile_w);
+        const uint32_t ty = (uint32_t)(kBlock / tile_w);
+        pd.tiles = std::max<uint32_t>(1, (uint32_t)pd.tiles_x * ((pr.ny + ty - 1) / ty));
+        pd.emit_samples = pr.p.emit_samples ? 1 : 0;
+        pl.emit = pl.emit || pd.emit_samples;
+        for (uint32_t b = 0; b < pd.cap / kBlock; ++b) pl.block_problem.push_back(k);
+        pl.prob.push_back(pd);
     }
+    pl.n_blocks = (uint32_t)pl.block_problem.size();
+
+    // Sweep groups (one instantiation each) and their work items.
+    for (uint32_t k = 0; k < pl.prob.size();) {
+        const Prepared &pr = *pl.dev[k];
+        Group g{k, k, 0, pr.p.storage, pr.p.cost_kind, pr.R, smax_of(pr.S), pl.prob[k].tiled, 0};
+        uint32_t e = k;
+        while (e < pl.prob.size()) {
+            const Prepared &q = *pl.dev[e];
+            if (q.p.storage != g.storage || q.p.cost_kind != g.cost || q.R != g.radius ||
+                smax_of(q.S) != g.smax || pl.prob[e].tiled != g.tiled)
+                break;
+            ProblemDev &pd = pl.prob[e];
+            if (g.tiled) g.lds = std::max(g.lds, tile_lds_bytes(tile_w, (int)q.stride, q.R));
+            pd.sweep_item0 = g.items;
+            g.items += (g.tiled ? pd.tiles : pd.cap / kBlock) * (uint32_t)pd.n_hb;
+            ++e;
+        }
+        g.p1 = e;
+        pl.sweep_groups.push_back(g);
+        k = e;
+    }
+    // Reciprocal groups (storage, cost, radius) and their work items.
+    for (uint32_t k = 0; k < pl.prob.size();) {
+        const Prepared &pr = *pl.dev[k];
+        Group g{k, k, 0, pr.p.storage, pr.p.cost_kind, pr.R, 0, 0, 0};
+        uint32_t e = k;
+        while (e < pl.prob.size()) {
+            const Prepared &q = *pl.dev[e];
+            if (q.p.storage != g.storage || q.p.cost_kind != g.cost || q.R != g.radius) break;
+            ProblemDev &pd = pl.prob[e];
+            pd.recip_item0 = g.items;
+            g.items += (pd.cap / kBlock) * (uint32_t)pd.S;
+            ++e;
+        }
+        g.p1 = e;
+        pl.recip_groups.push_back(g);
+        k = e;
+    }
+
+    // FP64 tables; pointers hold offsets until the upload rebases them.
+    for (uint32_t k = 0; k < pl.prob.size(); ++k) {
+        const Prepared &pr = *pl.dev[k];
+        ProblemDev &pd = pl.prob[k];
+        const size_t dp0 = pl.dp.size();
+        pd.src = reinterpret_cast<const SourceDev *>(pl.srcs.size());
+        pd.Hp = reinterpret_cast<const double *>(pl.Hp.size());
+        pd.Hr = reinterpret_cast<const double *>(pl.Hr.size());
+        pd.depth_p = reinterpret_cast<const double *>(dp0);
+        pd.depth_r = reinterpret_cast<const double *>(pl.dr.size());
+        for (int hh = 0; hh < pr.D; ++hh)
+            pl.dp.push_back(hyp_depth(hh, pr.D, pr.sweep_min, pr.sweep_max));
+        for (int si = 0; si < pr.S; ++si) {
+            const Cam &c = pr.src_cams[si];
+            const DevView *v = pr.src_views[si];
+            SourceDev sdv{};
+            sdv.tex = ViewTex{v->quad.p, v->luma.as<uint8_t>(), v->w, v->h};
+            std::memcpy(sdv.R, c.R, sizeof(sdv.R));
+            std::memcpy(sdv.t, c.t, sizeof(sdv.t));
+            plane_homography_parts(pr.ref, pr.w, pr.h, c, v->w, v->h, pr.focal, sdv.A, sdv.B);
+            sdv.min_depth = pr.src_min[si];
+            sdv.max_depth = pr.src_max[si];
+            for (int hh = 0; hh < pr.D; ++hh)
+                for (int e = 0; e < 9; ++e)
+                    pl.Hp.push_back(pl.dp[dp0 + hh] * sdv.A[e] + sdv.B[e]);
+            pl.srcs.push_back(sdv);
+        }
+        for (int si = 0; si < pr.S; ++si) {
+            const Cam &c = pr.src_cams[si];
+            const DevView *v = pr.src_views[si];
+            double Ar[9], Br[9];
+            plane_homography_parts(c, v->w, v->h, pr.ref, pr.w, pr.h, pr.focal, Ar, Br);
+            for (int hh = 0; hh < pr.D; ++hh) {
+                const double d = hyp_depth(hh, pr.D, pr.src_min[si], pr.src_max[si]);
+                pl.dr.push_back(d);
+                for (int e = 0; e < 9; ++e) pl.Hr.push_back(d * Ar[e] + Br[e]);
+            }
+        }
+    }
+    return VT3D_OK;
+}
+
+}  // namespace
+}  // namespace vt3d
+
+// A prepared batch on a session (also the one-shot path's batch of one).
+struct vt3d_batch {
+    vt3d_session *session = nullptr;
+    std::vector<std::unique_ptr<vt3d::Prepared>> prepared;
+    vt3d::Plan plan;
+    vt3d::Workspace own;           // the batch's buffers (one-shot solves use the session's)
+    vt3d::Workspace *ws = nullptr;
+    bool uploaded = false;
+    bool use_graph = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    vt3d::Args args{};
+    vt3d::Outputs outs{};
+    size_t meta_words = 0;
+    vt3d_timing timing{};
+    ~vt3d_batch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+namespace vt3d {
+namespace {
+
+// Uploads the plan's tables into the workspace and fills the launch arguments.
+int upload_plan(vt3d_batch &b) {
+    vt3d_session *s = b.session;
+    Workspace &ws = *b.ws;
+    Plan &pl = b.plan;
+    hipStream_t st = s->stream;
+    const size_t n = pl.prob.size();
+    int rc;
+    if ((rc = ws.prob.ensure(sizeof(ProblemDev) * n)) ||
+        (rc = ws.src.ensure(sizeof(SourceDev) * pl.srcs.size())) ||
+        (rc = ws.Hp.ensure(8 * pl.Hp.size())) || (rc = ws.Hr.ensure(8 * pl.Hr.size())) ||
+        (rc = ws.depth_p.ensure(8 * pl.dp.size())) || (rc = ws.depth_r.ensure(8 * pl.dr.size())) ||
+        (rc = ws.block_problem.ensure(4ull * pl.n_blocks)) ||
+        (rc = ws.site_flag.ensure(pl.slots)) || (rc = ws.samples.ensure(4ull * pl.slots)) ||
+        (rc = ws.site_map.ensure(4ull * pl.slots)) ||
+        (rc = ws.partial.ensure(sizeof(Top2) * pl.partial_slots)) ||
+        (rc = ws.decision.ensure(sizeof(Decision) * pl.slots)) ||
+        (rc = ws.recip.ensure(sizeof(Recip) * pl.recip_slots)) ||
+        (rc = ws.out.ensure(sizeof(SampleOut) * pl.slots)) ||
+        (rc = ws.block_counts.ensure(4ull * kCounters * pl.n_blocks)) ||
+        (rc = ws.block_offsets.ensure(4ull * kCounters * pl.n_blocks)) ||
+        (rc = ws.points.ensure(sizeof(vt3d_point3) * pl.slots)) ||
+        (rc = ws.sites.ensure(sizeof(vt3d_grid_site) * pl.slots)) ||
+        (rc = ws.attrs.ensure(sizeof(vt3d_point_attributes) * pl.slots)) ||
+        (rc = ws.errors.ensure(4ull * pl.slots)) || (rc = ws.supports.ensure(4ull * pl.slots)) ||
+        (rc = ws.fobs.ensure(4ull * pl.slots)) ||
+        (pl.emit && (rc = ws.diag.ensure(sizeof(vt3d_sample_diag) * pl.slots))))
+        return rc;
+    // meta: totals [n][kCounters] | n_samples [n] | out_offset [n + 1] | diag_offset [n + 1]
+    b.meta_words = n * kCounters + n + 2 * (n + 1);
+    if ((rc = ws.meta.ensure(4 * b.meta_words)) || (rc = ws.meta_host.ensure(4 * b.meta_words)))
+        return rc;
+    for (ProblemDev &pd : pl.prob) {
+        pd.src = ws.src.as<SourceDev>() + reinterpret_cast<size_t>(pd.src);
+        pd.Hp = ws.Hp.as<double>() + reinterpret_cast<size_t>(pd.Hp);
+        pd.Hr = ws.Hr.as<double>() + reinterpret_cast<size_t>(pd.Hr);
+        pd.depth_p = ws.depth_p.as<double>() + reinterpret_cast<size_t>(pd.depth_p);
+        pd.depth_r = ws.depth_r.as<double>() + reinterpret_cast<size_t>(pd.depth_r);
+    }
+    VT3D_HIP(hipMemcpyAsync(ws.prob.p, pl.prob.data(), sizeof(ProblemDev) * n,
+                            hipMemcpyHostToDevice, st));
+    VT3D_HIP(hipMemcpyAsync(ws.src.p, pl.srcs.data(), sizeof(SourceDev) * pl.srcs.size(),
+                            hipMemcpyHostToDevice, st));
+    VT3D_HIP(hipMemcpyAsync(ws.Hp.p, pl.Hp.data(), 8 * pl.Hp.size(), hipMemcpyHostToDevice, st));
+    VT3D_HIP(hipMemcpyAsync(ws.Hr.p, pl.Hr.data(), 8 * pl.Hr.size(), hipMemcpyHostToDevice, st));
+    VT3D_HIP(hipMemcpyAsync(ws.depth_p.p, pl.dp.data(), 8 * pl.dp.size(), hipMemcpyHostToDevice, st));
+    VT3D_HIP(hipMemcpyAsync(ws.depth_r.p, pl.dr.data(), 8 * pl.dr.size(), hipMemcpyHostToDevice, st));
+    VT3D_HIP(hipMemcpyAsync(ws.block_problem.p, pl.block_problem.data(), 4ull * pl.n_blocks,
+                            hipMemcpyHostToDevice, st));
+    // The table vectors are pageable: wait for the copies before they can be freed or reused.
+    VT3D_HIP(hipStreamSynchronize(st));
+
+    Args &a = b.args;
+    a = Args{};
+    a.prob = ws.prob.as<ProblemDev>();
+    a.n_prob = (uint32_t)n;
+    a.n_blocks = pl.n_blocks;
+    a.block_problem = ws.block_problem.as<uint32_t>();
+    a.site_flag = ws.site_flag.as<uint8_t>();
+    a.samples = ws.samples.as<uint32_t>();
+    a.sample_of_site = ws.site_map.as<int>();
+    a.partial = ws.partial.as<Top2>();
+    a.decision = ws.decision.as<Decision>();
+    a.recip = ws.recip.as<Recip>();
+    a.out = ws.out.as<SampleOut>();
+    a.block_counts = ws.block_counts.as<uint32_t>();
+    a.block_offsets = ws.block_offsets.as<uint32_t>();
+    uint32_t *meta = ws.meta.as<uint32_t>();
+    a.totals = meta;
+    a.n_samples = meta + n * kCounters;
+    a.out_offset = a.n_samples + n;
+    a.diag_offset = a.out_offset + n + 1;
+    Outputs &o = b.outs;
+    o.points = ws.points.as<vt3d_point3>();
+    o.sites = ws.sites.as<vt3d_grid_site>();
+    o.attrs = ws.attrs.as<vt3d_point_attributes>();
+    o.errors = ws.errors.as<float>();
+    o.supports = ws.supports.as<float>();
+    o.fused_obs = ws.fobs.as<float>();
+    o.diag = pl.emit ? ws.diag.as<vt3d_sample_diag>() : nullptr;
+    b.uploaded = true;
+    return VT3D_OK;
+}
+
+// The batch's launches on the session stream, with events between the stages (ev may be
+// null, as under graph capture).
+int enqueue(vt3d_batch &b, hipEvent_t *ev) {
+    hipStream_t st = b.session->stream;
+    const Plan &pl = b.plan;
+    const Args a = b.args;
+    auto mark = [&](int k) -> int {
+        if (ev) VT3D_HIP(hipEventRecord(ev[k], st));
+        return VT3D_OK;
+    };
+    int rc;
+    if ((rc = mark(1))) return rc;
+    launch_gate(a, st);
+    launch_scan(a, 1, st);
+    launch_compact_sites(a, st);
+    launch_problem_offsets(a, 0, st);
+    VT3D_HIP(hipGetLastError());
+    if ((rc = mark(2))) return rc;
+    for (const Group &g : pl.sweep_groups) {
+        Args ga = a;
+        ga.group_p0 = g.p0;
+        ga.group_p1 = g.p1;
+        ga.group_items = g.items;
+        if (!launch_group(g.storage, g.cost, g.radius, g.smax, false, g.tiled != 0, g.lds, ga,
+                          st)) {
+            set_error("no kernel for this cost / radius / storage");
+            return VT3D_E_INVALID;
+        }
+    }
+    VT3D_HIP(hipGetLastError());
+    if ((rc = mark(3))) return rc;
+    launch_decide(a, st);
+    for (const Group &g : pl.recip_groups) {
+        Args ga = a;
+        ga.group_p0 = g.p0;
+        ga.group_p1 = g.p1;
+        ga.group_items = g.items;
+        launch_group(g.storage, g.cost, g.radius, 0, true, false, 0, ga, st);
+    }
+    VT3D_HIP(hipGetLastError());
+    if ((rc = mark(4))) return rc;
+    launch_fuse(a, b.outs.diag, st);
+    launch_scan(a, 0, st);
+    launch_problem_offsets(a, 1, st);
+    launch_compact_points(a, b.outs, st);
+    VT3D_HIP(hipGetLastError());
+    VT3D_HIP(hipMemcpyAsync(b.ws->meta_host.p, b.ws->meta.p, 4 * b.meta_words,
+                            hipMemcpyDeviceToHost, st));
+    return mark(5);
+}
+
+// Runs the batch and builds one result per request.
+int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
+    vt3d_session *s = b.session;
+    hipStream_t st = s->stream;
+    VT3D_HIP(hipSetDevice(s->device));
+    for (int k = 0; k < 8; ++k)
+        if (!s->ev[k]) VT3D_HIP(hipEventCreate(&s->ev[k]));
+    const size_t nreq = b.prepared.size();
+    for (size_t i = 0; i < nreq; ++i) results[i] = nullptr;
+    int rc;
+    const size_t n = b.plan.prob.size();
+    if (n > 0 && !b.uploaded && (rc = upload_plan(b))) return rc;
+    VT3D_HIP(hipEventRecord(s->ev[0], st));
+    if (n > 0) {
+        if (b.use_graph) {
+            if (!b.exec) {
+                VT3D_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+                rc = enqueue(b, nullptr);
+                hipGraph_t g = nullptr;
+                hipError_t ce = hipStreamEndCapture(st, &g);
+                if (rc) {
+                    if (g) (void)hipGraphDestroy(g);
+                    return rc;
+                }
+                VT3D_HIP(ce);
+                b.graph = g;
+                VT3D_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
+            }
+            VT3D_HIP(hipGraphLaunch(b.exec, st));
+            for (int k = 1; k <= 5; ++k) VT3D_HIP(hipEventRecord(s->ev[k], st));
+        } else if ((rc = enqueue(b, s->ev))) {
+            return rc;
+        }
+    } else {
+        for (int k = 1; k <= 5; ++k) VT3D_HIP(hipEventRecord(s->ev[k], st));
+    }
+    VT3D_HIP(hipStreamSynchronize(st));
+
+    // Packed outputs.
+    const uint32_t *meta = b.ws->meta_host.as<uint32_t>();
+    const uint32_t *totals = meta;
+    const uint32_t *n_samples = meta + n * kCounters;
+    const uint32_t *out_offset = n_samples + n;
+    const uint32_t *diag_offset = out_offset + n + 1;
+    const uint32_t n_acc = n ? out_offset[n] : 0;
+    const uint32_t n_diag = n ? diag_offset[n] : 0;
+    std::vector<vt3d_point3> points(n_acc);
+    std::vector<vt3d_grid_site> sites(n_acc);
+    std::vector<vt3d_point_attributes> attrs(n_acc);
+    std::vector<float> errs(n_acc), sups(n_acc), fob(n_acc);
+    std::vector<vt3d_sample_diag> diag(n_diag);
+    if (n_acc) {
+        VT3D_HIP(hipMemcpyAsync(points.data(), b.outs.points, sizeof(vt3d_point3) * n_acc,
+                                hipMemcpyDeviceToHost, st));
+        VT3D_HIP(hipMemcpyAsync(sites.data(), b.outs.sites, sizeof(vt3d_grid_site) * n_acc,
+                                hipMemcpyDeviceToHost, st));
+        VT3D_HIP(hipMemcpyAsync(attrs.data(), b.outs.attrs, sizeof(vt3d_point_attributes) * n_acc,
+                                hipMemcpyDeviceToHost, st));
+        VT3D_HIP(hipMemcpyAsync(errs.data(), b.outs.errors, 4ull * n_acc, hipMemcpyDeviceToHost, st));
+        VT3D_HIP(hipMemcpyAsync(sups.data(), b.outs.supports, 4ull * n_acc, hipMemcpyDeviceToHost, st));
+        VT3D_HIP(hipMemcpyAsync(fob.data(), b.outs.fused_obs, 4ull * n_acc, hipMemcpyDeviceToHost, st));
+    }
+    if (n_diag)
+        VT3D_HIP(hipMemcpyAsync(diag.data(), b.outs.diag, sizeof(vt3d_sample_diag) * n_diag,
+                                hipMemcpyDeviceToHost, st));
+    VT3D_HIP(hipEventRecord(s->ev[6], st));
+    VT3D_HIP(hipStreamSynchronize(st));
+
+    vt3d_timing &tm = b.timing;
+    tm = vt3d_timing{};
+    tm.gate_ms = elapsed(s->ev[1], s->ev[2]);
+    tm.sweep_ms = elapsed(s->ev[2], s->ev[3]);
+    tm.reciprocal_ms = elapsed(s->ev[3], s->ev[4]);
+    tm.fuse_ms = elapsed(s->ev[4], s->ev[5]);
+    tm.download_ms = elapsed(s->ev[5], s->ev[6]);
+    tm.total_ms = elapsed(s->ev[0], s->ev[6]);
+
+    // Medians (median_option, dense_legacy.rs:1176-1182): the sorted value at index
+    // round((n - 1) / 2), i.e. n / 2.
+    auto median = [](float *v, size_t cnt, float *m) {
+        if (cnt == 0) return 0;
+        std::nth_element(v, v + cnt / 2, v + cnt);
+        *m = v[cnt / 2];
+        return 1;
+    };
+    for (size_t i = 0; i < nreq; ++i) {
+        Prepared &pr = *b.prepared[i];
+        if (pr.state == 1) {
+            if (pr.none) continue;
+            vt3d_dense_result *r = new_result();
+            if (!r) return VT3D_E_DEVICE;
+            std::memcpy(r, pr.host_result, sizeof(vt3d_dense_result));
+            results[i] = r;
+            continue;
+        }
+        const uint32_t k = b.plan.slot_of[i];
+        const uint32_t *tot = totals + (size_t)k * kCounters;
+        const uint32_t o0 = out_offset[k], na = out_offset[k + 1] - o0;
+        vt3d_dense_result *r = new_result();
+        if (!r) return VT3D_E_DEVICE;
+        results[i] = r;
+        if (na) {
+            r->points = (vt3d_point3 *)std::malloc(sizeof(vt3d_point3) * na);
+            r->grid_sites = (vt3d_grid_site *)std::malloc(sizeof(vt3d_grid_site) * na);
+            r->attributes =
+                (vt3d_point_attributes *)std::malloc(sizeof(vt3d_point_attributes) * na);
+            if (!r->points || !r->grid_sites || !r->attributes) return VT3D_E_DEVICE;
+            std::memcpy(r->points, points.data() + o0, sizeof(vt3d_point3) * na);
+            std::memcpy(r->grid_sites, sites.data() + o0, sizeof(vt3d_grid_site) * na);
+            std::memcpy(r->attributes, attrs.data() + o0, sizeof(vt3d_point_attributes) * na);
+        }
+        const uint32_t ns = n_samples[k];
+        if (pr.p.emit_samples && ns) {
+            r->samples = (vt3d_sample_diag *)std::malloc(sizeof(vt3d_sample_diag) * ns);
+            if (!r->samples) return VT3D_E_DEVICE;
+            std::memcpy(r->samples, diag.data() + diag_offset[k], sizeof(vt3d_sample_diag) * ns);
+            r->n_samples = ns;
+        }
+        vt3d_dense_stats &sx = r->stats;
+        sx.attempted = 1;
+        sx.has_reference_frame = 1;
+        sx.reference_frame = pr.ref_frame_out;
+        sx.source_views = (uint64_t)pr.S;
+        for (int q = 0; q < pr.S && q < 32; ++q) sx.source_frames[q] = pr.src_frames_out[q];
+        sx.sampled_pixels = ns;
+        sx.depth_hypotheses = (uint64_t)pr.D;
+        sx.accepted_points = na;
+        sx.grid_stride = pr.stride;
+        sx.grid_border = pr.border;
+        sx.reciprocal_checked_points = tot[1];
+        sx.reciprocal_rejected_points = tot[2];
+        sx.reciprocal_consistent_points = tot[3];
+        sx.fusion_input_observations = tot[4];
+        sx.fusion_rejected_observations = tot[5];
+        sx.fusion_rejected_points = tot[6];
+        sx.has_median_fusion_observations =
+            median(fob.data() + o0, na, &sx.median_fusion_observations);
+        sx.has_median_supporting_views = median(sups.data() + o0, na, &sx.median_supporting_views);
+        sx.has_median_photometric_error =
+            median(errs.data() + o0, na, &sx.median_photometric_error);
+        sx.has_search_depth = 1;
+        sx.search_min_depth = (float)pr.sweep_min;
+        sx.search_max_depth = (float)pr.sweep_max;
+        r->n_points = na;
+        r->primary_points = na;
+        tm.units += (uint64_t)ns * (uint64_t)pr.D;
+    }
+    return VT3D_OK;
+}
+
+vt3d_session *thread_session(int *rc) {
+    // One session per (thread, device) for the one-shot entry points. They live until the
+    // process ends: freeing device memory from thread-exit destructors can run after the HIP
+    // runtime has shut down.
+    thread_local std::vector<vt3d_session *> per_device;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+    if ((size_t)dev >= per_device.size()) per_device.resize((size_t)dev + 1, nullptr);
+    if (!per_device[dev]) {
+        vt3d_session *n = nullptr;
+        *rc = vt3d_session_create(dev, &n);
+        if (*rc) return nullptr;
+        per_device[dev] = n;
+    }
+    *rc = VT3D_OK;
+    return per_device[dev];
+}
+
+int resolve_params(const vt3d_sweep_params *params, vt3d_sweep_params *resolved) {
     if (params) *resolved = *params;
     else vt3d_sweep_params_reference(resolved);
     if (!params_ok(*resolved)) {
@@ -793,6 +1131,56 @@ int check_common(const vt3d_camera *cameras, size_t n_cameras, const double *spa
     }
     if (resolved->surface_completion_passes > 0) {
         set_error("surface completion is not available on the device path");
+        return VT3D_E_INVALID;
+    }
+    return VT3D_OK;
+}
+
+// Solves one request as a batch of one on the session's workspace.
+int solve_one(vt3d_session *s, FrameSource &fs, const vt3d_camera *cameras, size_t n_cameras,
+              const double *sparse_xyz, size_t n_sparse, double focal, int64_t ref_idx,
+              bool fine_grid, const vt3d_sweep_params &p, vt3d_poll_fn poll, void *poll_user,
+              vt3d_dense_result **out) {
+    VT3D_HIP(hipSetDevice(s->device));
+    vt3d_batch b;
+    b.session = s;
+    b.ws = &s->ws;
+    b.prepared.emplace_back(new Prepared());
+    Prepared &pr = *b.prepared.back();
+    Control ctl{poll, poll_user};
+    int rc = ref_idx >= 0
+                 ? prepare_reference_patch(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal,
+                                           (size_t)ref_idx, fine_grid, p, ctl, pr)
+                 : prepare_sweep(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal,
+                                 fine_grid, p, ctl, pr);
+    if (rc) return rc;
+    if (pr.state == 1) {
+        if (pr.none) return VT3D_NO_PATCH;
+        *out = pr.host_result;
+        pr.host_result = nullptr;
+        return VT3D_OK;
+    }
+    std::vector<Prepared *> reqs{&pr};
+    if ((rc = build_plan(reqs, s->tuning, b.plan))) return rc;
+    vt3d_dense_result *res = nullptr;
+    if ((rc = run_batch(b, &res))) {
+        vt3d_dense_result_free(res);
+        return rc;
+    }
+    s->timing = b.timing;
+    *out = res;
+    return VT3D_OK;
+}
+
+int check_out(vt3d_dense_result **out, const vt3d_camera *cameras, size_t n_cameras,
+              const double *sparse_xyz, size_t n_sparse) {
+    if (!out) {
+        set_error("out is null");
+        return VT3D_E_INVALID;
+    }
+    *out = nullptr;
+    if ((n_cameras && !cameras) || (n_sparse && !sparse_xyz)) {
+        set_error("null input array with nonzero length");
         return VT3D_E_INVALID;
     }
     return VT3D_OK;
@@ -865,16 +1253,15 @@ int32_t vt3d_dense_plane_sweep(const vt3d_frame *frames, size_t n_frames,
                                int32_t fine_grid, const vt3d_sweep_params *params,
                                vt3d_poll_fn poll, void *poll_user, vt3d_dense_result **out) {
     vt3d_sweep_params p;
-    int rc = check_common(cameras, n_cameras, sparse_xyz, n_sparse, params, out, &p);
-    if (rc) return rc;
+    int rc = check_out(out, cameras, n_cameras, sparse_xyz, n_sparse);
+    if (rc || (rc = resolve_params(params, &p))) return rc;
     if (n_frames && !frames) return VT3D_E_INVALID;
     vt3d_session *s = thread_session(&rc);
     if (!s) return rc;
     FrameSource fs{frames, nullptr, {}};
     for (size_t i = 0; i < n_frames; ++i) fs.map.push_back(i);
-    Control ctl{poll, poll_user};
-    return plane_sweep(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal, fine_grid != 0, p,
-                       ctl, out);
+    return solve_one(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal, -1, fine_grid != 0,
+                     p, poll, poll_user, out);
 }
 
 int32_t vt3d_dense_reference_patch(const vt3d_frame *frames, size_t n_frames,
@@ -884,16 +1271,15 @@ int32_t vt3d_dense_reference_patch(const vt3d_frame *frames, size_t n_frames,
                                    const vt3d_sweep_params *params, vt3d_poll_fn poll,
                                    void *poll_user, vt3d_dense_result **out) {
     vt3d_sweep_params p;
-    int rc = check_common(cameras, n_cameras, sparse_xyz, n_sparse, params, out, &p);
-    if (rc) return rc;
+    int rc = check_out(out, cameras, n_cameras, sparse_xyz, n_sparse);
+    if (rc || (rc = resolve_params(params, &p))) return rc;
     if (n_frames && !frames) return VT3D_E_INVALID;
     vt3d_session *s = thread_session(&rc);
     if (!s) return rc;
     FrameSource fs{frames, nullptr, {}};
     for (size_t i = 0; i < n_frames; ++i) fs.map.push_back(i);
-    Control ctl{poll, poll_user};
-    return reference_patch(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal,
-                           reference_camera_index, fine_grid != 0, p, ctl, out);
+    return solve_one(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal,
+                     (int64_t)reference_camera_index, fine_grid != 0, p, poll, poll_user, out);
 }
 
 void vt3d_dense_result_free(vt3d_dense_result *r) {
@@ -952,6 +1338,18 @@ int32_t vt3d_session_upload_frames(vt3d_session *s, const vt3d_frame *frames, si
     return VT3D_OK;
 }
 
+int32_t vt3d_session_set_tuning(vt3d_session *s, const vt3d_tuning *t) {
+    if (!s || !t) return VT3D_E_INVALID;
+    if (t->tile_width != 0 && t->tile_width != 16 && t->tile_width != 32 && t->tile_width != 64) {
+        set_error("tile_width must be 16, 32 or 64");
+        return VT3D_E_INVALID;
+    }
+    if (t->target_waves < 0) return VT3D_E_INVALID;
+    s->tuning.tile_width = t->tile_width ? t->tile_width : 32;
+    s->tuning.target_waves = t->target_waves ? t->target_waves : 4096;
+    return VT3D_OK;
+}
+
 int32_t vt3d_session_solve(vt3d_session *s, const vt3d_camera *cameras, size_t n_cameras,
                            const double *sparse_xyz, size_t n_sparse, double focal,
                            int64_t reference_camera_index, int32_t fine_grid,
@@ -959,22 +1357,87 @@ int32_t vt3d_session_solve(vt3d_session *s, const vt3d_camera *cameras, size_t n
                            vt3d_dense_result **out) {
     if (!s) return VT3D_E_INVALID;
     vt3d_sweep_params p;
-    int rc = check_common(cameras, n_cameras, sparse_xyz, n_sparse, params, out, &p);
-    if (rc) return rc;
+    int rc = check_out(out, cameras, n_cameras, sparse_xyz, n_sparse);
+    if (rc || (rc = resolve_params(params, &p))) return rc;
     FrameSource fs{nullptr, s, {}};
     for (size_t i = 0; i < s->views.size(); ++i) fs.map.push_back(i);
-    Control ctl{poll, poll_user};
-    if (reference_camera_index < 0)
-        return plane_sweep(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal, fine_grid != 0,
-                           p, ctl, out);
-    return reference_patch(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal,
-                           (size_t)reference_camera_index, fine_grid != 0, p, ctl, out);
+    return solve_one(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal,
+                     reference_camera_index, fine_grid != 0, p, poll, poll_user, out);
 }
 
 int32_t vt3d_session_last_timing(const vt3d_session *s, vt3d_timing *t) {
     if (!s || !t) return VT3D_E_INVALID;
     *t = s->timing;
     return VT3D_OK;
+}
+
+int32_t vt3d_batch_create(vt3d_session *s, const vt3d_request *reqs, size_t n,
+                          int32_t use_graph, vt3d_batch **out) {
+    if (!s || !out || (n && !reqs)) return VT3D_E_INVALID;
+    *out = nullptr;
+    VT3D_HIP(hipSetDevice(s->device));
+    std::unique_ptr<vt3d_batch> b(new vt3d_batch());
+    b->session = s;
+    b->ws = &b->own;
+    b->use_graph = use_graph != 0;
+    for (size_t i = 0; i < n; ++i) {
+        const vt3d_request &q = reqs[i];
+        vt3d_sweep_params p;
+        int rc;
+        if ((rc = resolve_params(q.params, &p))) return rc;
+        if ((q.n_cameras && !q.cameras) || (q.n_sparse && !q.sparse_xyz) ||
+            (q.n_frames && !q.frame_map)) {
+            set_error("null request array with nonzero length");
+            return VT3D_E_INVALID;
+        }
+        FrameSource fs{nullptr, s, {}};
+        for (size_t k = 0; k < q.n_frames; ++k) {
+            if (q.frame_map[k] >= s->views.size() || !s->views[q.frame_map[k]]) {
+                set_error("request frame_map names a frame the session does not hold");
+                return VT3D_E_INVALID;
+            }
+            fs.map.push_back((size_t)q.frame_map[k]);
+        }
+        b->prepared.emplace_back(new Prepared());
+        Control ctl{nullptr, nullptr};
+        rc = q.reference_camera_index >= 0
+                 ? prepare_reference_patch(s, fs, q.cameras, q.n_cameras, q.sparse_xyz,
+                                           q.n_sparse, q.focal, (size_t)q.reference_camera_index,
+                                           q.fine_grid != 0, p, ctl, *b->prepared.back())
+                 : prepare_sweep(s, fs, q.cameras, q.n_cameras, q.sparse_xyz, q.n_sparse, q.focal,
+                                 q.fine_grid != 0, p, ctl, *b->prepared.back());
+        if (rc) return rc;
+    }
+    std::vector<Prepared *> reqp;
+    for (auto &u : b->prepared) reqp.push_back(u.get());
+    int rc = build_plan(reqp, s->tuning, b->plan);
+    if (rc) return rc;
+    *out = b.release();
+    return VT3D_OK;
+}
+
+int32_t vt3d_batch_run(vt3d_batch *b, vt3d_dense_result **results) {
+    if (!b || (!results && !b->prepared.empty())) return VT3D_E_INVALID;
+    int rc = run_batch(*b, results);
+    if (rc) {
+        for (size_t i = 0; i < b->prepared.size(); ++i) {
+            vt3d_dense_result_free(results[i]);
+            results[i] = nullptr;
+        }
+    }
+    return rc;
+}
+
+int32_t vt3d_batch_last_timing(const vt3d_batch *b, vt3d_timing *t) {
+    if (!b || !t) return VT3D_E_INVALID;
+    *t = b->timing;
+    return VT3D_OK;
+}
+
+void vt3d_batch_destroy(vt3d_batch *b) {
+    if (!b) return;
+    if (b->session && b->session->stream) (void)hipStreamSynchronize(b->session->stream);
+    delete b;
 }
 
 }  // extern "C"

@@ -1,8 +1,16 @@
 // vt3d_internal.h -- device-side data layout shared by the kernels and the host driver.
+//
+// Every launch works on a batch of independent solves ("problems"); a single solve is a
+// batch of one. Each problem owns a block-aligned range of slots [base, base + cap) in the
+// per-site and per-sample arrays (cap >= its grid sites, a multiple of kBlock), so the
+// mapping from a workgroup to its problem is fixed at launch time and a batch's launches
+// can be captured in a HIP graph and replayed.
 #pragma once
 
 #include <hip/hip_runtime.h>
 #include <stdint.h>
+
+#include "../../include/vt3d_dense.h"
 
 namespace vt3d {
 
@@ -15,9 +23,8 @@ constexpr int kCounters = 8;        // per-block category counters of a compacti
 // correlation and costs 1 (uncorrelated).
 constexpr float kZnccMinVariance = 1.0e-4f;
 
-// One luma texture in HBM. `quad` holds, for every pixel (x, y), the four bilinear
-// neighbours (I[x,y], I[x+1,y], I[x,y+1], I[x+1,y+1]) so one tap is one gather: float4
-// (FP32 storage) or four 16-bit values (FP16 / BF16 storage). `luma` is the u8 plane.
+// One luma texture in HBM. `quad` holds, for every pixel, the bilinear coefficients of the
+// cell it anchors (see Store in vt3d_kernels.hip); `luma` is the u8 plane.
 struct ViewTex {
     const void *quad;
     const uint8_t *luma;
@@ -36,9 +43,10 @@ struct SourceDev {
     double min_depth, max_depth;
 };
 
-// Per-solve constants in device memory.
-struct SolveDev {
+// One problem of a batch: its reference, parameters, tables and layout.
+struct ProblemDev {
     ViewTex ref;
+    const uint8_t *ref_rgba;
     double Rr[9];
     double tr[3];
     double focal;
@@ -48,8 +56,26 @@ struct SolveDev {
     double ambiguity;
     double recip_rel, fusion_rel, err_scale, margin_sat;
     uint32_t border, stride, grid_nx, grid_ny;
+    // tables
+    const SourceDev *src;   // [S]
+    const double *Hp;       // [S][D][9] primary homographies
+    const double *Hr;       // [S][D][9] reverse homographies (source -> reference)
+    const double *depth_p;  // [D]
+    const double *depth_r;  // [S][D]
+    // layout
+    uint32_t base;          // first slot of the problem's sites / samples (multiple of kBlock)
+    uint32_t cap;           // slots, a multiple of kBlock, >= grid sites
+    uint32_t block0;        // base / kBlock
+    uint32_t pad0;
+    uint64_t partial_base;  // first Top2 slot: [hb][cap]
+    uint64_t recip_base;    // first Recip slot: [s][cap]
+    uint32_t sweep_item0;   // first work item of the primary sweep
+    uint32_t recip_item0;   // first work item of the reciprocal kernel
+    int n_hb, hblock;       // hypothesis blocks
+    int tiled, tile_w, tiles_x;
+    uint32_t tiles;
     int emit_samples;
-    int pad;
+    int pad1;
 };
 
 // Best and comparable-second candidate of one sample over a block of hypotheses, in the
@@ -97,41 +123,39 @@ struct SampleOut {
     int support;
 };
 
-// Kernel arguments shared by the per-solve kernels.
+// Kernel arguments of a batch.
 struct Args {
-    const SolveDev *solve;
-    const SourceDev *src;
-    const double *Hp;       // [S][D][9] primary homographies
-    const double *Hr;       // [S][D][9] reverse homographies (source -> reference)
-    const double *depth_p;  // [D]
-    const double *depth_r;  // [S][D]
-    const uint8_t *ref_rgba;
-    uint32_t ref_rgba_w;
-    // gate / sample list
-    uint8_t *site_flag;     // [grid_nx * grid_ny]
-    uint32_t *samples;      // [grid sites] packed x | y << 16
-    int *sample_of_site;    // [grid sites] sample index, -1 where the texture gate failed
-    uint32_t *n_samples;    // device counter (total of the gate scan)
-    uint32_t max_samples;
-    // sweep
-    Top2 *partial;          // [hblocks][max_samples]
-    int hblock, n_hblocks;
-    Decision *decision;     // [max_samples]
-    Recip *recip;           // [S][max_samples]
-    SampleOut *out;         // [max_samples]
-    // compaction scan
-    uint32_t *block_counts; // [blocks][kCounters]
-    uint32_t *block_offsets;// [blocks][kCounters]
-    uint32_t *totals;       // [kCounters]
-    uint32_t n_blocks;
+    const ProblemDev *prob;
+    uint32_t n_prob;
+    uint32_t n_blocks;              // kBlock-blocks over all problems' slots
+    const uint32_t *block_problem;  // [n_blocks]
+    // Problems [group_p0, group_p1) with group_items work items: the launch of one
+    // template instantiation of the sweep or reciprocal kernels.
+    uint32_t group_p0, group_p1, group_items;
+    // per slot
+    uint8_t *site_flag;
+    uint32_t *samples;              // packed x | y << 16 at base + local sample index
+    int *sample_of_site;            // local sample index at base + site, -1 if gated out
+    Top2 *partial;
+    Decision *decision;
+    Recip *recip;
+    SampleOut *out;
+    // compaction scans
+    uint32_t *block_counts;         // [n_blocks][kCounters]
+    uint32_t *block_offsets;        // [n_blocks][kCounters], offsets within the problem
+    uint32_t *totals;               // [n_prob][kCounters]
+    uint32_t *n_samples;            // [n_prob]
+    uint32_t *out_offset;           // [n_prob + 1] packed offsets of accepted points
+    uint32_t *diag_offset;          // [n_prob + 1] packed offsets of sample diagnostics
 };
 
-// Launch shape of the tiled primary sweep.
-struct TileLaunch {
-    int tile_w;        // grid sites per tile row; kBlock / tile_w rows
-    int tiles_x;       // tiles per grid row
-    uint32_t tiles;    // tiles in the grid
-    size_t lds_bytes;  // two buffers of the warped region
+// Packed outputs of a batch (problem p's entries start at out_offset[p] / diag_offset[p]).
+struct Outputs {
+    ::vt3d_point3 *points;
+    ::vt3d_grid_site *sites;
+    ::vt3d_point_attributes *attrs;
+    float *errors, *supports, *fused_obs;
+    ::vt3d_sample_diag *diag;  // null unless a problem emits samples
 };
 
 }  // namespace vt3d

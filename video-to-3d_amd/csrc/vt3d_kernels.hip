@@ -36,9 +36,18 @@ constexpr float kInfF = __builtin_huge_valf();
 
 template <int ST> struct Store;
 
+// Texture pointers arrive through structs in memory, where the compiler cannot see that
+// they point to global memory; loading through address space 1 makes the gathers global
+// loads (vector-memory counter only) instead of flat loads that also wait on LDS.
+typedef float vf4 __attribute__((ext_vector_type(4)));
+typedef uint32_t vu2 __attribute__((ext_vector_type(2)));
+typedef const __attribute__((address_space(1))) vf4 gvf4;
+typedef const __attribute__((address_space(1))) vu2 gvu2;
+
 template <> struct Store<VT3D_STORAGE_F32> {
     __device__ static __forceinline__ float4 coef(const void *q, uint32_t i) {
-        return reinterpret_cast<const float4 *>(q)[i];
+        const vf4 v = ((gvf4 *)q)[i];
+        return make_float4(v.x, v.y, v.z, v.w);
     }
     __device__ static __forceinline__ void put(void *q, uint32_t i, float a, float b, float c,
                                                float d) {
@@ -58,7 +67,7 @@ __device__ __forceinline__ uint32_t half_bits(float v) {
 
 template <> struct Store<VT3D_STORAGE_F16> {
     __device__ static __forceinline__ float4 coef(const void *q, uint32_t i) {
-        uint2 u = reinterpret_cast<const uint2 *>(q)[i];
+        const vu2 u = ((gvu2 *)q)[i];
         return make_float4(half_lo(u.x), half_hi(u.x), half_lo(u.y), half_hi(u.y));
     }
     __device__ static __forceinline__ void put(void *q, uint32_t i, float a, float b, float c,
@@ -71,7 +80,7 @@ template <> struct Store<VT3D_STORAGE_F16> {
 
 template <> struct Store<VT3D_STORAGE_BF16> {
     __device__ static __forceinline__ float4 coef(const void *q, uint32_t i) {
-        uint2 u = reinterpret_cast<const uint2 *>(q)[i];
+        const vu2 u = ((gvu2 *)q)[i];
         const float a = __uint_as_float(u.x << 16), b = __uint_as_float(u.x & 0xffff0000u);
         const float c = __uint_as_float(u.y << 16), d = __uint_as_float(u.y & 0xffff0000u);
         return make_float4(a, b - a, c - a, d - b - c + a);
@@ -83,6 +92,14 @@ template <> struct Store<VT3D_STORAGE_BF16> {
             make_uint2(bits(a) | (bits(b) << 16), bits(c) | (bits(d) << 16));
     }
 };
+
+// floor(x) as an int in one instruction (the compiler emits v_floor + v_cvt for the C form).
+// Callers pass finite values within int range.
+__device__ __forceinline__ int floor_to_int(float x) {
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
 
 // Bilinear value from a coefficient texel.
 __device__ __forceinline__ float bicoef(float4 k, float tx, float ty) {
@@ -297,18 +314,27 @@ __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 *
         const float row_den = fmaf(w.e, fdy, 1.f);
         const float row_x = w.bx * fdy;
         const float row_y = w.by * fdy;
+        // Issue the row's P gathers before consuming any, so each wave keeps P loads in
+        // flight instead of waiting out the memory latency tap by tap.
+        float4 kk[P];
+        float tx[P], ty[P];
 #pragma unroll
         for (int dx = -R; dx <= R; ++dx) {
-            const int k = (dy + R) * P + (dx + R);
             const float fdx = (float)dx;
             const float rinv = __builtin_amdgcn_rcpf(fmaf(w.g, fdx, row_den));
             const float qx = fmaf(fmaf(w.ax, fdx, row_x), rinv, w.fxc);
             const float qy = fmaf(fmaf(w.ay, fdx, row_y), rinv, w.fyc);
-            const int ixo = (int)floorf(qx), iyo = (int)floorf(qy);
-            const float tx = __builtin_amdgcn_fractf(qx), ty = __builtin_amdgcn_fractf(qy);
+            const int ixo = floor_to_int(qx), iyo = floor_to_int(qy);
+            tx[dx + R] = __builtin_amdgcn_fractf(qx);
+            ty[dx + R] = __builtin_amdgcn_fractf(qy);
             const uint32_t idx =
                 min(base + (uint32_t)__mul24(iyo, (int)sw) + (uint32_t)ixo, nlast);
-            const float v = bicoef(Store<ST>::coef(tex, idx), tx, ty);
+            kk[dx + R] = Store<ST>::coef(tex, idx);
+        }
+#pragma unroll
+        for (int dx = -R; dx <= R; ++dx) {
+            const int k = (dy + R) * P + (dx + R);
+            const float v = bicoef(kk[dx + R], tx[dx + R], ty[dx + R]);
             if constexpr (COST == VT3D_COST_ZSAD) {
                 sv[k] = v;
                 s1 += v;
@@ -375,6 +401,20 @@ __device__ __forceinline__ float lds_patch_cost(const float *__restrict__ L, int
     return finish_cost<COST, T>(rp, st, s1, s2, srs, sv);
 }
 
+// ---- XCD-aware block order -----------------------------------------------------------------
+//
+// The dispatcher hands consecutive workgroups to the 8 XCDs of an MI355X round-robin, and
+// each XCD has its own L2. Renumbering the blocks so that XCD x runs the contiguous range
+// [x * per, (x + 1) * per) of logical items keeps neighbouring items (which read
+// neighbouring texels) in one L2. `nblocks` is a multiple of kXcds; logical items at or past
+// the real item count are padding.
+constexpr uint32_t kXcds = 8;
+
+__device__ __forceinline__ uint32_t xcd_chunk(uint32_t block, uint32_t nblocks) {
+    const uint32_t per = nblocks / kXcds;
+    return (block % kXcds) * per + block / kXcds;
+}
+
 // ---- candidate order -----------------------------------------------------------------------
 
 __device__ __forceinline__ bool better(int sa, float ca, int ha, int sb, float cb, int hb) {
@@ -423,8 +463,7 @@ __device__ __forceinline__ void block_count(const uint32_t (&c)[kCounters], uint
     }
 }
 
-// Rank of this thread among the block's threads with flag set (thread order), and the
-// block-wide prefix base read from block_offsets.
+// Rank of this thread among the block's threads with flag set (thread order).
 __device__ __forceinline__ uint32_t block_rank(bool flag) {
     __shared__ uint32_t wave_tot[kBlock / 64];
     const uint64_t m = __ballot(flag);
@@ -436,19 +475,34 @@ __device__ __forceinline__ uint32_t block_rank(bool flag) {
     return base + (uint32_t)__popcll(m & lanemask_lt());
 }
 
+// Problem of a work item within the launched group: the last problem of the group whose
+// first item is <= item. RECIP selects the reciprocal kernel's item numbering.
+template <bool RECIP>
+__device__ __forceinline__ uint32_t problem_of_item(const Args &a, uint32_t item) {
+    uint32_t lo = a.group_p0, hi = a.group_p1;
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        const uint32_t first = RECIP ? a.prob[mid].recip_item0 : a.prob[mid].sweep_item0;
+        if (first <= item) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
 __global__ __launch_bounds__(kBlock) void k_gate(Args a) {
-    const SolveDev &sd = *a.solve;
-    const uint32_t nsites = sd.grid_nx * sd.grid_ny;
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const ProblemDev &pd = a.prob[a.block_problem[b]];
+    const uint32_t nsites = pd.grid_nx * pd.grid_ny;
+    const uint32_t j = (b - pd.block0) * kBlock + threadIdx.x;
     uint32_t c[kCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
     if (j < nsites) {
-        const uint32_t x = sd.border + (j % sd.grid_nx) * sd.stride;
-        const uint32_t y = sd.border + (j / sd.grid_nx) * sd.stride;
+        const uint32_t x = pd.border + (j % pd.grid_nx) * pd.stride;
+        const uint32_t y = pd.border + (j / pd.grid_nx) * pd.stride;
         // patch_contrast at integer taps (dense_legacy.rs:1097-1111): bilinear at an integer
         // coordinate returns the pixel exactly. The grid border r + 2 keeps taps in bounds.
-        const int R = sd.radius;
-        const uint8_t *l = sd.ref.luma;
-        const uint32_t w = sd.ref.w;
+        const int R = pd.radius;
+        const uint8_t *l = pd.ref.luma;
+        const uint32_t w = pd.ref.w;
         uint32_t mn = 255, mx = 0;
         for (int dy = -R; dy <= R; ++dy)
             for (int dx = -R; dx <= R; ++dx) {
@@ -456,25 +510,29 @@ __global__ __launch_bounds__(kBlock) void k_gate(Args a) {
                 mn = min(mn, v);
                 mx = max(mx, v);
             }
-        const bool pass = (float)(mx - mn) >= sd.min_contrast;
-        a.site_flag[j] = pass ? 1 : 0;
+        const bool pass = (float)(mx - mn) >= pd.min_contrast;
+        a.site_flag[pd.base + j] = pass ? 1 : 0;
         c[0] = pass ? 1 : 0;
     }
     block_count(c, a.block_counts);
 }
 
-// Exclusive scan of block_counts over blocks, per category; one workgroup of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan(Args a, uint32_t *n_out) {
+// Exclusive scan of block_counts over each problem's blocks, per category; one workgroup of
+// 1024 threads per problem. gate != 0: the first category is the texture gate, whose total
+// is the problem's sample count.
+__global__ __launch_bounds__(1024) void k_scan(Args a, int gate) {
     __shared__ uint32_t wave_sum[16];
     __shared__ uint32_t carry_s;
-    const uint32_t nb = a.n_blocks;
+    const uint32_t p = blockIdx.x;
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t b0 = pd.block0, nb = pd.cap / kBlock;
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
     for (int k = 0; k < kCounters; ++k) {
         if (threadIdx.x == 0) carry_s = 0;
         __syncthreads();
         for (uint32_t base = 0; base < nb; base += 1024) {
             const uint32_t b = base + threadIdx.x;
-            const uint32_t v = b < nb ? a.block_counts[b * kCounters + k] : 0;
+            const uint32_t v = b < nb ? a.block_counts[(b0 + b) * kCounters + k] : 0;
             uint32_t incl = v;
 #pragma unroll
             for (int off = 1; off < 64; off <<= 1) {
@@ -486,32 +544,68 @@ __global__ __launch_bounds__(1024) void k_scan(Args a, uint32_t *n_out) {
             uint32_t wbase = 0;
             for (int wv = 0; wv < wave; ++wv) wbase += wave_sum[wv];
             const uint32_t carry = carry_s;
-            if (b < nb) a.block_offsets[b * kCounters + k] = carry + wbase + incl - v;
+            if (b < nb) a.block_offsets[(b0 + b) * kCounters + k] = carry + wbase + incl - v;
             __syncthreads();
             if (threadIdx.x == 1023) carry_s = carry + wbase + incl;
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            a.totals[k] = carry_s;
-            if (k == 0 && n_out) *n_out = carry_s;
+            a.totals[p * kCounters + k] = carry_s;
+            if (gate && k == 0) a.n_samples[p] = carry_s;
         }
         __syncthreads();
     }
 }
 
+// Exclusive scan over problems: which == 0, sample diagnostics (n_samples of problems that
+// emit them) into diag_offset; which == 1, accepted points (totals[p][0]) into out_offset.
+// Entry n_prob holds the batch total. One workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_problem_offsets(Args a, int which) {
+    __shared__ uint32_t wave_sum[16];
+    __shared__ uint32_t carry_s;
+    const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint32_t *dst = which == 0 ? a.diag_offset : a.out_offset;
+    if (threadIdx.x == 0) carry_s = 0;
+    __syncthreads();
+    for (uint32_t base = 0; base < a.n_prob; base += 1024) {
+        const uint32_t p = base + threadIdx.x;
+        uint32_t v = 0;
+        if (p < a.n_prob)
+            v = which == 0 ? (a.prob[p].emit_samples ? a.n_samples[p] : 0u)
+                           : a.totals[p * kCounters];
+        uint32_t incl = v;
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+            const uint32_t o = __shfl_up(incl, off, 64);
+            if (lane >= off) incl += o;
+        }
+        if (lane == 63) wave_sum[wave] = incl;
+        __syncthreads();
+        uint32_t wbase = 0;
+        for (int wv = 0; wv < wave; ++wv) wbase += wave_sum[wv];
+        const uint32_t carry = carry_s;
+        if (p < a.n_prob) dst[p] = carry + wbase + incl - v;
+        __syncthreads();
+        if (threadIdx.x == 1023) carry_s = carry + wbase + incl;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) dst[a.n_prob] = carry_s;
+}
+
 __global__ __launch_bounds__(kBlock) void k_compact_sites(Args a) {
-    const SolveDev &sd = *a.solve;
-    const uint32_t nsites = sd.grid_nx * sd.grid_ny;
-    const uint32_t j = blockIdx.x * kBlock + threadIdx.x;
-    const bool flag = j < nsites && a.site_flag[j] != 0;
+    const uint32_t b = blockIdx.x;
+    const ProblemDev &pd = a.prob[a.block_problem[b]];
+    const uint32_t nsites = pd.grid_nx * pd.grid_ny;
+    const uint32_t j = (b - pd.block0) * kBlock + threadIdx.x;
+    const bool flag = j < nsites && a.site_flag[pd.base + j] != 0;
     const uint32_t r = block_rank(flag);
-    if (j < nsites) a.sample_of_site[j] = -1;
+    if (j < nsites) a.sample_of_site[pd.base + j] = -1;
     if (flag) {
-        const uint32_t x = sd.border + (j % sd.grid_nx) * sd.stride;
-        const uint32_t y = sd.border + (j / sd.grid_nx) * sd.stride;
-        const uint32_t i = a.block_offsets[blockIdx.x * kCounters] + r;
-        a.samples[i] = x | (y << 16);
-        a.sample_of_site[j] = (int)i;
+        const uint32_t x = pd.border + (j % pd.grid_nx) * pd.stride;
+        const uint32_t y = pd.border + (j / pd.grid_nx) * pd.stride;
+        const uint32_t i = a.block_offsets[b * kCounters] + r;
+        a.samples[pd.base + i] = x | (y << 16);
+        a.sample_of_site[pd.base + j] = (int)i;
     }
 }
 
@@ -529,24 +623,44 @@ __device__ __forceinline__ void sorted_insert(float (&l)[N], float c) {
     }
 }
 
-// One lane per texture-gated sample; blockIdx.y selects a block of hypotheses. For each
-// hypothesis: the cost against every source, the photometric gate, support = number of
-// sources that pass, error = quantile(sorted errors, 0.5) (index n >> 1, dense_legacy.rs:
-// 1168-1174), then the running top-2 in candidate order.
+// Support (sources passing the photometric gate) and error (quantile 0.5 of their sorted
+// errors, index n >> 1; dense_legacy.rs:319-329, 1168-1174) of one hypothesis, folded into
+// the running top-2.
+template <int SMAX>
+__device__ __forceinline__ void fold_hypothesis(Top2 &t, const float (&l)[SMAX], int hyp) {
+    int cnt = 0;
+#pragma unroll
+    for (int k = 0; k < SMAX; ++k) cnt += l[k] < kInfF ? 1 : 0;
+    if (cnt > 0) {
+        const int mid = cnt >> 1;
+        float med = l[0];
+#pragma unroll
+        for (int k = 1; k < SMAX; ++k) med = k == mid ? l[k] : med;
+        top2_insert(t, cnt, med, hyp);
+    }
+}
+
+// Direct primary sweep: one lane per texture-gated sample; a work item is (problem, block of
+// 256 samples, hypothesis block). For each hypothesis: the cost against every source, the
+// photometric gate, support and median error, then the running top-2 in candidate order.
 template <int R, int COST, int ST, int SMAX>
 __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep(Args a) {
     constexpr int P = 2 * R + 1;
     constexpr int T = P * P;
-    const SolveDev &sd = *a.solve;
-    const uint32_t n = *a.n_samples;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t pk = a.samples[i];
+    const uint32_t item = xcd_chunk(blockIdx.x, gridDim.x);
+    if (item >= a.group_items) return;
+    const uint32_t p = problem_of_item<false>(a, item);
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t local = item - pd.sweep_item0;
+    const int hb = (int)(local % (uint32_t)pd.n_hb);
+    const uint32_t i = (local / (uint32_t)pd.n_hb) * kBlock + threadIdx.x;
+    if (i >= a.n_samples[p]) return;
+    const uint32_t pk = a.samples[pd.base + i];
     const int x = (int)(pk & 0xffffu), y = (int)(pk >> 16);
     float rp[T];
     {
-        const uint8_t *l = sd.ref.luma;
-        const uint32_t w = sd.ref.w;
+        const uint8_t *l = pd.ref.luma;
+        const uint32_t w = pd.ref.w;
 #pragma unroll
         for (int dy = -R; dy <= R; ++dy)
 #pragma unroll
@@ -554,10 +668,10 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep(Args a) {
                 rp[(dy + R) * P + (dx + R)] = (float)l[(size_t)(y + dy) * w + (x + dx)];
     }
     const RefStats st = ref_stats<COST, T>(rp);
-    const int S = sd.S, D = sd.D;
-    const int h0 = (int)blockIdx.y * a.hblock;
-    const int h1 = min(h0 + a.hblock, D);
-    const float maxc = sd.max_cost;
+    const int S = pd.S, D = pd.D;
+    const int h0 = hb * pd.hblock;
+    const int h1 = min(h0 + pd.hblock, D);
+    const float maxc = pd.max_cost;
     const double cx = (double)x, cy = (double)y;
     Top2 t{-1, 0, kInfF, -1, 0, kInfF};
     for (int hyp = h0; hyp < h1; ++hyp) {
@@ -565,25 +679,16 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep(Args a) {
 #pragma unroll
         for (int k = 0; k < SMAX; ++k) l[k] = kInfF;
         for (int s = 0; s < S; ++s) {
-            const SourceDev &src = a.src[s];
-            const double *H = a.Hp + ((size_t)s * D + hyp) * 9;
+            const SourceDev &src = pd.src[s];
+            const double *H = pd.Hp + ((size_t)s * D + hyp) * 9;
             float c = patch_cost<R, COST, ST>(rp, st, H, cx, cy, src.tex.quad, src.tex.w,
                                               src.tex.h);
             c = c <= maxc ? c : kInfF;
             sorted_insert<SMAX>(l, c);
         }
-        int cnt = 0;
-#pragma unroll
-        for (int k = 0; k < SMAX; ++k) cnt += l[k] < kInfF ? 1 : 0;
-        if (cnt > 0) {
-            const int mid = cnt >> 1;
-            float med = l[0];
-#pragma unroll
-            for (int k = 1; k < SMAX; ++k) med = k == mid ? l[k] : med;
-            top2_insert(t, cnt, med, hyp);
-        }
+        fold_hypothesis<SMAX>(t, l, hyp);
     }
-    a.partial[(size_t)blockIdx.y * a.max_samples + i] = t;
+    a.partial[pd.partial_base + (size_t)hb * pd.cap + i] = t;
 }
 
 // Tiled primary sweep. A workgroup owns a TX x TY tile of grid sites (TX * TY = kBlock);
@@ -592,28 +697,36 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep(Args a) {
 // as patch_error warps a tap) into LDS, then each lane sums its own patch from LDS. Patches
 // of neighbouring sites overlap when the grid stride s is below the patch size P, so each
 // warp and gather serves up to (P / s)^2 taps. Double-buffered LDS: one barrier per step.
+// A work item is (problem, tile, hypothesis block), hypothesis block fastest, taken in XCD
+// chunks: each XCD sweeps a contiguous strip of tiles over all hypotheses, so its L2 holds
+// one strip's source footprints instead of the whole image's.
 template <int R, int COST, int ST, int SMAX>
-__global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep_tile(Args a, int tile_w,
-                                                                 int tiles_x) {
+__global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep_tile(Args a) {
     extern __shared__ float lds[];
     constexpr int P = 2 * R + 1;
     constexpr int T = P * P;
-    const SolveDev &sd = *a.solve;
-    const int TX = tile_w, TY = kBlock / tile_w;
-    const int stride = (int)sd.stride;
+    const uint32_t item = xcd_chunk(blockIdx.x, gridDim.x);
+    if (item >= a.group_items) return;  // padding: the whole block leaves before any barrier
+    const uint32_t p = problem_of_item<false>(a, item);
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t local = item - pd.sweep_item0;
+    const int tile = (int)(local / (uint32_t)pd.n_hb);
+    const int hb = (int)(local % (uint32_t)pd.n_hb);
+    const int TX = pd.tile_w, TY = kBlock / pd.tile_w;
+    const int stride = (int)pd.stride;
     const int RW = (TX - 1) * stride + P, RH = (TY - 1) * stride + P;
     const int npx = RW * RH;
-    const int bx = (int)blockIdx.x % tiles_x, by = (int)blockIdx.x / tiles_x;
+    const int bx = tile % pd.tiles_x, by = tile / pd.tiles_x;
     const int lx = (int)threadIdx.x % TX, ly = (int)threadIdx.x / TX;
     const int gx = bx * TX + lx, gy = by * TY + ly;
-    const bool in_grid = gx < (int)sd.grid_nx && gy < (int)sd.grid_ny;
-    const int sidx = in_grid ? a.sample_of_site[gy * (int)sd.grid_nx + gx] : -1;
+    const bool in_grid = gx < (int)pd.grid_nx && gy < (int)pd.grid_ny;
+    const int sidx = in_grid ? a.sample_of_site[pd.base + gy * pd.grid_nx + gx] : -1;
     const bool active = sidx >= 0;
     float rp[T];
     {
-        const int x = (int)sd.border + gx * stride, y = (int)sd.border + gy * stride;
-        const uint8_t *l = sd.ref.luma;
-        const uint32_t w = sd.ref.w;
+        const int x = (int)pd.border + gx * stride, y = (int)pd.border + gy * stride;
+        const uint8_t *l = pd.ref.luma;
+        const uint32_t w = pd.ref.w;
 #pragma unroll
         for (int dy = -R; dy <= R; ++dy)
 #pragma unroll
@@ -623,15 +736,15 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep_tile(Args a, int ti
     }
     const RefStats st = ref_stats<COST, T>(rp);
     // Region origin (reference pixels) and this thread's walk over the region.
-    const int X0 = (int)sd.border + bx * TX * stride - R;
-    const int Y0 = (int)sd.border + by * TY * stride - R;
+    const int X0 = (int)pd.border + bx * TX * stride - R;
+    const int Y0 = (int)pd.border + by * TY * stride - R;
     const int py0 = (int)threadIdx.x / RW, px0 = (int)threadIdx.x - py0 * RW;
     const int step_y = kBlock / RW, step_x = kBlock - step_y * RW;
-    const int S = sd.S, D = sd.D;
-    const int h0 = (int)blockIdx.y * a.hblock;
-    const int h1 = min(h0 + a.hblock, D);
-    const float maxc = sd.max_cost;
-    const float *tile = lds + (ly * stride) * RW + lx * stride;
+    const int S = pd.S, D = pd.D;
+    const int h0 = hb * pd.hblock;
+    const int h1 = min(h0 + pd.hblock, D);
+    const float maxc = pd.max_cost;
+    const float *mine = lds + (ly * stride) * RW + lx * stride;
     Top2 t{-1, 0, kInfF, -1, 0, kInfF};
     int buf = 0;
     for (int hyp = h0; hyp < h1; ++hyp) {
@@ -639,8 +752,8 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep_tile(Args a, int ti
 #pragma unroll
         for (int k = 0; k < SMAX; ++k) l[k] = kInfF;
         for (int s = 0; s < S; ++s) {
-            const SourceDev &src = a.src[s];
-            const double *H = a.Hp + ((size_t)s * D + hyp) * 9;
+            const SourceDev &src = pd.src[s];
+            const double *H = pd.Hp + ((size_t)s * D + hyp) * 9;
             float *L = lds + buf * npx;
             for (int px = px0, py = py0; py < RH;) {
                 L[py * RW + px] = warp_sample<ST>(H, (double)(X0 + px), (double)(Y0 + py),
@@ -654,23 +767,14 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep_tile(Args a, int ti
             }
             __syncthreads();
             float c = kInfF;
-            if (active) c = lds_patch_cost<R, COST>(tile + buf * npx, RW, rp, st);
+            if (active) c = lds_patch_cost<R, COST>(mine + buf * npx, RW, rp, st);
             c = c <= maxc ? c : kInfF;
             sorted_insert<SMAX>(l, c);
             buf ^= 1;
         }
-        int cnt = 0;
-#pragma unroll
-        for (int k = 0; k < SMAX; ++k) cnt += l[k] < kInfF ? 1 : 0;
-        if (cnt > 0) {
-            const int mid = cnt >> 1;
-            float med = l[0];
-#pragma unroll
-            for (int k = 1; k < SMAX; ++k) med = k == mid ? l[k] : med;
-            top2_insert(t, cnt, med, hyp);
-        }
+        fold_hypothesis<SMAX>(t, l, hyp);
     }
-    if (active) a.partial[(size_t)blockIdx.y * a.max_samples + sidx] = t;
+    if (active) a.partial[pd.partial_base + (size_t)hb * pd.cap + sidx] = t;
 }
 
 // ---- k_decide ------------------------------------------------------------------------------
@@ -702,15 +806,16 @@ __device__ __forceinline__ bool project_d(const double *R, const double *t, cons
 }
 
 __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
-    const SolveDev &sd = *a.solve;
-    const uint32_t n = *a.n_samples;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= n) return;
+    const uint32_t b = blockIdx.x;
+    const uint32_t p = a.block_problem[b];
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t i = (b - pd.block0) * kBlock + threadIdx.x;
+    if (i >= a.n_samples[p]) return;
     Top2 t{-1, 0, kInfF, -1, 0, kInfF};
-    for (int b = 0; b < a.n_hblocks; ++b) {
-        const Top2 p = a.partial[(size_t)b * a.max_samples + i];
-        top2_insert(t, p.s1, p.c1, p.h1);
-        top2_insert(t, p.s2, p.c2, p.h2);
+    for (int hb = 0; hb < pd.n_hb; ++hb) {
+        const Top2 q = a.partial[pd.partial_base + (size_t)hb * pd.cap + i];
+        top2_insert(t, q.s1, q.c1, q.h1);
+        top2_insert(t, q.s2, q.c2, q.h2);
     }
     Decision d{};
     d.best_h = t.h1;
@@ -728,55 +833,59 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
             d.second_cost = t.c2;
             d.margin = (double)t.c2 - (double)t.c1;
         }
-        if (t.c1 > sd.max_cost) {
+        if (t.c1 > pd.max_cost) {
             d.status = VT3D_SAMPLE_REJECT_COST;
         } else {
             bool ambiguous;
-            if (sd.ambiguity_kind == VT3D_AMBIGUITY_MARGIN)
-                ambiguous = d.margin < sd.ambiguity;
+            if (pd.ambiguity_kind == VT3D_AMBIGUITY_MARGIN)
+                ambiguous = d.margin < pd.ambiguity;
             else
-                ambiguous = has_second && (double)t.c1 > sd.ambiguity * (double)t.c2;
+                ambiguous = has_second && (double)t.c1 > pd.ambiguity * (double)t.c2;
             d.status = ambiguous ? VT3D_SAMPLE_REJECT_AMBIGUITY : 0;
         }
     }
     if (d.status == 0) {
-        const uint32_t pk = a.samples[i];
+        const uint32_t pk = a.samples[pd.base + i];
         const double x = (double)(pk & 0xffffu), y = (double)(pk >> 16);
-        d.depth = a.depth_p[d.best_h];
-        unproject_d(sd.Rr, sd.tr, x, y, d.depth, sd.ref.w, sd.ref.h, sd.focal, d.pos);
-        d.proj_ok = project_d(sd.Rr, sd.tr, d.pos, sd.ref.w, sd.ref.h, sd.focal, &d.xr, &d.yr,
+        d.depth = pd.depth_p[d.best_h];
+        unproject_d(pd.Rr, pd.tr, x, y, d.depth, pd.ref.w, pd.ref.h, pd.focal, d.pos);
+        d.proj_ok = project_d(pd.Rr, pd.tr, d.pos, pd.ref.w, pd.ref.h, pd.focal, &d.xr, &d.yr,
                               &d.dr)
                         ? 1
                         : 0;
     }
-    a.decision[i] = d;
+    a.decision[pd.base + i] = d;
 }
 
 // ---- k_recip -------------------------------------------------------------------------------
 
-// One lane per (sample, source); blockIdx.y is the source, so every wave shares the source
-// and its reverse homographies. reciprocal_depth_observations (dense_legacy.rs:795-856):
-// direct cost at the re-projected reference point, projection into the source, the texture
-// gate there, a reverse sweep over the source's landmark envelope against the reference
-// image (estimate_single_view_depth, dense_legacy.rs:887-949), the 8% agreement test and the
-// unprojection of the source pixel at the recovered depth.
+// One lane per (sample, source); a work item is (problem, block of 256 samples, source),
+// source fastest, so every wave shares the source and its reverse homographies.
+// reciprocal_depth_observations (dense_legacy.rs:795-856): direct cost at the re-projected
+// reference point, projection into the source, the texture gate there, a reverse sweep over
+// the source's landmark envelope against the reference image (estimate_single_view_depth,
+// dense_legacy.rs:887-949), the 8% agreement test and the unprojection of the source pixel
+// at the recovered depth.
 template <int R, int COST, int ST>
 __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     constexpr int P = 2 * R + 1;
     constexpr int T = P * P;
-    const SolveDev &sd = *a.solve;
-    const uint32_t n = *a.n_samples;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const int s = (int)blockIdx.y;
-    if (i >= n) return;
+    const uint32_t item = xcd_chunk(blockIdx.x, gridDim.x);
+    if (item >= a.group_items) return;
+    const uint32_t p = problem_of_item<true>(a, item);
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t local = item - pd.recip_item0;
+    const int s = (int)(local % (uint32_t)pd.S);
+    const uint32_t i = (local / (uint32_t)pd.S) * kBlock + threadIdx.x;
+    if (i >= a.n_samples[p]) return;
     Recip out{{0.0, 0.0, 0.0}, 0, -1};
-    const Decision &d = a.decision[i];
-    const SourceDev &src = a.src[s];
-    const int D = sd.D;
-    const float maxc = sd.max_cost;
+    const Decision &d = a.decision[pd.base + i];
+    const SourceDev &src = pd.src[s];
+    const int D = pd.D;
+    const float maxc = pd.max_cost;
     bool ok = d.status == 0 && d.proj_ok;
     float rp[T];
-    if (ok) ok = sample_patch<R>(sd.ref.luma, sd.ref.w, sd.ref.h, d.xr, d.yr, rp);
+    if (ok) ok = sample_patch<R>(pd.ref.luma, pd.ref.w, pd.ref.h, d.xr, d.yr, rp);
     if (ok) {
         const RefStats st = ref_stats<COST, T>(rp);
         double H[9];
@@ -787,7 +896,7 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
         ok = c <= maxc;
     }
     double xs = 0.0, ys = 0.0, zs = 0.0;
-    if (ok) ok = project_d(src.R, src.t, d.pos, src.tex.w, src.tex.h, sd.focal, &xs, &ys, &zs);
+    if (ok) ok = project_d(src.R, src.t, d.pos, src.tex.w, src.tex.h, pd.focal, &xs, &ys, &zs);
     if (ok) ok = sample_patch<R>(src.tex.luma, src.tex.w, src.tex.h, xs, ys, rp);
     if (ok) {
         float mn = rp[0], mx = rp[0];
@@ -796,16 +905,16 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
             mn = fminf(mn, rp[k]);
             mx = fmaxf(mx, rp[k]);
         }
-        ok = (mx - mn) >= sd.min_contrast;
+        ok = (mx - mn) >= pd.min_contrast;
     }
     if (ok) {
         const RefStats st = ref_stats<COST, T>(rp);
         float best_c = kInfF;
         int best_h = -1;
-        const double *Hs = a.Hr + (size_t)s * D * 9;
+        const double *Hs = pd.Hr + (size_t)s * D * 9;
         for (int hyp = 0; hyp < D; ++hyp) {
             const float c = patch_cost<R, COST, ST>(rp, st, Hs + (size_t)hyp * 9, xs, ys,
-                                                    sd.ref.quad, sd.ref.w, sd.ref.h);
+                                                    pd.ref.quad, pd.ref.w, pd.ref.h);
             // cost ascending, then depth (hypothesis) ascending: strict < keeps the first
             if (c <= maxc && c < best_c) {
                 best_c = c;
@@ -815,18 +924,18 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
         ok = best_h >= 0;
         out.rh = best_h;
         if (ok) {
-            const double dh = a.depth_r[(size_t)s * D + best_h];
+            const double dh = pd.depth_r[(size_t)s * D + best_h];
             // reciprocal_depth_agrees, dense_legacy.rs:951-957.
             ok = isfinite(zs) && isfinite(dh) && zs > 0.0 && dh > 0.0 &&
-                 fabs(zs - dh) / fmax(fmax(zs, dh), 1.0e-9) <= sd.recip_rel;
+                 fabs(zs - dh) / fmax(fmax(zs, dh), 1.0e-9) <= pd.recip_rel;
             if (ok) {
-                unproject_d(src.R, src.t, xs, ys, dh, src.tex.w, src.tex.h, sd.focal, out.pos);
+                unproject_d(src.R, src.t, xs, ys, dh, src.tex.w, src.tex.h, pd.focal, out.pos);
                 ok = isfinite(out.pos[0]) && isfinite(out.pos[1]) && isfinite(out.pos[2]);
             }
         }
     }
     out.ok = ok ? 1 : 0;
-    a.recip[(size_t)s * a.max_samples + i] = out;
+    a.recip[pd.recip_base + (size_t)s * pd.cap + i] = out;
 }
 
 // ---- k_fuse --------------------------------------------------------------------------------
@@ -844,22 +953,23 @@ __device__ __forceinline__ uint32_t fnv1a(uint32_t h, uint32_t v) {
 // fuse_depth_observations (dense_legacy.rs:859-884) and the confidence and attributes of
 // dense_legacy.rs:387-410, in FP64; counters for DenseStats (dense_legacy.rs:363-385).
 __global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag) {
-    const SolveDev &sd = *a.solve;
-    const uint32_t n = *a.n_samples;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t p = a.block_problem[b];
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t i = (b - pd.block0) * kBlock + threadIdx.x;
     uint32_t c[kCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (i < n) {
-        const Decision d = a.decision[i];
+    if (i < a.n_samples[p]) {
+        const Decision d = a.decision[pd.base + i];
         SampleOut o{};
         o.status = d.status;
         o.error = d.best_cost;
         o.support = d.best_support;
         if (d.status == 0) {
-            const double radius = fmax(fabs(d.depth) * sd.fusion_rel, 1.0e-4);
+            const double radius = fmax(fabs(d.depth) * pd.fusion_rel, 1.0e-4);
             double sx = d.pos[0], sy = d.pos[1], sz = d.pos[2];
             uint32_t obs = 1, rej = 0, cnt = 0, mask = 0, sig = 2166136261u;
-            for (int s = 0; s < sd.S; ++s) {
-                const Recip r = a.recip[(size_t)s * a.max_samples + i];
+            for (int s = 0; s < pd.S; ++s) {
+                const Recip r = a.recip[pd.recip_base + (size_t)s * pd.cap + i];
                 if (r.rh >= 0) sig = fnv1a(fnv1a(sig, (uint32_t)s), (uint32_t)r.rh);
                 if (!r.ok) continue;
                 ++cnt;
@@ -884,16 +994,16 @@ __global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag)
             } else {
                 o.fused_obs = obs;
                 o.fused_rejected = rej;
-                if (obs < (uint32_t)sd.min_obs) {
+                if (obs < (uint32_t)pd.min_obs) {
                     o.status = VT3D_SAMPLE_REJECT_FUSION;
                 } else {
                     o.status = VT3D_SAMPLE_ACCEPTED;
-                    const double S = (double)sd.S;
+                    const double S = (double)pd.S;
                     const double support_conf = (double)d.best_support / S;
                     const double recip_conf = fmin(fmax((double)(obs - 1) / S, 0.0), 1.0);
-                    const double err_conf = 1.0 / (1.0 + (double)d.best_cost / sd.err_scale);
+                    const double err_conf = 1.0 / (1.0 + (double)d.best_cost / pd.err_scale);
                     const double dm =
-                        isfinite(d.margin) ? fmin(fmax(d.margin / sd.margin_sat, 0.0), 1.0) : 1.0;
+                        isfinite(d.margin) ? fmin(fmax(d.margin / pd.margin_sat, 0.0), 1.0) : 1.0;
                     const double conf =
                         fmin(fmax(support_conf * recip_conf * err_conf * fmax(dm, 0.2), 0.05), 1.0);
                     o.conf = (float)conf;
@@ -905,7 +1015,7 @@ __global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag)
                 }
             }
         }
-        a.out[i] = o;
+        a.out[pd.base + i] = o;
         const int st = o.status;
         c[0] = st == VT3D_SAMPLE_ACCEPTED;
         c[1] = st >= VT3D_SAMPLE_REJECT_RECIPROCAL;
@@ -914,8 +1024,8 @@ __global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag)
         c[4] = st >= VT3D_SAMPLE_REJECT_FUSION ? 1 + o.recip_count : 0;
         c[5] = st >= VT3D_SAMPLE_REJECT_FUSION ? o.fused_rejected : 0;
         c[6] = st == VT3D_SAMPLE_REJECT_FUSION;
-        if (diag) {
-            const uint32_t pk = a.samples[i];
+        if (diag && pd.emit_samples) {
+            const uint32_t pk = a.samples[pd.base + i];
             vt3d_sample_diag g;
             g.x = pk & 0xffffu;
             g.y = pk >> 16;
@@ -929,44 +1039,42 @@ __global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag)
             g.fused_observations = o.fused_obs;
             g.reciprocal_signature = o.recip_sig;
             g.pad = 0;
-            diag[i] = g;
+            diag[a.diag_offset[p] + i] = g;
         }
     }
     block_count(c, a.block_counts);
 }
 
-__global__ __launch_bounds__(kBlock) void k_compact_points(Args a, vt3d_point3 *points,
-                                                           vt3d_grid_site *sites,
-                                                           vt3d_point_attributes *attrs,
-                                                           float *errors, float *supports,
-                                                           float *fused_obs) {
-    const SolveDev &sd = *a.solve;
-    const uint32_t n = *a.n_samples;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const bool flag = i < n && a.out[i].status == VT3D_SAMPLE_ACCEPTED;
+// Accepted samples -> packed outputs in reference order (problem p's points start at
+// out_offset[p]).
+__global__ __launch_bounds__(kBlock) void k_compact_points(Args a, Outputs o) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t p = a.block_problem[b];
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t i = (b - pd.block0) * kBlock + threadIdx.x;
+    const bool flag = i < a.n_samples[p] && a.out[pd.base + i].status == VT3D_SAMPLE_ACCEPTED;
     const uint32_t r = block_rank(flag);
     if (!flag) return;
-    const uint32_t dst = a.block_offsets[blockIdx.x * kCounters] + r;
-    const SampleOut o = a.out[i];
-    const uint32_t pk = a.samples[i];
+    const uint32_t dst = a.out_offset[p] + a.block_offsets[b * kCounters] + r;
+    const SampleOut s = a.out[pd.base + i];
+    const uint32_t pk = a.samples[pd.base + i];
     const uint32_t x = pk & 0xffffu, y = pk >> 16;
-    const uint8_t *px = a.ref_rgba + ((size_t)y * a.ref_rgba_w + x) * 4;
-    vt3d_point3 p;
-    p.x = o.px;
-    p.y = o.py;
-    p.z = o.pz;
-    p.confidence = o.conf;
-    p.r = px[0];
-    p.g = px[1];
-    p.b = px[2];
-    p.pad = 0;
-    points[dst] = p;
-    sites[dst] = vt3d_grid_site{x, y};
-    attrs[dst] = vt3d_point_attributes{o.recip_conf, o.depth_margin};
-    errors[dst] = o.error;
-    supports[dst] = (float)o.support;
-    fused_obs[dst] = (float)o.fused_obs;
-    (void)sd;
+    const uint8_t *px = pd.ref_rgba + ((size_t)y * pd.ref.w + x) * 4;
+    vt3d_point3 pt;
+    pt.x = s.px;
+    pt.y = s.py;
+    pt.z = s.pz;
+    pt.confidence = s.conf;
+    pt.r = px[0];
+    pt.g = px[1];
+    pt.b = px[2];
+    pt.pad = 0;
+    o.points[dst] = pt;
+    o.sites[dst] = vt3d_grid_site{x, y};
+    o.attrs[dst] = vt3d_point_attributes{s.recip_conf, s.depth_margin};
+    o.errors[dst] = s.error;
+    o.supports[dst] = (float)s.support;
+    o.fused_obs[dst] = (float)s.fused_obs;
 }
 
 // ---- launchers ------------------------------------------------------------------------------
@@ -979,38 +1087,39 @@ void launch_build_view_t(const uint8_t *rgba, uint32_t w, uint32_t h, uint8_t *l
                        rgba, w, h, luma, quad);
 }
 
+static uint32_t xcd_grid(uint32_t items) { return (items + kXcds - 1) / kXcds * kXcds; }
+
 template <int R, int COST, int ST, int SMAX>
-void launch_sweep_s(const Args &a, uint32_t blocks, const TileLaunch *tl, hipStream_t s) {
-    if (tl) {
-        hipLaunchKernelGGL((k_sweep_tile<R, COST, ST, SMAX>), dim3(tl->tiles, a.n_hblocks),
-                           dim3(kBlock), tl->lds_bytes, s, a, tl->tile_w, tl->tiles_x);
-    } else {
-        hipLaunchKernelGGL((k_sweep<R, COST, ST, SMAX>), dim3(blocks, a.n_hblocks), dim3(kBlock),
-                           0, s, a);
-    }
+void launch_sweep_s(const Args &a, bool tiled, size_t lds_bytes, hipStream_t s) {
+    const uint32_t grid = xcd_grid(a.group_items);
+    if (tiled)
+        hipLaunchKernelGGL((k_sweep_tile<R, COST, ST, SMAX>), dim3(grid), dim3(kBlock), lds_bytes,
+                           s, a);
+    else
+        hipLaunchKernelGGL((k_sweep<R, COST, ST, SMAX>), dim3(grid), dim3(kBlock), 0, s, a);
 }
 
 template <int R, int COST, int ST>
-void launch_sweep_t(const Args &a, int smax, uint32_t blocks, const TileLaunch *tl,
-                    hipStream_t s) {
-    if (smax <= 4) launch_sweep_s<R, COST, ST, 4>(a, blocks, tl, s);
-    else if (smax <= 8) launch_sweep_s<R, COST, ST, 8>(a, blocks, tl, s);
-    else if (smax <= 16) launch_sweep_s<R, COST, ST, 16>(a, blocks, tl, s);
-    else launch_sweep_s<R, COST, ST, 32>(a, blocks, tl, s);
+void launch_sweep_t(const Args &a, int smax, bool tiled, size_t lds_bytes, hipStream_t s) {
+    if (smax <= 4) launch_sweep_s<R, COST, ST, 4>(a, tiled, lds_bytes, s);
+    else if (smax <= 8) launch_sweep_s<R, COST, ST, 8>(a, tiled, lds_bytes, s);
+    else if (smax <= 16) launch_sweep_s<R, COST, ST, 16>(a, tiled, lds_bytes, s);
+    else launch_sweep_s<R, COST, ST, 32>(a, tiled, lds_bytes, s);
 }
 
 template <int R, int COST, int ST>
-void launch_recip_t(const Args &a, int S, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL((k_recip<R, COST, ST>), dim3(blocks, S), dim3(kBlock), 0, s, a);
+void launch_recip_t(const Args &a, hipStream_t s) {
+    hipLaunchKernelGGL((k_recip<R, COST, ST>), dim3(xcd_grid(a.group_items)), dim3(kBlock), 0,
+                       s, a);
 }
 
 template <int COST, int ST>
-bool dispatch_r(int radius, const Args &a, int smax, int S, uint32_t blocks, bool recip,
-                const TileLaunch *tl, hipStream_t s) {
+bool dispatch_r(int radius, const Args &a, int smax, bool recip, bool tiled, size_t lds,
+                hipStream_t s) {
 #define VT3D_CASE(RR)                                                           \
     case RR:                                                                    \
-        if (recip) launch_recip_t<RR, COST, ST>(a, S, blocks, s);               \
-        else launch_sweep_t<RR, COST, ST>(a, smax, blocks, tl, s);              \
+        if (recip) launch_recip_t<RR, COST, ST>(a, s);                          \
+        else launch_sweep_t<RR, COST, ST>(a, smax, tiled, lds, s);              \
         return true;
     if constexpr (COST == VT3D_COST_ZSAD) {
         switch (radius) {
@@ -1033,22 +1142,24 @@ bool dispatch_r(int radius, const Args &a, int smax, int S, uint32_t blocks, boo
 }
 
 template <int ST>
-bool dispatch_cost(int cost, int radius, const Args &a, int smax, int S, uint32_t blocks,
-                   bool recip, const TileLaunch *tl, hipStream_t s) {
+bool dispatch_cost(int cost, int radius, const Args &a, int smax, bool recip, bool tiled,
+                   size_t lds, hipStream_t s) {
     if (cost == VT3D_COST_ZSAD)
-        return dispatch_r<VT3D_COST_ZSAD, ST>(radius, a, smax, S, blocks, recip, tl, s);
-    return dispatch_r<VT3D_COST_ZNCC, ST>(radius, a, smax, S, blocks, recip, tl, s);
+        return dispatch_r<VT3D_COST_ZSAD, ST>(radius, a, smax, recip, tiled, lds, s);
+    return dispatch_r<VT3D_COST_ZNCC, ST>(radius, a, smax, recip, tiled, lds, s);
 }
 
-bool launch_sweep_or_recip(int storage, int cost, int radius, const Args &a, int smax, int S,
-                           uint32_t blocks, bool recip, const TileLaunch *tl, hipStream_t s) {
+// One launch of the primary sweep (recip == false) or reciprocal kernel over the problem
+// group a.group_p0 .. a.group_p1.
+bool launch_group(int storage, int cost, int radius, int smax, bool recip, bool tiled,
+                  size_t lds, const Args &a, hipStream_t s) {
     switch (storage) {
     case VT3D_STORAGE_F32:
-        return dispatch_cost<VT3D_STORAGE_F32>(cost, radius, a, smax, S, blocks, recip, tl, s);
+        return dispatch_cost<VT3D_STORAGE_F32>(cost, radius, a, smax, recip, tiled, lds, s);
     case VT3D_STORAGE_F16:
-        return dispatch_cost<VT3D_STORAGE_F16>(cost, radius, a, smax, S, blocks, recip, tl, s);
+        return dispatch_cost<VT3D_STORAGE_F16>(cost, radius, a, smax, recip, tiled, lds, s);
     case VT3D_STORAGE_BF16:
-        return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, S, blocks, recip, tl, s);
+        return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, recip, tiled, lds, s);
     default: return false;
     }
 }
@@ -1074,26 +1185,26 @@ size_t tile_lds_bytes(int tile_w, int stride, int radius) {
     return 2ull * 4ull * (size_t)((tile_w - 1) * stride + P) * (size_t)((TY - 1) * stride + P);
 }
 
-void launch_gate(const Args &a, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_gate, dim3(blocks), dim3(kBlock), 0, s, a);
+void launch_gate(const Args &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_gate, dim3(a.n_blocks), dim3(kBlock), 0, s, a);
 }
-void launch_scan(const Args &a, uint32_t *n_out, hipStream_t s) {
-    hipLaunchKernelGGL(k_scan, dim3(1), dim3(1024), 0, s, a, n_out);
+void launch_scan(const Args &a, int gate, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan, dim3(a.n_prob), dim3(1024), 0, s, a, gate);
 }
-void launch_compact_sites(const Args &a, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_compact_sites, dim3(blocks), dim3(kBlock), 0, s, a);
+void launch_problem_offsets(const Args &a, int which, hipStream_t s) {
+    hipLaunchKernelGGL(k_problem_offsets, dim3(1), dim3(1024), 0, s, a, which);
 }
-void launch_decide(const Args &a, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_decide, dim3(blocks), dim3(kBlock), 0, s, a);
+void launch_compact_sites(const Args &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_sites, dim3(a.n_blocks), dim3(kBlock), 0, s, a);
 }
-void launch_fuse(const Args &a, vt3d_sample_diag *diag, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_fuse, dim3(blocks), dim3(kBlock), 0, s, a, diag);
+void launch_decide(const Args &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_decide, dim3(a.n_blocks), dim3(kBlock), 0, s, a);
 }
-void launch_compact_points(const Args &a, vt3d_point3 *points, vt3d_grid_site *sites,
-                           vt3d_point_attributes *attrs, float *errors, float *supports,
-                           float *fused_obs, uint32_t blocks, hipStream_t s) {
-    hipLaunchKernelGGL(k_compact_points, dim3(blocks), dim3(kBlock), 0, s, a, points, sites, attrs,
-                       errors, supports, fused_obs);
+void launch_fuse(const Args &a, vt3d_sample_diag *diag, hipStream_t s) {
+    hipLaunchKernelGGL(k_fuse, dim3(a.n_blocks), dim3(kBlock), 0, s, a, diag);
+}
+void launch_compact_points(const Args &a, const Outputs &o, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_points, dim3(a.n_blocks), dim3(kBlock), 0, s, a, o);
 }
 
 }  // namespace vt3d

@@ -86,6 +86,17 @@ def _bind(lib: C.CDLL) -> None:
     lib.vt3d_session_solve.restype = C.c_int32
     lib.vt3d_session_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
     lib.vt3d_session_last_timing.restype = C.c_int32
+    lib.vt3d_session_set_tuning.argtypes = [C.c_void_p, P(abi.Tuning)]
+    lib.vt3d_session_set_tuning.restype = C.c_int32
+    lib.vt3d_batch_create.argtypes = [C.c_void_p, P(abi.Request), C.c_size_t, C.c_int32,
+                                      P(C.c_void_p)]
+    lib.vt3d_batch_create.restype = C.c_int32
+    lib.vt3d_batch_run.argtypes = [C.c_void_p, P(P(abi.DenseResult))]
+    lib.vt3d_batch_run.restype = C.c_int32
+    lib.vt3d_batch_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
+    lib.vt3d_batch_last_timing.restype = C.c_int32
+    lib.vt3d_batch_destroy.argtypes = [C.c_void_p]
+    lib.vt3d_batch_destroy.restype = None
 
 
 EXPORTED_SYMBOLS = (
@@ -94,6 +105,8 @@ EXPORTED_SYMBOLS = (
     "vt3d_dense_reference_patch", "vt3d_dense_result_free", "vt3d_session_create",
     "vt3d_session_destroy", "vt3d_session_upload_frames", "vt3d_session_solve",
     "vt3d_session_last_timing", "vt3d_last_error", "vt3d_plane_homography",
+    "vt3d_session_set_tuning", "vt3d_batch_create", "vt3d_batch_run",
+    "vt3d_batch_last_timing", "vt3d_batch_destroy",
 )
 
 
@@ -201,6 +214,18 @@ class Session:
         _check(library().vt3d_session_last_timing(self._h, C.byref(t)))
         return t
 
+    def set_tuning(self, tile_width: int = 0, target_waves: int = 0) -> None:
+        """Launch shape of later solves and batches (0 keeps a default)."""
+        t = abi.Tuning(int(tile_width), int(target_waves))
+        _check(library().vt3d_session_set_tuning(self._h, C.byref(t)))
+
+    def batch(self, requests: Sequence[dict], use_graph: bool = False) -> "Batch":
+        """Prepares many solves on the uploaded frames as one pipeline. Each request is a
+        dict with cameras, sparse_points, focal and optionally frame_map (request frame ->
+        session frame, default identity), reference_camera_index (-1: plain sweep),
+        fine_grid and sweep."""
+        return Batch(self, requests, use_graph)
+
     def close(self) -> None:
         if self._h:
             library().vt3d_session_destroy(self._h)
@@ -211,3 +236,77 @@ class Session:
             self.close()
         except Exception:
             pass
+
+
+class Batch:
+    """Many solves on one session's frames, run as one kernel pipeline (optionally replayed
+    from a captured HIP graph). Build once, run many times."""
+
+    def __init__(self, session: Session, requests: Sequence[dict], use_graph: bool = False):
+        lib = library()
+        self._session = session  # the session must outlive the batch
+        self._keep = []
+        n = len(requests)
+        self._reqs = (abi.Request * max(1, n))()
+        n_frames = len(session._frames._rgba) if getattr(session, "_frames", None) else 0
+        for i, rq in enumerate(requests):
+            inp = abi.Inputs([], rq["cameras"], rq["sparse_points"])
+            fmap = rq.get("frame_map")
+            fmap = np.arange(n_frames, dtype=np.uint64) if fmap is None else \
+                np.ascontiguousarray(np.asarray(fmap, dtype=np.uint64))
+            sweep = rq.get("sweep")
+            self._keep += [inp, fmap, sweep]
+            r = self._reqs[i]
+            r.cameras = C.cast(inp.cameras, C.POINTER(abi.Camera))
+            r.n_cameras = inp.n_cameras
+            r.sparse_xyz = inp.sparse_ptr
+            r.n_sparse = inp.n_sparse
+            r.frame_map = fmap.ctypes.data_as(C.POINTER(C.c_uint64))
+            r.n_frames = fmap.shape[0]
+            r.focal = float(rq["focal"])
+            r.reference_camera_index = int(rq.get("reference_camera_index", -1))
+            r.fine_grid = int(bool(rq.get("fine_grid", False)))
+            r.params = C.pointer(sweep) if sweep is not None else C.POINTER(abi.SweepParams)()
+        self.n = n
+        self._h = C.c_void_p()
+        _check(lib.vt3d_batch_create(session._h, self._reqs, n, int(bool(use_graph)),
+                                     C.byref(self._h)))
+
+    def run(self) -> list:
+        """One DenseAnalysis per request (None where the reference remap returns None)."""
+        lib = library()
+        outs = (C.POINTER(abi.DenseResult) * max(1, self.n))()
+        _check(lib.vt3d_batch_run(self._h, outs))
+        return [_take(outs[i]) if outs[i] else None for i in range(self.n)]
+
+    def timing(self) -> abi.Timing:
+        t = abi.Timing()
+        _check(library().vt3d_batch_last_timing(self._h, C.byref(t)))
+        return t
+
+    def close(self) -> None:
+        if self._h:
+            library().vt3d_batch_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def pack_frames(frame_lists: Sequence[Sequence[np.ndarray]]):
+    """Deduplicates the frames of many solves (by array identity) into one list for a session
+    upload; returns (frames, maps) with maps[i][k] the session index of solve i's frame k."""
+    frames, index, maps = [], {}, []
+    for fl in frame_lists:
+        m = []
+        for f in fl:
+            key = id(f)
+            if key not in index:
+                index[key] = len(frames)
+                frames.append(f)
+            m.append(index[key])
+        maps.append(np.asarray(m, dtype=np.uint64))
+    return frames, maps

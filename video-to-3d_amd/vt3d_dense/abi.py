@@ -122,6 +122,19 @@ class Timing(C.Structure):
                 ("units", C.c_uint64)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("tile_width", C.c_int32), ("target_waves", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("cameras", C.POINTER(Camera)), ("n_cameras", C.c_size_t),
+                ("sparse_xyz", C.POINTER(C.c_double)), ("n_sparse", C.c_size_t),
+                ("frame_map", C.POINTER(C.c_uint64)), ("n_frames", C.c_size_t),
+                ("focal", C.c_double), ("reference_camera_index", C.c_int64),
+                ("fine_grid", C.c_int32), ("pad", C.c_int32),
+                ("params", C.POINTER(SweepParams))]
+
+
 POLL_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p)
 
 POINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("confidence", "<f4"),
