@@ -1,15 +1,22 @@
 #!/usr/bin/env python3
-"""Depth-hypothesis x sample throughput of the dense plane sweep on one MI355X.
+"""Depth-hypothesis x sample throughput of the dense plane sweep on one MI355X: batched
+FP16-storage solves against the plain FP32 kernel.
 
-Workload (default): the plain FP32 GPU kernel at native size -- 1 reference + 8 sources at
-640x360 rendered from a seeded 6-plane textured box room, handheld random-walk poses,
-128 inverse-depth hypotheses over 0.5-8 m, 7x7 ZNCC, stride 2, reciprocal check and fusion.
-A step is one solve of that reference (frames resident in HBM): selection, table upload, the
-kernel pipeline and the download of the results. Units = texture-gated samples x hypotheses.
+Two legs, both with frames resident in HBM; units = texture-gated samples x hypotheses.
 
-Prints one JSON line. `roofline` prices the sweep kernel (k_sweep) against the FP32 vector
-peak with the naive 24*S*P^2 FLOPs per unit; `cpu_baseline` times the FP64 oracle on a band
-of grid rows of the same workload on the host cores.
+* batched (the `value`): 120 keyframes of one seeded box-room clip, each the reference of a
+  solve with its 8 nearest frames; frame sizes a seeded mix of 640x360, 360x640 and 480x270;
+  96 inverse-depth hypotheses over 0.5-8 m, 7x7 ZNCC, stride 2, FP16 texel storage with FP32
+  accumulation, reciprocal check and fusion. All 120 solves form one batch whose launches are
+  captured in a HIP graph and replayed; the launch shape is autotuned. A step is one replay
+  plus the download and unpacking of all 120 results.
+* plain: 1 reference + 8 sources at 640x360 from a seeded 6-plane box room, 128 hypotheses,
+  7x7 ZNCC, stride 2, FP32 storage; a step is one solve (selection, table upload, kernel
+  pipeline, download).
+
+Prints one JSON line. `roofline` prices the batched leg's primary sweep kernel against the
+FP32 vector peak with 24*S*P^2 FLOPs per unit; `cpu_baseline` times the FP64 oracle on a band
+of grid rows of one batched keyframe on the host cores.
 """
 from __future__ import annotations
 
@@ -35,9 +42,9 @@ def _host_threads() -> int:
     return max(1, min(n, 16))
 
 
-def _traffic(workload: str):
-    """HBM bytes per k_sweep launch from the committed rocprofv3 --pmc summary, if any."""
-    path = os.path.join(REPO, "profiles", f"k_sweep_{workload}_traffic.json")
+def _traffic(name: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(REPO, "profiles", f"{name}_traffic.json")
     if not os.path.exists(path):
         return None
     with open(path) as f:
@@ -103,12 +110,132 @@ def copy_calibration(mib: int) -> int:
     return 0
 
 
+def _pct(vals, q):
+    v = sorted(vals)
+    return v[min(len(v) - 1, int(round(q * (len(v) - 1))))]
+
+
+def plain_leg(V, scenes, steps: int, warmup: int, storage: str):
+    """One C2-size solve per step on a session (the plain FP32 kernel)."""
+    problem = scenes.config_c2()
+    settings = dict(problem.settings, surface_completion_passes=0)
+    sweep = V.params(settings, storage=storage)
+    sess = V.Session(0)
+    sess.upload(problem.frames, storage)
+
+    def solve():
+        return sess.solve(problem.cameras, problem.sparse, problem.focal, problem.reference,
+                          sweep=sweep)
+
+    for _ in range(warmup):
+        solve()
+    walls, units, sweep_ms, recip_ms, dev_ms = [], [], [], [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        res = solve()
+        walls.append(time.perf_counter() - t0)
+        tm = sess.timing()
+        units.append(tm.units)
+        sweep_ms.append(tm.sweep_ms)
+        recip_ms.append(tm.reciprocal_ms)
+        dev_ms.append(tm.total_ms)
+    sess.close()
+    walls_ms = [w * 1e3 for w in walls]
+    return {
+        "value": sum(units) / sum(walls),
+        "device_value": sum(units) / (1e-3 * sum(dev_ms)),
+        "ms_per_step": statistics.mean(walls_ms),
+        "latency_ms": {"p50": _pct(walls_ms, 0.5), "p99": _pct(walls_ms, 0.99)},
+        "device_ms": {"total": statistics.mean(dev_ms), "sweep": statistics.mean(sweep_ms),
+                      "reciprocal": statistics.mean(recip_ms)},
+        "workload": "640x360 ref + 8 sources, 128 hyps 0.5-8 m, 7x7 ZNCC, stride 2, "
+                    f"{storage.upper()} storage, reciprocal + fusion",
+        "sampled_pixels": int(res.stats["sampled_pixels"]),
+        "accepted_points": int(res.stats["accepted_points"]),
+    }
+
+
+def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log):
+    """All `count` keyframe solves of the batched clip as one graph-replayed batch."""
+    from vt3d_dense import autotune
+
+    t0 = time.perf_counter()
+    problems = scenes.config_c3(count=count)
+    gen_s = time.perf_counter() - t0
+    frames, maps = V.pack_frames([p.frames for p in problems])
+    sess = V.Session(0)
+    sess.upload(frames, "f16")
+    reqs = []
+    for p, m in zip(problems, maps):
+        st = dict(p.settings, surface_completion_passes=0)
+        reqs.append({"cameras": p.cameras, "sparse_points": p.sparse, "focal": p.focal,
+                     "reference_camera_index": p.reference, "frame_map": m,
+                     "sweep": V.params(st)})
+    tuned = None
+    if tune:
+        tuned = autotune.tune(sess, reqs, use_graph=True, log=log)
+        reqs = tuned["requests"]
+    batch = sess.batch(reqs, use_graph=True)
+    for _ in range(max(1, warmup)):
+        batch.run()
+    walls, dev_ms = [], []
+    units = 0
+    for _ in range(steps):
+        t1 = time.perf_counter()
+        res = batch.run()
+        walls.append(time.perf_counter() - t1)
+        tm = batch.timing()
+        dev_ms.append(tm.total_ms)
+        units += tm.units
+    batch.close()
+    # Stage times of the same batch launched eagerly (HIP events between the stages on the
+    # session stream; the graph replays the same launches).
+    eager = sess.batch(reqs, use_graph=False)
+    eager.run()
+    sweep_ms, recip_ms, eager_ms = [], [], []
+    for _ in range(max(3, steps // 2)):
+        eager.run()
+        tm = eager.timing()
+        sweep_ms.append(tm.sweep_ms)
+        recip_ms.append(tm.reciprocal_ms)
+        eager_ms.append(tm.total_ms)
+    eager.close()
+    sess.close()
+    walls_ms = [w * 1e3 for w in walls]
+    per_step_units = units // max(1, steps)
+    ok = [r for r in res if r is not None]
+    return {
+        "value": units / sum(walls),
+        "device_value": units / (1e-3 * sum(dev_ms)),
+        "ms_per_step": statistics.mean(walls_ms),
+        "latency_ms": {"p50": _pct(walls_ms, 0.5), "p99": _pct(walls_ms, 0.99)},
+        "device_ms": {"graph_total": statistics.mean(dev_ms),
+                      "eager_total": statistics.mean(eager_ms),
+                      "sweep": statistics.mean(sweep_ms),
+                      "reciprocal": statistics.mean(recip_ms)},
+        "units_per_step": per_step_units,
+        "solves": len(problems),
+        "sampled_pixels": int(sum(r.stats["sampled_pixels"] for r in ok)),
+        "accepted_points": int(sum(r.stats["accepted_points"] for r in ok)),
+        "frames_resident": len(frames),
+        "scene_gen_s": gen_s,
+        "tuning": None if tuned is None else {"choice": tuned["choice"],
+                                             "cached": tuned["cached"],
+                                             "table": tuned.get("table")},
+        "problem0": problems[0],
+        "S": 8, "P": 7,
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--keyframes", type=int, default=120,
+                    help="solves in the batched leg (the batched workload is 120)")
+    ap.add_argument("--plain-storage", default="f32", choices=["f32", "f16", "bf16"])
+    ap.add_argument("--no-tune", action="store_true", help="default launch shape, no autotuner")
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="grid rows in the CPU-baseline sample (0: twice the host threads)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -123,87 +250,74 @@ def main() -> int:
               file=sys.stderr)
         return 2
 
-    import numpy as np
-
     import vt3d_dense as V
     from vt3d_dense import scenes
 
-    problem = scenes.config_c2()
-    settings = dict(problem.settings, surface_completion_passes=0)
-    sweep = V.params(settings, storage=args.storage)
-    sess = V.Session(0)
-    sess.upload(problem.frames, args.storage)
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
 
-    def solve():
-        return sess.solve(problem.cameras, problem.sparse, problem.focal, problem.reference,
-                          sweep=sweep)
-
-    for _ in range(args.warmup):
-        solve()
-    walls, units, sweep_ms, recip_ms, dev_ms = [], [], [], [], []
     t_all = time.perf_counter()
-    for _ in range(args.steps):
-        t0 = time.perf_counter()
-        res = solve()
-        walls.append(time.perf_counter() - t0)
-        tm = sess.timing()
-        units.append(tm.units)
-        sweep_ms.append(tm.sweep_ms)
-        recip_ms.append(tm.reciprocal_ms)
-        dev_ms.append(tm.total_ms)
-    elapsed = time.perf_counter() - t_all
-    value = sum(units) / sum(walls)
-    S = res.stats["source_views"]
-    D = res.stats["depth_hypotheses"]
-    P = 2 * sweep.patch_radius + 1
+    plain = plain_leg(V, scenes, args.steps, args.warmup, args.plain_storage)
+    log(f"plain leg: {plain['value']:.4g} units/s")
+    bat = batched_leg(V, scenes, args.steps, args.warmup, args.keyframes, not args.no_tune, log)
+    log(f"batched leg: {bat['value']:.4g} units/s")
+    S, P = bat.pop("S"), bat.pop("P")
+    problem0 = bat.pop("problem0")
     flops_per_unit = 24 * S * P * P
-    sweep_s = statistics.mean(sweep_ms) * 1e-3
-    achieved = units[-1] * flops_per_unit / sweep_s / 1e12
-    walls_ms = sorted(w * 1e3 for w in walls)
-
-    def pct(q):
-        return walls_ms[min(len(walls_ms) - 1, int(round(q * (len(walls_ms) - 1))))]
-
+    sweep_s = bat["device_ms"]["sweep"] * 1e-3
+    achieved = bat["units_per_step"] * flops_per_unit / sweep_s / 1e12
+    kernel = ("k_sweep_tile<3, ZNCC, F16, 8>"
+              if not bat["tuning"] or bat["tuning"]["choice"]["kernel"] == "tiled"
+              else "k_sweep<3, ZNCC, F16, 8>")
     line = {
         "metric": "depth-hyp×samples/s batched vs plain GPU + p50/p99 solve latency",
-        "value": value,
+        "value": bat["value"],
         "unit": "depth-hyp*samples/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * sum(walls) / len(walls),
+        "ms_per_step": bat["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.storage == "f32" else f"{args.storage}-storage/f32",
-        "data": "synthetic (seeded 6-plane box room, value-noise texture)",
-        "config": {"workload": "plain GPU kernel, 640x360 ref + 8 sources, 128 hyps 0.5-8 m, "
-                               "7x7 ZNCC, stride 2, reciprocal + fusion",
-                   "width": 640, "height": 360, "sources": int(S), "hypotheses": int(D),
-                   "patch": P, "stride": int(sweep.grid_stride), "storage": args.storage,
-                   "sampled_pixels": int(res.stats["sampled_pixels"]),
-                   "accepted_points": int(res.stats["accepted_points"])},
-        "latency_ms": {"p50": pct(0.5), "p99": pct(0.99)},
-        "device_ms": {"total": statistics.mean(dev_ms), "k_sweep": statistics.mean(sweep_ms),
-                      "reciprocal": statistics.mean(recip_ms)},
+        "dtype": "f16-storage/f32",
+        "data": "synthetic (seeded box-room clips, value-noise texture)",
+        "config": {"workload": f"batched: {bat['solves']} keyframes (640x360 / 360x640 / "
+                               "480x270 mix) each with its 8 nearest frames, 96 hyps 0.5-8 m, "
+                               "7x7 ZNCC, stride 2, FP16 storage, reciprocal + fusion, one "
+                               "hipGraph replay per step",
+                   "solves": bat["solves"], "sources": S, "hypotheses": 96, "patch": P,
+                   "stride": 2, "storage": "f16",
+                   "sampled_pixels": bat["sampled_pixels"],
+                   "accepted_points": bat["accepted_points"]},
+        "batched": bat,
+        "plain": plain,
+        "batched_vs_plain": bat["value"] / plain["value"],
+        "batched_vs_plain_device": bat["device_value"] / plain["device_value"],
+        "latency_ms": plain["latency_ms"],
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": _traffic("c2"),
-                     "kernel": "k_sweep", "flops_per_unit": flops_per_unit},
-        "timed_wall_s": elapsed,
+                     "traffic": _traffic("k_sweep_tile_c3"),
+                     "kernel": kernel, "flops_per_unit": flops_per_unit,
+                     "units_per_launch": bat["units_per_step"]},
     }
     if not args.no_cpu_baseline:
-        diag = sess.solve(problem.cameras, problem.sparse, problem.focal, problem.reference,
-                          sweep=V.params(settings, storage=args.storage, emit_samples=1))
         threads = _host_threads()
         rows = args.cpu_rows if args.cpu_rows > 0 else 2 * threads
-        cpu, agreement = cpu_baseline(problem, settings, rows, threads, diag.samples)
+        settings = dict(problem0.settings, surface_completion_passes=0, storage="f32")
+        sess = V.Session(0)
+        sess.upload(problem0.frames, "f16")
+        diag = sess.solve(problem0.cameras, problem0.sparse, problem0.focal, problem0.reference,
+                          sweep=V.params(settings, storage="f16", emit_samples=1))
+        sess.close()
+        cpu, agreement = cpu_baseline(problem0, settings, rows, threads, diag.samples)
+        cpu["sample"] = f"keyframe 0 ({problem0.name}): " + cpu["sample"]
         line["cpu_baseline"] = cpu
         if agreement:
             line["parity_band"] = agreement
     else:
         line["cpu_baseline"] = None
-    sess.close()
+    line["timed_wall_s"] = time.perf_counter() - t_all
     print(json.dumps(line), flush=True)
     return 0
 

@@ -14,6 +14,7 @@
 #include <limits>
 #include <memory>
 #include <string>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -123,6 +124,7 @@ struct Workspace {
     DevBuf block_counts, block_offsets, meta;
     DevBuf points, sites, attrs, errors, supports, fobs, diag;
     HostBuf meta_host;
+    HostBuf out_host;  // staging of the packed outputs
 };
 
 }  // namespace
@@ -992,7 +994,7 @@ int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
     }
     VT3D_HIP(hipStreamSynchronize(st));
 
-    // Packed outputs.
+    // Packed outputs: one copy per array into pinned staging memory.
     const uint32_t *meta = b.ws->meta_host.as<uint32_t>();
     const uint32_t *totals = meta;
     const uint32_t *n_samples = meta + n * kCounters;
@@ -1000,27 +1002,37 @@ int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
     const uint32_t *diag_offset = out_offset + n + 1;
     const uint32_t n_acc = n ? out_offset[n] : 0;
     const uint32_t n_diag = n ? diag_offset[n] : 0;
-    std::vector<vt3d_point3> points(n_acc);
-    std::vector<vt3d_grid_site> sites(n_acc);
-    std::vector<vt3d_point_attributes> attrs(n_acc);
-    std::vector<float> errs(n_acc), sups(n_acc), fob(n_acc);
-    std::vector<vt3d_sample_diag> diag(n_diag);
-    if (n_acc) {
-        VT3D_HIP(hipMemcpyAsync(points.data(), b.outs.points, sizeof(vt3d_point3) * n_acc,
-                                hipMemcpyDeviceToHost, st));
-        VT3D_HIP(hipMemcpyAsync(sites.data(), b.outs.sites, sizeof(vt3d_grid_site) * n_acc,
-                                hipMemcpyDeviceToHost, st));
-        VT3D_HIP(hipMemcpyAsync(attrs.data(), b.outs.attrs, sizeof(vt3d_point_attributes) * n_acc,
-                                hipMemcpyDeviceToHost, st));
-        VT3D_HIP(hipMemcpyAsync(errs.data(), b.outs.errors, 4ull * n_acc, hipMemcpyDeviceToHost, st));
-        VT3D_HIP(hipMemcpyAsync(sups.data(), b.outs.supports, 4ull * n_acc, hipMemcpyDeviceToHost, st));
-        VT3D_HIP(hipMemcpyAsync(fob.data(), b.outs.fused_obs, 4ull * n_acc, hipMemcpyDeviceToHost, st));
-    }
-    if (n_diag)
-        VT3D_HIP(hipMemcpyAsync(diag.data(), b.outs.diag, sizeof(vt3d_sample_diag) * n_diag,
-                                hipMemcpyDeviceToHost, st));
+    const size_t off_sites = sizeof(vt3d_point3) * (size_t)n_acc;
+    const size_t off_attrs = off_sites + sizeof(vt3d_grid_site) * (size_t)n_acc;
+    const size_t off_errs = off_attrs + sizeof(vt3d_point_attributes) * (size_t)n_acc;
+    const size_t off_sups = off_errs + 4ull * n_acc;
+    const size_t off_fobs = off_sups + 4ull * n_acc;
+    const size_t off_diag = (off_fobs + 4ull * n_acc + 15) & ~(size_t)15;
+    const size_t host_bytes = off_diag + sizeof(vt3d_sample_diag) * (size_t)n_diag;
+    if ((rc = b.ws->out_host.ensure(host_bytes))) return rc;
+    uint8_t *hb = b.ws->out_host.as<uint8_t>();
+    auto d2h = [&](size_t off, const void *src, size_t bytes) -> int {
+        if (bytes) VT3D_HIP(hipMemcpyAsync(hb + off, src, bytes, hipMemcpyDeviceToHost, st));
+        return VT3D_OK;
+    };
+    if ((rc = d2h(0, b.outs.points, off_sites)) ||
+        (rc = d2h(off_sites, b.outs.sites, off_attrs - off_sites)) ||
+        (rc = d2h(off_attrs, b.outs.attrs, off_errs - off_attrs)) ||
+        (rc = d2h(off_errs, b.outs.errors, 4ull * n_acc)) ||
+        (rc = d2h(off_sups, b.outs.supports, 4ull * n_acc)) ||
+        (rc = d2h(off_fobs, b.outs.fused_obs, 4ull * n_acc)) ||
+        (rc = d2h(off_diag, b.outs.diag, sizeof(vt3d_sample_diag) * (size_t)n_diag)))
+        return rc;
     VT3D_HIP(hipEventRecord(s->ev[6], st));
     VT3D_HIP(hipStreamSynchronize(st));
+    const vt3d_point3 *points = reinterpret_cast<const vt3d_point3 *>(hb);
+    const vt3d_grid_site *sites = reinterpret_cast<const vt3d_grid_site *>(hb + off_sites);
+    const vt3d_point_attributes *attrs =
+        reinterpret_cast<const vt3d_point_attributes *>(hb + off_attrs);
+    float *errs = reinterpret_cast<float *>(hb + off_errs);
+    float *sups = reinterpret_cast<float *>(hb + off_sups);
+    float *fob = reinterpret_cast<float *>(hb + off_fobs);
+    const vt3d_sample_diag *diag = reinterpret_cast<const vt3d_sample_diag *>(hb + off_diag);
 
     vt3d_timing &tm = b.timing;
     tm = vt3d_timing{};
@@ -1032,22 +1044,24 @@ int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
     tm.total_ms = elapsed(s->ev[0], s->ev[6]);
 
     // Medians (median_option, dense_legacy.rs:1176-1182): the sorted value at index
-    // round((n - 1) / 2), i.e. n / 2.
+    // round((n - 1) / 2), i.e. n / 2. The staging copies are scratch, so selection runs in
+    // place.
     auto median = [](float *v, size_t cnt, float *m) {
         if (cnt == 0) return 0;
         std::nth_element(v, v + cnt / 2, v + cnt);
         *m = v[cnt / 2];
         return 1;
     };
-    for (size_t i = 0; i < nreq; ++i) {
+    // One request's result from the staged arrays.
+    auto build = [&](size_t i) -> int {
         Prepared &pr = *b.prepared[i];
         if (pr.state == 1) {
-            if (pr.none) continue;
+            if (pr.none) return VT3D_OK;
             vt3d_dense_result *r = new_result();
             if (!r) return VT3D_E_DEVICE;
             std::memcpy(r, pr.host_result, sizeof(vt3d_dense_result));
             results[i] = r;
-            continue;
+            return VT3D_OK;
         }
         const uint32_t k = b.plan.slot_of[i];
         const uint32_t *tot = totals + (size_t)k * kCounters;
@@ -1061,15 +1075,15 @@ int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
             r->attributes =
                 (vt3d_point_attributes *)std::malloc(sizeof(vt3d_point_attributes) * na);
             if (!r->points || !r->grid_sites || !r->attributes) return VT3D_E_DEVICE;
-            std::memcpy(r->points, points.data() + o0, sizeof(vt3d_point3) * na);
-            std::memcpy(r->grid_sites, sites.data() + o0, sizeof(vt3d_grid_site) * na);
-            std::memcpy(r->attributes, attrs.data() + o0, sizeof(vt3d_point_attributes) * na);
+            std::memcpy(r->points, points + o0, sizeof(vt3d_point3) * na);
+            std::memcpy(r->grid_sites, sites + o0, sizeof(vt3d_grid_site) * na);
+            std::memcpy(r->attributes, attrs + o0, sizeof(vt3d_point_attributes) * na);
         }
         const uint32_t ns = n_samples[k];
         if (pr.p.emit_samples && ns) {
             r->samples = (vt3d_sample_diag *)std::malloc(sizeof(vt3d_sample_diag) * ns);
             if (!r->samples) return VT3D_E_DEVICE;
-            std::memcpy(r->samples, diag.data() + diag_offset[k], sizeof(vt3d_sample_diag) * ns);
+            std::memcpy(r->samples, diag + diag_offset[k], sizeof(vt3d_sample_diag) * ns);
             r->n_samples = ns;
         }
         vt3d_dense_stats &sx = r->stats;
@@ -1089,17 +1103,36 @@ int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
         sx.fusion_input_observations = tot[4];
         sx.fusion_rejected_observations = tot[5];
         sx.fusion_rejected_points = tot[6];
-        sx.has_median_fusion_observations =
-            median(fob.data() + o0, na, &sx.median_fusion_observations);
-        sx.has_median_supporting_views = median(sups.data() + o0, na, &sx.median_supporting_views);
-        sx.has_median_photometric_error =
-            median(errs.data() + o0, na, &sx.median_photometric_error);
+        sx.has_median_fusion_observations = median(fob + o0, na, &sx.median_fusion_observations);
+        sx.has_median_supporting_views = median(sups + o0, na, &sx.median_supporting_views);
+        sx.has_median_photometric_error = median(errs + o0, na, &sx.median_photometric_error);
         sx.has_search_depth = 1;
         sx.search_min_depth = (float)pr.sweep_min;
         sx.search_max_depth = (float)pr.sweep_max;
         r->n_points = na;
         r->primary_points = na;
-        tm.units += (uint64_t)ns * (uint64_t)pr.D;
+        return VT3D_OK;
+    };
+    // Large batches build their results on host threads (requests are independent).
+    const size_t n_threads =
+        std::min<size_t>({nreq, (size_t)std::max(1u, std::thread::hardware_concurrency()),
+                          (size_t)(n_acc / 65536 + 1), (size_t)16});
+    std::vector<int> rcs(std::max<size_t>(1, n_threads), VT3D_OK);
+    if (n_threads <= 1) {
+        for (size_t i = 0; i < nreq && rcs[0] == VT3D_OK; ++i) rcs[0] = build(i);
+    } else {
+        std::vector<std::thread> pool;
+        for (size_t t = 0; t < n_threads; ++t)
+            pool.emplace_back([&, t]() {
+                for (size_t i = t; i < nreq && rcs[t] == VT3D_OK; i += n_threads) rcs[t] = build(i);
+            });
+        for (std::thread &th : pool) th.join();
+    }
+    for (int e : rcs)
+        if (e != VT3D_OK) return e;
+    for (size_t i = 0; i < nreq; ++i) {
+        const Prepared &pr = *b.prepared[i];
+        if (pr.state == 0) tm.units += (uint64_t)n_samples[b.plan.slot_of[i]] * (uint64_t)pr.D;
     }
     return VT3D_OK;
 }

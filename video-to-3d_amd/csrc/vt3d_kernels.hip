@@ -349,23 +349,78 @@ __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 *
     return finish_cost<COST, T>(rp, st, s1, s2, srs, sv);
 }
 
-// Bilinear sample of a source texture at the FP64 warp of reference pixel (u, v) under the
-// homography H, exactly as patch_error evaluates one tap; NaN when the tap is invalid.
+// FP64 reciprocal: the hardware estimate refined by two Newton steps (each squares the
+// relative error), within an ulp or two of 1 / z for the positive normal z it is used on.
+__device__ __forceinline__ double recip_f64(double z) {
+    double r = __builtin_amdgcn_rcp(z);
+    double e = fma(-z, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-z, r, 1.0);
+    return fma(r, e, r);
+}
+
+typedef const __attribute__((address_space(1))) double gdouble;
+
+// A homography row-major in registers (loads through the global address space, so the
+// compiler keeps it out of the loops that write LDS).
+__device__ __forceinline__ void load_h(const double *__restrict__ H, double (&h)[9]) {
+    const gdouble *g = (const gdouble *)H;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) h[k] = g[k];
+}
+
+// Warps kWarpBatch pixels of a tile's region per thread and stores their bilinear source
+// values into LDS, exactly as patch_error evaluates one tap (FP64 homography, FP64 division
+// of the homogeneous coordinates; dense_legacy.rs:1052-1062); NaN marks an invalid tap
+// (source depth <= 1e-4 or outside [0, w-1) x [0, h-1)). Region pixel q (row-major, pitch
+// RW) sits at reference pixel (X0 + q % RW, Y0 + q / RW); thread t handles q = t + j * kBlock.
+// All of a batch's gathers are issued before any is consumed, and an invalid pixel gathers
+// texel 0 (always in bounds) and discards it, so the loop has no branches.
+constexpr int kWarpBatch = 6;
+
 template <int ST>
-__device__ __forceinline__ float warp_sample(const double *__restrict__ H, double u, double v,
-                                             const void *__restrict__ tex, uint32_t sw,
-                                             uint32_t sh) {
-    const double n = fma(H[0], u, fma(H[1], v, H[2]));
-    const double m = fma(H[3], u, fma(H[4], v, H[5]));
-    const double z = fma(H[6], u, fma(H[7], v, H[8]));
-    const float nan = __builtin_nanf("");
-    if (!(z > 1.0e-4)) return nan;
-    const double r = 1.0 / z;
-    const double xs = n * r, ys = m * r;
-    if (!(xs >= 0.0 && ys >= 0.0 && xs < (double)(sw - 1) && ys < (double)(sh - 1))) return nan;
-    const double fx = floor(xs), fy = floor(ys);
-    const uint32_t idx = (uint32_t)fy * sw + (uint32_t)fx;
-    return bicoef(Store<ST>::coef(tex, idx), (float)(xs - fx), (float)(ys - fy));
+__device__ __forceinline__ void warp_region(float *__restrict__ L, const double (&h)[9], int X0,
+                                            int Y0, int RW, int npx, int px0, int py0,
+                                            int step_x, int step_y, const void *__restrict__ tex,
+                                            uint32_t sw, uint32_t sh) {
+    const double wlim = (double)(sw - 1), hlim = (double)(sh - 1);
+    int px = px0, py = py0;
+    for (int q0 = (int)threadIdx.x; q0 < npx; q0 += kWarpBatch * kBlock) {
+        uint32_t idx[kWarpBatch];
+        float tx[kWarpBatch], ty[kWarpBatch];
+        bool ok[kWarpBatch];
+#pragma unroll
+        for (int j = 0; j < kWarpBatch; ++j) {
+            const bool in = q0 + j * kBlock < npx;
+            const double u = (double)(X0 + px), v = (double)(Y0 + py);
+            const double n = fma(h[0], u, fma(h[1], v, h[2]));
+            const double m = fma(h[3], u, fma(h[4], v, h[5]));
+            const double z = fma(h[6], u, fma(h[7], v, h[8]));
+            const double r = recip_f64(z > 1.0e-4 ? z : 1.0);
+            const double xs = n * r, ys = m * r;
+            const bool valid = in && z > 1.0e-4 && xs >= 0.0 && ys >= 0.0 && xs < wlim &&
+                               ys < hlim;
+            const double fx = floor(valid ? xs : 0.0), fy = floor(valid ? ys : 0.0);
+            ok[j] = valid;
+            tx[j] = (float)((valid ? xs : 0.0) - fx);
+            ty[j] = (float)((valid ? ys : 0.0) - fy);
+            idx[j] = (uint32_t)fy * sw + (uint32_t)fx;
+            px += step_x;
+            py += step_y;
+            const bool wrap = px >= RW;
+            px = wrap ? px - RW : px;
+            py = wrap ? py + 1 : py;
+        }
+        float4 k[kWarpBatch];
+#pragma unroll
+        for (int j = 0; j < kWarpBatch; ++j) k[j] = Store<ST>::coef(tex, idx[j]);
+        const float nan = __builtin_nanf("");
+#pragma unroll
+        for (int j = 0; j < kWarpBatch; ++j) {
+            const int q = q0 + j * kBlock;
+            if (q < npx) L[q] = ok[j] ? bicoef(k[j], tx[j], ty[j]) : nan;
+        }
+    }
 }
 
 // Patch cost from a tile of warped source values in LDS (row pitch `pitch` floats); NaN in
@@ -680,8 +735,9 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep(Args a) {
         for (int k = 0; k < SMAX; ++k) l[k] = kInfF;
         for (int s = 0; s < S; ++s) {
             const SourceDev &src = pd.src[s];
-            const double *H = pd.Hp + ((size_t)s * D + hyp) * 9;
-            float c = patch_cost<R, COST, ST>(rp, st, H, cx, cy, src.tex.quad, src.tex.w,
+            double h[9];
+            load_h(pd.Hp + ((size_t)s * D + hyp) * 9, h);
+            float c = patch_cost<R, COST, ST>(rp, st, h, cx, cy, src.tex.quad, src.tex.w,
                                               src.tex.h);
             c = c <= maxc ? c : kInfF;
             sorted_insert<SMAX>(l, c);
@@ -753,18 +809,11 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep_tile(Args a) {
         for (int k = 0; k < SMAX; ++k) l[k] = kInfF;
         for (int s = 0; s < S; ++s) {
             const SourceDev &src = pd.src[s];
-            const double *H = pd.Hp + ((size_t)s * D + hyp) * 9;
+            double h[9];
+            load_h(pd.Hp + ((size_t)s * D + hyp) * 9, h);
             float *L = lds + buf * npx;
-            for (int px = px0, py = py0; py < RH;) {
-                L[py * RW + px] = warp_sample<ST>(H, (double)(X0 + px), (double)(Y0 + py),
-                                                  src.tex.quad, src.tex.w, src.tex.h);
-                px += step_x;
-                py += step_y;
-                if (px >= RW) {
-                    px -= RW;
-                    ++py;
-                }
-            }
+            warp_region<ST>(L, h, X0, Y0, RW, npx, px0, py0, step_x, step_y, src.tex.quad,
+                            src.tex.w, src.tex.h);
             __syncthreads();
             float c = kInfF;
             if (active) c = lds_patch_cost<R, COST>(mine + buf * npx, RW, rp, st);
@@ -913,8 +962,10 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
         int best_h = -1;
         const double *Hs = pd.Hr + (size_t)s * D * 9;
         for (int hyp = 0; hyp < D; ++hyp) {
-            const float c = patch_cost<R, COST, ST>(rp, st, Hs + (size_t)hyp * 9, xs, ys,
-                                                    pd.ref.quad, pd.ref.w, pd.ref.h);
+            double h[9];
+            load_h(Hs + (size_t)hyp * 9, h);
+            const float c = patch_cost<R, COST, ST>(rp, st, h, xs, ys, pd.ref.quad, pd.ref.w,
+                                                    pd.ref.h);
             // cost ascending, then depth (hypothesis) ascending: strict < keeps the first
             if (c <= maxc && c < best_c) {
                 best_c = c;

@@ -128,11 +128,26 @@ def _check(rc: int) -> None:
         raise Vt3dError(f"vt3d call failed ({rc}): {msg}")
 
 
+class _ResultOwner:
+    """Frees a C result once no array viewing its buffers is left."""
+
+    __slots__ = ("ptr", "free")
+
+    def __init__(self, ptr, free):
+        self.ptr = ptr
+        self.free = free
+
+    def __del__(self):
+        if self.ptr:
+            self.free(self.ptr)
+            self.ptr = None
+
+
 def _take(ptr) -> DenseAnalysis:
-    try:
-        return abi.unpack_result(ptr.contents)
-    finally:
-        library().vt3d_dense_result_free(ptr)
+    """Unpacks a C result; its arrays view the C buffers, which are freed with the last
+    array (or at once if the result has none)."""
+    owner = _ResultOwner(ptr, library().vt3d_dense_result_free)
+    return abi.unpack_result(ptr.contents, owner)
 
 
 def _rgba(frames: Sequence[np.ndarray]):

@@ -163,14 +163,21 @@ class DenseAnalysis:
     skip_code: int
 
 
-def _array(ptr, n: int, dtype: np.dtype) -> np.ndarray:
+def _array(ptr, n: int, dtype: np.dtype, owner=None) -> np.ndarray:
+    """n records at ptr. With an owner, a view of the C buffer that keeps the owner (which
+    frees the buffer) alive; otherwise a copy."""
     if n == 0 or not ptr:
         return np.zeros(0, dtype)
-    buf = C.string_at(ptr, n * dtype.itemsize)
-    return np.frombuffer(buf, dtype=dtype).copy()
+    raw = (C.c_uint8 * (n * dtype.itemsize)).from_address(C.cast(ptr, C.c_void_p).value)
+    if owner is None:
+        return np.frombuffer(raw, dtype=dtype).copy()
+    raw._owner = owner
+    return np.frombuffer(raw, dtype=dtype)
 
 
-def unpack_result(res: DenseResult) -> DenseAnalysis:
+def unpack_result(res: DenseResult, owner=None) -> DenseAnalysis:
+    """DenseAnalysis of a C result. With `owner` (an object that frees the result when it is
+    collected) the point arrays are views of the C buffers; without, copies."""
     st = res.stats
     stats: Dict[str, object] = {}
     for name, _ in DenseStats._fields_:
@@ -190,11 +197,12 @@ def unpack_result(res: DenseResult) -> DenseAnalysis:
     reason = res.skip_reason.decode()
     stats["skip_reason"] = reason if res.skip_code else None
     n = int(res.n_points)
-    samples = _array(res.samples, int(res.n_samples), SAMPLE_DTYPE) if res.samples else None
-    return DenseAnalysis(stats, _array(res.points, n, POINT_DTYPE),
-                         _array(res.grid_sites, n, SITE_DTYPE),
-                         _array(res.attributes, n, ATTR_DTYPE), int(res.primary_points), samples,
-                         int(res.skip_code))
+    samples = (_array(res.samples, int(res.n_samples), SAMPLE_DTYPE, owner) if res.samples
+               else None)
+    return DenseAnalysis(stats, _array(res.points, n, POINT_DTYPE, owner),
+                         _array(res.grid_sites, n, SITE_DTYPE, owner),
+                         _array(res.attributes, n, ATTR_DTYPE, owner), int(res.primary_points),
+                         samples, int(res.skip_code))
 
 
 class Inputs:

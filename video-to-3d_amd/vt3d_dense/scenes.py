@@ -363,28 +363,45 @@ def config_c2(seed: int = SEED_C2) -> Problem:
 C3_SIZES = ((640, 360), (360, 640), (480, 270))
 
 
-def config_c3(count: int = 120, seed: int = SEED_C3, sources: int = 8) -> List[Problem]:
+def _render_many(planes, jobs, seed: int, threads: int):
+    """Renders (camera, W, H, f) jobs on a thread pool; numpy releases the GIL in the large
+    array operations of render(), so threads scale on the host cores."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    if threads <= 0:
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(threads) as ex:
+        return list(ex.map(lambda j: render(planes, j[0], j[1], j[2], j[3], seed)[0], jobs))
+
+
+def config_c3(count: int = 120, seed: int = SEED_C3, sources: int = 8,
+              threads: int = 0) -> List[Problem]:
     """`count` keyframes of one room clip, each the reference with its `sources` nearest
-    frames; each keyframe solve has a seeded size class (landscape, portrait, small)."""
+    frames; each keyframe solve has a seeded size class (landscape, portrait, small). Solves
+    of the same size class share frame arrays."""
     rng = np.random.default_rng(seed)
     n_frames = count + sources
     cams = random_walk(n_frames, seed, sigma_t=0.02)
     planes = room_planes(seed)
     size_of = [C3_SIZES[int(k)] for k in rng.integers(0, len(C3_SIZES), count)]
-    cache = {}
-    problems = []
     half = sources // 2
+    keys = []
+    for k in range(count):
+        W, H = size_of[k]
+        for i in range(k, k + sources + 1):
+            if (i, W, H) not in keys:
+                keys.append((i, W, H))
+    imgs = _render_many(planes, [(cams[i], W, H, 0.86 * max(W, H)) for i, W, H in keys], seed,
+                        threads)
+    cache = dict(zip(keys, imgs))
+    problems = []
     for k in range(count):
         W, H = size_of[k]
         f = 0.86 * max(W, H)
         ref = k + half
         idx = list(range(ref - half, ref - half + sources + 1))
-        frames = []
-        for i in idx:
-            key = (i, W, H)
-            if key not in cache:
-                cache[key] = render(planes, cams[i], W, H, f, seed)[0]
-            frames.append(cache[key])
+        frames = [cache[(i, W, H)] for i in idx]
         sub = [cams[i] for i in idx]
         sparse = landmarks(planes, sub, [(W, H)] * len(sub), f, 512, seed + k)
         problems.append(Problem(f"c3-key{k}-{W}x{H}", frames, sub, sparse, f, ref - idx[0],
