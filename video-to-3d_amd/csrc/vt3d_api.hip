@@ -336,8 +336,8 @@ struct FrameSource {
 
 int upload_view(vt3d_session *s, DevView &v, const vt3d_frame &f, int storage) {
     if (!f.rgba || f.width < 2 || f.height < 2 || f.width > 65535 || f.height > 65535 ||
-        (size_t)f.width * f.height > (1u << 28)) {
-        set_error("frame has no pixels, is under 2 px, or exceeds 65535 px or 2^28 pixels");
+        (size_t)f.width * f.height > (1u << 27)) {
+        set_error("frame has no pixels, is under 2 px, or exceeds 65535 px or 2^27 pixels");
         return VT3D_E_INVALID;
     }
     const size_t n = (size_t)f.width * f.height;

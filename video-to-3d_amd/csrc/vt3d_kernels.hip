@@ -29,7 +29,7 @@ constexpr float kInfF = __builtin_huge_valf();
 //
 // Every pixel (x, y) of a view stores the bilinear coefficients of the cell it anchors:
 //   a = I[x,y], b = I[x+1,y] - a, c = I[x,y+1] - a, d = I[x+1,y+1] - I[x+1,y] - I[x,y+1] + a
-// so the sample at (x + tx, y + ty) is a + tx b + ty c + tx ty d -- one gather and four FMA
+// so the sample at (x + tx, y + ty) is (a + tx b) + ty (c + tx d) -- one gather and three FMA
 // per tap. Luma is an integer 0..255, so a..d are integers in [-510, 510] and FP16 holds them
 // exactly. BF16 (8 significant bits) holds integers only up to 256 exactly, so BF16 textures
 // store the four raw neighbours (exact) and form the coefficients after the load.
@@ -93,6 +93,59 @@ template <> struct Store<VT3D_STORAGE_BF16> {
     }
 };
 
+// A texture for gathers, in one of two forms (measured per kernel, neither wins everywhere):
+//  BUF: a buffer resource over the coefficient texels. Offsets are 32-bit byte offsets (no
+//       64-bit address arithmetic per tap) and the hardware range check returns zeros for an
+//       offset past the texture instead of touching memory beyond it. The base address must
+//       be wave-uniform (every caller passes one texture per wave).
+//  global: 64-bit addresses through the global address space, the texel index clamped to the
+//       texture.
+constexpr uint32_t kBufferFlags = 0x00020000u;  // gfx9 buffer descriptor word 3
+
+template <int ST, bool BUF> struct TexRef;
+
+template <int ST> struct TexRef<ST, true> {
+    static constexpr uint32_t kTexelBytes = ST == VT3D_STORAGE_F32 ? 16u : 8u;
+    __amdgpu_buffer_rsrc_t rsrc;
+    uint32_t w, h;
+    __device__ __forceinline__ TexRef(const void *q, uint32_t w_, uint32_t h_) : w(w_), h(h_) {
+        const uint64_t a = reinterpret_cast<uint64_t>(q);
+        const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
+        const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
+        void *base = reinterpret_cast<void *>(((uint64_t)hi << 32) | lo);
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)(w_ * h_ * kTexelBytes),
+                                                 (int)kBufferFlags);
+    }
+    // Coefficients (a, b, c, d) of texel `i` (see Store).
+    __device__ __forceinline__ float4 coef(uint32_t i) const {
+        const uint32_t off = i * kTexelBytes;
+        if constexpr (ST == VT3D_STORAGE_F32) {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)off, 0, 0);
+            return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]),
+                               __uint_as_float(v[2]), __uint_as_float(v[3]));
+        } else {
+            const auto v = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)off, 0, 0);
+            if constexpr (ST == VT3D_STORAGE_F16) {
+                return make_float4(half_lo(v[0]), half_hi(v[0]), half_lo(v[1]), half_hi(v[1]));
+            } else {
+                const float a = __uint_as_float(v[0] << 16), b = __uint_as_float(v[0] & 0xffff0000u);
+                const float c = __uint_as_float(v[1] << 16), d = __uint_as_float(v[1] & 0xffff0000u);
+                return make_float4(a, b - a, c - a, d - b - c + a);
+            }
+        }
+    }
+};
+
+template <int ST> struct TexRef<ST, false> {
+    const void *q;
+    uint32_t w, h;
+    __device__ __forceinline__ TexRef(const void *q_, uint32_t w_, uint32_t h_)
+        : q(q_), w(w_), h(h_) {}
+    __device__ __forceinline__ float4 coef(uint32_t i) const {
+        return Store<ST>::coef(q, min(i, w * h - 1u));
+    }
+};
+
 // floor(x) as an int in one instruction (the compiler emits v_floor + v_cvt for the C form).
 // Callers pass finite values within int range.
 __device__ __forceinline__ int floor_to_int(float x) {
@@ -103,7 +156,9 @@ __device__ __forceinline__ int floor_to_int(float x) {
 
 // Bilinear value from a coefficient texel.
 __device__ __forceinline__ float bicoef(float4 k, float tx, float ty) {
-    return fmaf(tx * ty, k.w, fmaf(ty, k.z, fmaf(tx, k.y, k.x)));
+    // (a + tx b) + ty (c + tx d): three FMAs (with FP16 texels, three mixed-precision FMAs
+    // that read the halves directly).
+    return fmaf(ty, fmaf(tx, k.w, k.z), fmaf(tx, k.y, k.x));
 }
 
 __device__ __forceinline__ float bilerp(float4 q, float tx, float ty) {
@@ -138,12 +193,12 @@ __global__ __launch_bounds__(kBlock) void k_build_view(const uint8_t *__restrict
 
 // ---- patch statistics and cost ------------------------------------------------------------
 
-// Occupancy cap of the sweep kernels: at most 512 / VT3D_SWEEP_WAVES VGPRs per lane, so the
-// compiler keeps fewer gathers in flight per wave and the SIMD holds more waves instead.
-#ifndef VT3D_SWEEP_WAVES
-#define VT3D_SWEEP_WAVES 4
-#endif
-#define VT3D_WAVES __attribute__((amdgpu_waves_per_eu(VT3D_SWEEP_WAVES)))
+// Occupancy caps (waves per SIMD; a cap of n allows 512 / n VGPRs per lane). The direct
+// sweep and the reciprocal kernel run best at 4, where the compiler keeps fewer gathers in
+// flight per wave and the SIMD holds more waves; the tiled sweep at 3, where its warp batch
+// and reference patch fit without spilling.
+#define VT3D_WAVES __attribute__((amdgpu_waves_per_eu(4)))
+#define VT3D_TILE_WAVES __attribute__((amdgpu_waves_per_eu(3)))
 
 struct RefStats {
     float mean;     // mean of the raw reference values (the ZNCC accumulation pivot)
@@ -211,6 +266,16 @@ __device__ __forceinline__ bool sample_patch(const uint8_t *__restrict__ l, uint
     return true;
 }
 
+// FP64 reciprocal: the hardware estimate refined by two Newton steps (each squares the
+// relative error), within an ulp or two of 1 / z for the positive normal z it is used on.
+__device__ __forceinline__ double recip_f64(double z) {
+    double r = __builtin_amdgcn_rcp(z);
+    double e = fma(-z, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-z, r, 1.0);
+    return fma(r, e, r);
+}
+
 // Centre warp of one patch: FP64 homography at the centre, FP32 tap offsets from it.
 struct Warp {
     int ixc, iyc;
@@ -232,7 +297,7 @@ __device__ __forceinline__ Warp make_warp(const double *__restrict__ H, double c
     Warp w;
     // The centre is a tap: project() rejects it unless its source depth exceeds 1e-4.
     w.ok = dc > 1.0e-4;
-    const double rd = 1.0 / (w.ok ? dc : 1.0);
+    const double rd = recip_f64(w.ok ? dc : 1.0);
     const double xc = nc * rd, yc = mc * rd;
     w.ok = w.ok && xc > -4096.0 && xc < (double)sw + 4096.0 && yc > -4096.0 &&
            yc < (double)sh + 4096.0;
@@ -278,15 +343,16 @@ __device__ __forceinline__ float finish_cost(const float (&rp)[T], const RefStat
 // Validity is decided on the four corner taps before any texel is read. Over a patch the
 // source depth is affine in (dx, dy), so its minimum is at a corner; with that depth positive
 // the warp is linear-fractional, so the extremes of the source x and y are at corners too.
-// Interior taps therefore lie inside the corners' box; their texel index is still clamped so
-// that FP32 rounding at a box edge can never address outside the texture.
-template <int R, int COST, int ST>
+// Interior taps therefore lie inside the corners' box; should FP32 rounding at a box edge
+// put one a texel outside the texture, the gather reads zeros (buffer form) or the last
+// texel (global form) instead of memory beyond the texture.
+template <int R, int COST, int ST, bool BUF>
 __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
                                             const RefStats &st, const double *__restrict__ H,
-                                            double cx, double cy, const void *__restrict__ tex,
-                                            uint32_t sw, uint32_t sh) {
+                                            double cx, double cy, const TexRef<ST, BUF> &tex) {
     constexpr int P = 2 * R + 1;
     constexpr int T = P * P;
+    const uint32_t sw = tex.w, sh = tex.h;
     const Warp w = make_warp(H, cx, cy, sw, sh);
     bool ok = w.ok;
     const float fr = (float)R;
@@ -303,7 +369,6 @@ __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 *
              qy < (float)((int)sh - 1 - w.iyc);
     }
     if (!ok) return kInfF;
-    const uint32_t nlast = sw * sh - 1u;
     const uint32_t base = (uint32_t)(w.iyc * (int)sw + w.ixc);
     const float pivot = st.mean;
     float s1 = 0.f, s2 = 0.f, srs = 0.f;
@@ -327,9 +392,7 @@ __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 *
             const int ixo = floor_to_int(qx), iyo = floor_to_int(qy);
             tx[dx + R] = __builtin_amdgcn_fractf(qx);
             ty[dx + R] = __builtin_amdgcn_fractf(qy);
-            const uint32_t idx =
-                min(base + (uint32_t)__mul24(iyo, (int)sw) + (uint32_t)ixo, nlast);
-            kk[dx + R] = Store<ST>::coef(tex, idx);
+            kk[dx + R] = tex.coef(base + (uint32_t)__mul24(iyo, (int)sw) + (uint32_t)ixo);
         }
 #pragma unroll
         for (int dx = -R; dx <= R; ++dx) {
@@ -347,16 +410,6 @@ __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 *
         }
     }
     return finish_cost<COST, T>(rp, st, s1, s2, srs, sv);
-}
-
-// FP64 reciprocal: the hardware estimate refined by two Newton steps (each squares the
-// relative error), within an ulp or two of 1 / z for the positive normal z it is used on.
-__device__ __forceinline__ double recip_f64(double z) {
-    double r = __builtin_amdgcn_rcp(z);
-    double e = fma(-z, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-z, r, 1.0);
-    return fma(r, e, r);
 }
 
 typedef const __attribute__((address_space(1))) double gdouble;
@@ -378,12 +431,12 @@ __device__ __forceinline__ void load_h(const double *__restrict__ H, double (&h)
 // texel 0 (always in bounds) and discards it, so the loop has no branches.
 constexpr int kWarpBatch = 6;
 
-template <int ST>
+template <int ST, bool BUF>
 __device__ __forceinline__ void warp_region(float *__restrict__ L, const double (&h)[9], int X0,
                                             int Y0, int RW, int npx, int px0, int py0,
-                                            int step_x, int step_y, const void *__restrict__ tex,
-                                            uint32_t sw, uint32_t sh) {
-    const double wlim = (double)(sw - 1), hlim = (double)(sh - 1);
+                                            int step_x, int step_y, const TexRef<ST, BUF> &tex) {
+    const uint32_t sw = tex.w;
+    const double wlim = (double)(tex.w - 1), hlim = (double)(tex.h - 1);
     int px = px0, py = py0;
     for (int q0 = (int)threadIdx.x; q0 < npx; q0 += kWarpBatch * kBlock) {
         uint32_t idx[kWarpBatch];
@@ -413,7 +466,7 @@ __device__ __forceinline__ void warp_region(float *__restrict__ L, const double 
         }
         float4 k[kWarpBatch];
 #pragma unroll
-        for (int j = 0; j < kWarpBatch; ++j) k[j] = Store<ST>::coef(tex, idx[j]);
+        for (int j = 0; j < kWarpBatch; ++j) k[j] = tex.coef(idx[j]);
         const float nan = __builtin_nanf("");
 #pragma unroll
         for (int j = 0; j < kWarpBatch; ++j) {
@@ -737,8 +790,8 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep(Args a) {
             const SourceDev &src = pd.src[s];
             double h[9];
             load_h(pd.Hp + ((size_t)s * D + hyp) * 9, h);
-            float c = patch_cost<R, COST, ST>(rp, st, h, cx, cy, src.tex.quad, src.tex.w,
-                                              src.tex.h);
+            const TexRef<ST, false> tex(src.tex.quad, src.tex.w, src.tex.h);
+            float c = patch_cost<R, COST, ST, false>(rp, st, h, cx, cy, tex);
             c = c <= maxc ? c : kInfF;
             sorted_insert<SMAX>(l, c);
         }
@@ -757,7 +810,7 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep(Args a) {
 // chunks: each XCD sweeps a contiguous strip of tiles over all hypotheses, so its L2 holds
 // one strip's source footprints instead of the whole image's.
 template <int R, int COST, int ST, int SMAX>
-__global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep_tile(Args a) {
+__global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
     extern __shared__ float lds[];
     constexpr int P = 2 * R + 1;
     constexpr int T = P * P;
@@ -812,8 +865,8 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep_tile(Args a) {
             double h[9];
             load_h(pd.Hp + ((size_t)s * D + hyp) * 9, h);
             float *L = lds + buf * npx;
-            warp_region<ST>(L, h, X0, Y0, RW, npx, px0, py0, step_x, step_y, src.tex.quad,
-                            src.tex.w, src.tex.h);
+            const TexRef<ST, false> tex(src.tex.quad, src.tex.w, src.tex.h);
+            warp_region<ST, false>(L, h, X0, Y0, RW, npx, px0, py0, step_x, step_y, tex);
             __syncthreads();
             float c = kInfF;
             if (active) c = lds_patch_cost<R, COST>(mine + buf * npx, RW, rp, st);
@@ -930,6 +983,8 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     Recip out{{0.0, 0.0, 0.0}, 0, -1};
     const Decision &d = a.decision[pd.base + i];
     const SourceDev &src = pd.src[s];
+    const TexRef<ST, true> src_tex(src.tex.quad, src.tex.w, src.tex.h);
+    const TexRef<ST, true> ref_tex(pd.ref.quad, pd.ref.w, pd.ref.h);
     const int D = pd.D;
     const float maxc = pd.max_cost;
     bool ok = d.status == 0 && d.proj_ok;
@@ -940,8 +995,7 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
         double H[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) H[k] = fma(d.dr, src.A[k], src.B[k]);
-        const float c = patch_cost<R, COST, ST>(rp, st, H, d.xr, d.yr, src.tex.quad, src.tex.w,
-                                                src.tex.h);
+        const float c = patch_cost<R, COST, ST, true>(rp, st, H, d.xr, d.yr, src_tex);
         ok = c <= maxc;
     }
     double xs = 0.0, ys = 0.0, zs = 0.0;
@@ -964,8 +1018,7 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
         for (int hyp = 0; hyp < D; ++hyp) {
             double h[9];
             load_h(Hs + (size_t)hyp * 9, h);
-            const float c = patch_cost<R, COST, ST>(rp, st, h, xs, ys, pd.ref.quad, pd.ref.w,
-                                                    pd.ref.h);
+            const float c = patch_cost<R, COST, ST, true>(rp, st, h, xs, ys, ref_tex);
             // cost ascending, then depth (hypothesis) ascending: strict < keeps the first
             if (c <= maxc && c < best_c) {
                 best_c = c;
