@@ -2,7 +2,7 @@
 """Depth-hypothesis x sample throughput of the dense plane sweep on one MI355X: batched
 FP16-storage solves against the plain FP32 kernel.
 
-Two legs, both with frames resident in HBM; units = texture-gated samples x hypotheses.
+Three legs, all with frames resident in HBM; units = texture-gated samples x hypotheses.
 
 * batched (the `value`): 120 keyframes of one seeded box-room clip, each the reference of a
   solve with its 8 nearest frames; frame sizes a seeded mix of 640x360, 360x640 and 480x270;
@@ -13,6 +13,10 @@ Two legs, both with frames resident in HBM; units = texture-gated samples x hypo
 * plain: 1 reference + 8 sources at 640x360 from a seeded 6-plane box room, 128 hypotheses,
   7x7 ZNCC, stride 2, FP32 storage; a step is one solve (selection, table upload, kernel
   pipeline, download).
+* serving: a Poisson stream (200/s) of solves from 16 clips of uneven size, sources,
+  hypotheses and patch size, admitted by the continuous-batching server under a 64 GB HBM
+  budget; `latency_ms` is its per-request latency (submit -> result on the host), then the
+  same requests submitted at once give the served throughput.
 
 Prints one JSON line. `roofline` prices the batched leg's primary sweep kernel against the
 FP32 vector peak with 24*S*P^2 FLOPs per unit; `cpu_baseline` times the FP64 oracle on a band
@@ -61,6 +65,7 @@ def cpu_baseline(problem, settings, rows: int, threads: int, gpu_samples=None):
     sw = O.params(settings)
     ny = len(range(sw.patch_radius + 2, problem.frames[problem.reference].shape[0]
                    - (sw.patch_radius + 2), sw.grid_stride))
+    rows = min(rows, ny) if rows > 0 else ny
     first = max(0, ny // 2 - rows // 2)
     O.set_row_band(first, rows)
     try:
@@ -227,6 +232,47 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
     }
 
 
+def serving_leg(V, scenes, n_requests: int, warmup: int, log):
+    """The serving stream: Poisson arrivals at 200/s from 16 clips (long edge 360-640,
+    4-12 sources, 64-192 hypotheses, 5x5 / 7x7 / 9x9 ZNCC, stride 2, FP16 storage), admitted by
+    continuous batching under a 64 GB HBM budget; then the same requests all at once."""
+    from vt3d_dense import serving
+
+    t0 = time.perf_counter()
+    spec, arrivals = scenes.c4_stream(n_requests + warmup)
+    clips = scenes.c4_clips(spec)
+    gen_s = time.perf_counter() - t0
+    frames, base = [], []
+    for c in clips:
+        base.append(len(frames))
+        frames += c["frames"]
+    sess = V.Session(0)
+    sess.upload(frames, "f16")
+    requests = []
+    for a in arrivals:
+        rq = scenes.c4_request(a, clips, base)
+        rq["sweep"] = V.params(rq["settings"])
+        requests.append(rq)
+    server = V.Server(sess, max_batch=64, max_inflight=2, hbm_budget_bytes=64 << 30)
+    try:
+        opened = serving.open_loop(server, requests, [a["arrival"] for a in arrivals],
+                                   warmup=warmup)
+        log(f"serving open loop: p50 {opened['latency_ms']['p50']:.2f} ms "
+            f"p99 {opened['latency_ms']['p99']:.2f} ms, {opened['units_per_s']:.4g} units/s")
+        sat = serving.saturate(server, requests)
+        log(f"serving saturated: {sat['units_per_s']:.4g} units/s")
+        st = server.stats()
+    finally:
+        server.close()
+        sess.close()
+    return {"open_loop": opened, "saturated": sat, "server": st, "clips": len(clips),
+            "frames_resident": len(frames), "scene_gen_s": gen_s,
+            "workload": f"{n_requests} requests after {warmup} warm-up, Poisson 200/s from "
+                        "16 clips (long edge 360-640, 4-12 sources, 64-192 hyps, 5x5/7x7/9x9 "
+                        "ZNCC, stride 2, FP16), 64 GB HBM budget, up to 64 requests per "
+                        "launch, 2 launches in flight"}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -236,8 +282,11 @@ def main() -> int:
                     help="solves in the batched leg (the batched workload is 120)")
     ap.add_argument("--plain-storage", default="f32", choices=["f32", "f16", "bf16"])
     ap.add_argument("--no-tune", action="store_true", help="default launch shape, no autotuner")
+    ap.add_argument("--serving-requests", type=int, default=2000,
+                    help="measured requests of the serving stream (0: skip the serving leg)")
+    ap.add_argument("--serving-warmup", type=int, default=200)
     ap.add_argument("--cpu-rows", type=int, default=0,
-                    help="grid rows in the CPU-baseline sample (0: twice the host threads)")
+                    help="grid rows in the CPU-baseline sample (0: the whole keyframe)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration for rocprof)")
@@ -261,6 +310,9 @@ def main() -> int:
     log(f"plain leg: {plain['value']:.4g} units/s")
     bat = batched_leg(V, scenes, args.steps, args.warmup, args.keyframes, not args.no_tune, log)
     log(f"batched leg: {bat['value']:.4g} units/s")
+    serv = None
+    if args.serving_requests > 0:
+        serv = serving_leg(V, scenes, args.serving_requests, args.serving_warmup, log)
     S, P = bat.pop("S"), bat.pop("P")
     problem0 = bat.pop("problem0")
     flops_per_unit = 24 * S * P * P
@@ -294,7 +346,11 @@ def main() -> int:
         "plain": plain,
         "batched_vs_plain": bat["value"] / plain["value"],
         "batched_vs_plain_device": bat["device_value"] / plain["device_value"],
-        "latency_ms": plain["latency_ms"],
+        "latency_ms": (serv["open_loop"]["latency_ms"] if serv else plain["latency_ms"]),
+        "latency_source": ("serving open loop, submit -> result on host" if serv
+                           else "plain solve wall time"),
+        "serving": serv,
+        "served_vs_plain": (serv["saturated"]["units_per_s"] / plain["value"] if serv else None),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": _traffic("k_sweep_tile_c3"),
@@ -303,7 +359,7 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         threads = _host_threads()
-        rows = args.cpu_rows if args.cpu_rows > 0 else 2 * threads
+        rows = args.cpu_rows  # 0: every grid row of the keyframe
         settings = dict(problem0.settings, surface_completion_passes=0, storage="f32")
         sess = V.Session(0)
         sess.upload(problem0.frames, "f16")

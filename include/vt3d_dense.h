@@ -327,6 +327,59 @@ int32_t vt3d_batch_run(vt3d_batch *batch, vt3d_dense_result **results);
 int32_t vt3d_batch_last_timing(const vt3d_batch *batch, vt3d_timing *timing);
 void vt3d_batch_destroy(vt3d_batch *batch);
 
+/* ---- serving: continuous batching on a session ----------------------------------------- */
+/* No counterpart in the reference. A server owns a worker thread that takes submitted
+ * requests off a queue and launches them in batches on its own HIP streams (one stream and
+ * one set of device buffers per in-flight batch). While batches run, newly arrived requests
+ * are prepared and launched as soon as an in-flight slot and HBM budget free up, so their
+ * work overlaps on the device. Frames stay resident in the session; each request's results
+ * are identical to the same request solved alone. Do not upload frames to, tune or solve on
+ * the session while a server on it runs. */
+
+typedef struct vt3d_server vt3d_server;
+
+typedef struct vt3d_server_config {
+    uint32_t struct_size;          /* sizeof(vt3d_server_config) */
+    int32_t max_batch_requests;    /* requests per launch (0: 64) */
+    int32_t max_inflight_batches;  /* batches on the device at once, 1..8 (0: 2) */
+    int32_t pad;
+    uint64_t hbm_budget_bytes;     /* frames + every in-flight batch's buffers (0: 64 GiB) */
+} vt3d_server_config;
+
+typedef struct vt3d_completion {
+    uint64_t tag;                  /* the submit tag */
+    int32_t status;                /* VT3D_OK, VT3D_NO_PATCH or a negative error */
+    int32_t batch_requests;        /* requests in the launch that carried this one */
+    vt3d_dense_result *result;     /* owned by the caller (vt3d_dense_result_free); NULL unless OK */
+    double queue_ms;               /* submit -> launch of its batch */
+    double latency_ms;             /* submit -> result ready on the host */
+    double batch_device_ms;        /* device time of its batch (HIP events) */
+    uint64_t units;                /* sampled_pixels * depth_hypotheses */
+} vt3d_completion;
+
+typedef struct vt3d_server_stats {
+    uint64_t submitted, completed, batches, rejected_budget;
+    uint64_t max_batch_requests_seen;
+    uint64_t frame_bytes;          /* session frames resident in HBM */
+    uint64_t workspace_bytes;      /* device buffers of the in-flight slots (high water) */
+    uint64_t budget_bytes;
+} vt3d_server_stats;
+
+/* Starts the worker thread. VT3D_E_BUDGET if the session's frames alone exceed the budget. */
+int32_t vt3d_server_create(vt3d_session *session, const vt3d_server_config *config,
+                           vt3d_server **out);
+/* Queues a request (inputs are copied). Validation errors are returned here; a request
+ * whose working set alone exceeds one slot's share of the budget completes with
+ * VT3D_E_BUDGET. */
+int32_t vt3d_server_submit(vt3d_server *server, const vt3d_request *request, uint64_t tag);
+/* Moves up to `capacity` completions into out[], waiting up to timeout_ms (< 0: forever)
+ * for the first one; *n_out receives the count. */
+int32_t vt3d_server_poll(vt3d_server *server, vt3d_completion *out, size_t capacity,
+                         int32_t timeout_ms, size_t *n_out);
+int32_t vt3d_server_get_stats(vt3d_server *server, vt3d_server_stats *stats);
+/* Finishes queued and in-flight requests, stops the worker and frees unpolled results. */
+void vt3d_server_destroy(vt3d_server *server);
+
 /* Last device or validation error message of the calling thread ("" if none). */
 const char *vt3d_last_error(void);
 

@@ -7,12 +7,16 @@
 // stream. Everything per grid sample runs on the GPU; there is no CPU path for it. A single
 // solve is a batch of one.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
+#include <condition_variable>
+#include <deque>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <tuple>
@@ -117,7 +121,7 @@ struct DevView {
     bool valid = false;
 };
 
-// Device buffers of one batch pipeline.
+// Device buffers and stage events of one batch pipeline.
 struct Workspace {
     DevBuf prob, src, Hp, Hr, depth_p, depth_r, block_problem;
     DevBuf site_flag, samples, site_map, partial, decision, recip, out;
@@ -125,6 +129,29 @@ struct Workspace {
     DevBuf points, sites, attrs, errors, supports, fobs, diag;
     HostBuf meta_host;
     HostBuf out_host;  // staging of the packed outputs
+    hipEvent_t ev[8] = {};
+    Workspace() = default;
+    Workspace(const Workspace &) = delete;
+    Workspace &operator=(const Workspace &) = delete;
+    int ensure_events() {
+        for (hipEvent_t &e : ev)
+            if (!e) VT3D_HIP(hipEventCreate(&e));
+        return VT3D_OK;
+    }
+    // Device bytes held by the buffers.
+    size_t device_bytes() const {
+        const DevBuf *all[] = {&prob, &src, &Hp, &Hr, &depth_p, &depth_r, &block_problem,
+                               &site_flag, &samples, &site_map, &partial, &decision, &recip,
+                               &out, &block_counts, &block_offsets, &meta, &points, &sites,
+                               &attrs, &errors, &supports, &fobs, &diag};
+        size_t t = 0;
+        for (const DevBuf *b : all) t += b->cap;
+        return t;
+    }
+    ~Workspace() {
+        for (hipEvent_t &e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
 };
 
 }  // namespace
@@ -136,7 +163,6 @@ struct vt3d_session {
     int storage = VT3D_STORAGE_F32;
     std::vector<std::unique_ptr<vt3d::DevView>> views;  // uploaded frames by frame index
     vt3d::Workspace ws;                                  // one-shot and session solves
-    hipEvent_t ev[8] = {};
     vt3d_timing timing{};
     vt3d_tuning tuning{32, 4096};
 };
@@ -804,6 +830,7 @@ struct vt3d_batch {
     vt3d::Plan plan;
     vt3d::Workspace own;           // the batch's buffers (one-shot solves use the session's)
     vt3d::Workspace *ws = nullptr;
+    hipStream_t stream = nullptr;  // the session stream, or a serving slot's stream
     bool uploaded = false;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -823,10 +850,9 @@ namespace {
 
 // Uploads the plan's tables into the workspace and fills the launch arguments.
 int upload_plan(vt3d_batch &b) {
-    vt3d_session *s = b.session;
     Workspace &ws = *b.ws;
     Plan &pl = b.plan;
-    hipStream_t st = s->stream;
+    hipStream_t st = b.stream;
     const size_t n = pl.prob.size();
     int rc;
     if ((rc = ws.prob.ensure(sizeof(ProblemDev) * n)) ||
@@ -908,7 +934,7 @@ int upload_plan(vt3d_batch &b) {
 // The batch's launches on the session stream, with events between the stages (ev may be
 // null, as under graph capture).
 int enqueue(vt3d_batch &b, hipEvent_t *ev) {
-    hipStream_t st = b.session->stream;
+    hipStream_t st = b.stream;
     const Plan &pl = b.plan;
     const Args a = b.args;
     auto mark = [&](int k) -> int {
@@ -956,19 +982,16 @@ int enqueue(vt3d_batch &b, hipEvent_t *ev) {
     return mark(5);
 }
 
-// Runs the batch and builds one result per request.
-int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
-    vt3d_session *s = b.session;
-    hipStream_t st = s->stream;
-    VT3D_HIP(hipSetDevice(s->device));
-    for (int k = 0; k < 8; ++k)
-        if (!s->ev[k]) VT3D_HIP(hipEventCreate(&s->ev[k]));
-    const size_t nreq = b.prepared.size();
-    for (size_t i = 0; i < nreq; ++i) results[i] = nullptr;
+// Enqueues the batch on its stream: table upload on first use, then the kernel pipeline
+// (or its captured graph) and the copy of the per-problem totals to pinned memory.
+int launch_batch(vt3d_batch &b) {
+    hipStream_t st = b.stream;
+    Workspace &ws = *b.ws;
     int rc;
+    if ((rc = ws.ensure_events())) return rc;
     const size_t n = b.plan.prob.size();
     if (n > 0 && !b.uploaded && (rc = upload_plan(b))) return rc;
-    VT3D_HIP(hipEventRecord(s->ev[0], st));
+    VT3D_HIP(hipEventRecord(ws.ev[0], st));
     if (n > 0) {
         if (b.use_graph) {
             if (!b.exec) {
@@ -985,13 +1008,25 @@ int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
                 VT3D_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
             }
             VT3D_HIP(hipGraphLaunch(b.exec, st));
-            for (int k = 1; k <= 5; ++k) VT3D_HIP(hipEventRecord(s->ev[k], st));
-        } else if ((rc = enqueue(b, s->ev))) {
+            for (int k = 1; k <= 5; ++k) VT3D_HIP(hipEventRecord(ws.ev[k], st));
+        } else if ((rc = enqueue(b, ws.ev))) {
             return rc;
         }
     } else {
-        for (int k = 1; k <= 5; ++k) VT3D_HIP(hipEventRecord(s->ev[k], st));
+        for (int k = 1; k <= 5; ++k) VT3D_HIP(hipEventRecord(ws.ev[k], st));
     }
+    return VT3D_OK;
+}
+
+// Waits for a launched batch, downloads its packed outputs and builds one result per
+// request (results[i] stays NULL where the reference remap returns None).
+int finish_batch(vt3d_batch &b, vt3d_dense_result **results) {
+    hipStream_t st = b.stream;
+    Workspace &ws = *b.ws;
+    const size_t nreq = b.prepared.size();
+    for (size_t i = 0; i < nreq; ++i) results[i] = nullptr;
+    int rc;
+    const size_t n = b.plan.prob.size();
     VT3D_HIP(hipStreamSynchronize(st));
 
     // Packed outputs: one copy per array into pinned staging memory.
@@ -1023,7 +1058,7 @@ int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
         (rc = d2h(off_fobs, b.outs.fused_obs, 4ull * n_acc)) ||
         (rc = d2h(off_diag, b.outs.diag, sizeof(vt3d_sample_diag) * (size_t)n_diag)))
         return rc;
-    VT3D_HIP(hipEventRecord(s->ev[6], st));
+    VT3D_HIP(hipEventRecord(ws.ev[6], st));
     VT3D_HIP(hipStreamSynchronize(st));
     const vt3d_point3 *points = reinterpret_cast<const vt3d_point3 *>(hb);
     const vt3d_grid_site *sites = reinterpret_cast<const vt3d_grid_site *>(hb + off_sites);
@@ -1036,12 +1071,12 @@ int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
 
     vt3d_timing &tm = b.timing;
     tm = vt3d_timing{};
-    tm.gate_ms = elapsed(s->ev[1], s->ev[2]);
-    tm.sweep_ms = elapsed(s->ev[2], s->ev[3]);
-    tm.reciprocal_ms = elapsed(s->ev[3], s->ev[4]);
-    tm.fuse_ms = elapsed(s->ev[4], s->ev[5]);
-    tm.download_ms = elapsed(s->ev[5], s->ev[6]);
-    tm.total_ms = elapsed(s->ev[0], s->ev[6]);
+    tm.gate_ms = elapsed(ws.ev[1], ws.ev[2]);
+    tm.sweep_ms = elapsed(ws.ev[2], ws.ev[3]);
+    tm.reciprocal_ms = elapsed(ws.ev[3], ws.ev[4]);
+    tm.fuse_ms = elapsed(ws.ev[4], ws.ev[5]);
+    tm.download_ms = elapsed(ws.ev[5], ws.ev[6]);
+    tm.total_ms = elapsed(ws.ev[0], ws.ev[6]);
 
     // Medians (median_option, dense_legacy.rs:1176-1182): the sorted value at index
     // round((n - 1) / 2), i.e. n / 2. The staging copies are scratch, so selection runs in
@@ -1137,6 +1172,15 @@ int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
     return VT3D_OK;
 }
 
+// Launches the batch and builds its results.
+int run_batch(vt3d_batch &b, vt3d_dense_result **results) {
+    VT3D_HIP(hipSetDevice(b.session->device));
+    for (size_t i = 0; i < b.prepared.size(); ++i) results[i] = nullptr;
+    const int rc = launch_batch(b);
+    if (rc) return rc;
+    return finish_batch(b, results);
+}
+
 vt3d_session *thread_session(int *rc) {
     // One session per (thread, device) for the one-shot entry points. They live until the
     // process ends: freeing device memory from thread-exit destructors can run after the HIP
@@ -1178,6 +1222,7 @@ int solve_one(vt3d_session *s, FrameSource &fs, const vt3d_camera *cameras, size
     vt3d_batch b;
     b.session = s;
     b.ws = &s->ws;
+    b.stream = s->stream;
     b.prepared.emplace_back(new Prepared());
     Prepared &pr = *b.prepared.back();
     Control ctl{poll, poll_user};
@@ -1217,6 +1262,285 @@ int check_out(vt3d_dense_result **out, const vt3d_camera *cameras, size_t n_came
         return VT3D_E_INVALID;
     }
     return VT3D_OK;
+}
+
+
+// Validates a request against the session's frames and resolves its parameters.
+int check_request(vt3d_session *s, const vt3d_request &q, vt3d_sweep_params *p) {
+    int rc;
+    if ((rc = resolve_params(q.params, p))) return rc;
+    if ((q.n_cameras && !q.cameras) || (q.n_sparse && !q.sparse_xyz) ||
+        (q.n_frames && !q.frame_map)) {
+        set_error("null request array with nonzero length");
+        return VT3D_E_INVALID;
+    }
+    for (size_t k = 0; k < q.n_frames; ++k)
+        if (q.frame_map[k] >= s->views.size() || !s->views[q.frame_map[k]]) {
+            set_error("request frame_map names a frame the session does not hold");
+            return VT3D_E_INVALID;
+        }
+    return VT3D_OK;
+}
+
+// Host selection of one checked request on the session's frames.
+int prepare_request(vt3d_session *s, const vt3d_request &q, const vt3d_sweep_params &p,
+                    Prepared &pr) {
+    FrameSource fs{nullptr, s, {}};
+    for (size_t k = 0; k < q.n_frames; ++k) fs.map.push_back((size_t)q.frame_map[k]);
+    Control ctl{nullptr, nullptr};
+    return q.reference_camera_index >= 0
+               ? prepare_reference_patch(s, fs, q.cameras, q.n_cameras, q.sparse_xyz, q.n_sparse,
+                                         q.focal, (size_t)q.reference_camera_index,
+                                         q.fine_grid != 0, p, ctl, pr)
+               : prepare_sweep(s, fs, q.cameras, q.n_cameras, q.sparse_xyz, q.n_sparse, q.focal,
+                               q.fine_grid != 0, p, ctl, pr);
+}
+
+// Upper bound of the device bytes one prepared request adds to a batch's buffers: its slots
+// in every per-site array, its hypothesis-block partials at the blocking it would get alone
+// (a batch never splits a request's hypotheses finer than that) and its tables.
+uint64_t request_bytes(const Prepared &pr, const vt3d_tuning &tuning) {
+    if (pr.state != 0) return 0;
+    const uint64_t blocks = std::max<uint64_t>(1, ((uint64_t)pr.nx * pr.ny + kBlock - 1) / kBlock);
+    const uint64_t cap = blocks * kBlock;
+    const uint64_t target = tuning.target_waves > 0 ? (uint64_t)tuning.target_waves : 4096;
+    const uint64_t n_hb = std::min<uint64_t>((uint64_t)pr.D, (target + 4 * blocks - 1) / (4 * blocks));
+    const uint64_t per_slot = 1 + 4 + 4 + sizeof(Decision) + sizeof(SampleOut) +
+                              sizeof(vt3d_point3) + sizeof(vt3d_grid_site) +
+                              sizeof(vt3d_point_attributes) + 12 +
+                              (pr.p.emit_samples ? sizeof(vt3d_sample_diag) : 0);
+    const uint64_t tables = sizeof(ProblemDev) + (uint64_t)pr.S * sizeof(SourceDev) +
+                            2ull * pr.S * pr.D * 9 * 8 + 8ull * pr.D * (1 + pr.S);
+    return cap * per_slot + cap * n_hb * sizeof(Top2) + cap * (uint64_t)pr.S * sizeof(Recip) +
+           blocks * kCounters * 8 + tables + 64 * 4;
+}
+
+// ---- serving ---------------------------------------------------------------------------------
+
+using Clock = std::chrono::steady_clock;
+
+double ms_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration<double, std::milli>(b - a).count();
+}
+
+// A submitted request with its inputs copied.
+struct ServerReq {
+    uint64_t tag = 0;
+    std::vector<vt3d_camera> cams;
+    std::vector<double> sparse;
+    std::vector<uint64_t> fmap;
+    vt3d_sweep_params p{};
+    vt3d_request q{};  // points into the vectors above
+    Clock::time_point arrival;
+    void bind() {
+        q.cameras = cams.data();
+        q.n_cameras = cams.size();
+        q.sparse_xyz = sparse.data();
+        q.n_sparse = sparse.size() / 3;
+        q.frame_map = fmap.data();
+        q.n_frames = fmap.size();
+        q.params = &p;
+    }
+};
+
+// One in-flight batch position: its stream, device buffers and the batch it carries.
+struct Slot {
+    hipStream_t stream = nullptr;
+    Workspace ws;
+    std::unique_ptr<vt3d_batch> batch;
+    std::vector<std::unique_ptr<ServerReq>> reqs;
+    Clock::time_point launched;
+    bool busy = false;
+    uint64_t seq = 0;  // launch order
+};
+
+}  // namespace
+}  // namespace vt3d
+
+struct vt3d_server {
+    vt3d_session *session = nullptr;
+    int max_batch = 64;
+    uint64_t budget = 0, frame_bytes = 0, slot_budget = 0;
+    std::vector<std::unique_ptr<vt3d::Slot>> slots;
+    std::mutex mu;
+    std::condition_variable cv_submit, cv_done;
+    std::deque<std::unique_ptr<vt3d::ServerReq>> pending;
+    std::deque<vt3d_completion> done;
+    std::thread worker;
+    bool stop = false;
+    uint64_t launches = 0;
+    vt3d_server_stats stats{};
+};
+
+namespace vt3d {
+namespace {
+
+void complete(vt3d_server *sv, std::vector<vt3d_completion> &out) {
+    std::lock_guard<std::mutex> lk(sv->mu);
+    for (vt3d_completion &c : out) {
+        sv->done.push_back(c);
+        ++sv->stats.completed;
+    }
+    out.clear();
+    sv->cv_done.notify_all();
+}
+
+vt3d_completion failed(const ServerReq &r, int status) {
+    vt3d_completion c{};
+    c.tag = r.tag;
+    c.status = status;
+    c.latency_ms = ms_between(r.arrival, Clock::now());
+    return c;
+}
+
+// Downloads a finished slot's batch and turns it into completions.
+void retire(vt3d_server *sv, Slot &sl) {
+    vt3d_batch &b = *sl.batch;
+    const size_t n = sl.reqs.size();
+    std::vector<vt3d_dense_result *> res(n, nullptr);
+    std::vector<vt3d_completion> out;
+    const int rc = finish_batch(b, res.data());
+    const Clock::time_point now = Clock::now();
+    for (size_t i = 0; i < n; ++i) {
+        const ServerReq &r = *sl.reqs[i];
+        vt3d_completion c{};
+        c.tag = r.tag;
+        c.batch_requests = (int32_t)n;
+        c.queue_ms = ms_between(r.arrival, sl.launched);
+        c.latency_ms = ms_between(r.arrival, now);
+        c.batch_device_ms = b.timing.total_ms;
+        if (rc) {
+            vt3d_dense_result_free(res[i]);
+            c.status = rc;
+        } else if (!res[i]) {
+            c.status = VT3D_NO_PATCH;
+        } else {
+            c.status = VT3D_OK;
+            c.result = res[i];
+            const Prepared &pr = *b.prepared[i];
+            if (pr.state == 0) c.units = res[i]->stats.sampled_pixels * (uint64_t)pr.D;
+        }
+        out.push_back(c);
+    }
+    sl.batch.reset();
+    sl.reqs.clear();
+    complete(sv, out);
+}
+
+// Prepares up to max_batch queued requests that fit the slot's budget and launches them on
+// the slot. Requests that do not fit go back to the front of the queue in order.
+void admit(vt3d_server *sv, Slot &sl, std::vector<std::unique_ptr<ServerReq>> taken) {
+    vt3d_session *s = sv->session;
+    std::unique_ptr<vt3d_batch> b(new vt3d_batch());
+    b->session = s;
+    b->ws = &sl.ws;
+    b->stream = sl.stream;
+    std::vector<std::unique_ptr<ServerReq>> kept, back;
+    std::vector<vt3d_completion> out;
+    uint64_t bytes = 0;
+    for (size_t i = 0; i < taken.size(); ++i) {
+        std::unique_ptr<ServerReq> &r = taken[i];
+        std::unique_ptr<Prepared> pr(new Prepared());
+        const int rc = prepare_request(s, r->q, r->p, *pr);
+        if (rc) {
+            out.push_back(failed(*r, rc));
+            continue;
+        }
+        const uint64_t need = request_bytes(*pr, s->tuning);
+        if (need > sv->slot_budget) {
+            out.push_back(failed(*r, VT3D_E_BUDGET));
+            std::lock_guard<std::mutex> lk(sv->mu);
+            ++sv->stats.rejected_budget;
+            continue;
+        }
+        if (bytes + need > sv->slot_budget) {  // the rest waits for a later batch
+            for (size_t k = i; k < taken.size(); ++k) back.push_back(std::move(taken[k]));
+            break;
+        }
+        bytes += need;
+        b->prepared.push_back(std::move(pr));
+        kept.push_back(std::move(r));
+    }
+    if (!back.empty()) {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        for (size_t k = back.size(); k-- > 0;) sv->pending.push_front(std::move(back[k]));
+    }
+    if (!out.empty()) complete(sv, out);
+    if (kept.empty()) return;
+    std::vector<Prepared *> reqp;
+    for (auto &u : b->prepared) reqp.push_back(u.get());
+    int rc = build_plan(reqp, s->tuning, b->plan);
+    sl.launched = Clock::now();
+    if (!rc) rc = launch_batch(*b);
+    if (rc) {
+        for (auto &r : kept) out.push_back(failed(*r, rc));
+        complete(sv, out);
+        return;
+    }
+    sl.batch = std::move(b);
+    sl.reqs = std::move(kept);
+    std::lock_guard<std::mutex> lk(sv->mu);
+    sl.busy = true;
+    sl.seq = ++sv->launches;
+    ++sv->stats.batches;
+    sv->stats.max_batch_requests_seen =
+        std::max<uint64_t>(sv->stats.max_batch_requests_seen, sl.reqs.size());
+    uint64_t ws_bytes = 0;
+    for (auto &x : sv->slots) ws_bytes += x->ws.device_bytes();
+    sv->stats.workspace_bytes = std::max<uint64_t>(sv->stats.workspace_bytes, ws_bytes);
+}
+
+void serve(vt3d_server *sv) {
+    (void)hipSetDevice(sv->session->device);
+    std::unique_lock<std::mutex> lk(sv->mu);
+    for (;;) {
+        bool progressed = false;
+        // Retire finished batches, oldest first.
+        for (;;) {
+            Slot *oldest = nullptr;
+            for (auto &x : sv->slots)
+                if (x->busy && (!oldest || x->seq < oldest->seq)) oldest = x.get();
+            if (!oldest || hipEventQuery(oldest->ws.ev[5]) != hipSuccess) break;
+            lk.unlock();
+            retire(sv, *oldest);
+            lk.lock();
+            oldest->busy = false;
+            progressed = true;
+        }
+        bool any_busy = false;
+        Slot *free_slot = nullptr;
+        for (auto &x : sv->slots) {
+            any_busy = any_busy || x->busy;
+            if (!x->busy && !free_slot) free_slot = x.get();
+        }
+        if (sv->stop && sv->pending.empty() && !any_busy) break;
+        if (free_slot && !sv->pending.empty()) {
+            std::vector<std::unique_ptr<ServerReq>> taken;
+            while (!sv->pending.empty() && (int)taken.size() < sv->max_batch) {
+                taken.push_back(std::move(sv->pending.front()));
+                sv->pending.pop_front();
+            }
+            lk.unlock();
+            admit(sv, *free_slot, std::move(taken));
+            lk.lock();
+            continue;
+        }
+        if (progressed) continue;
+        if (!any_busy) {
+            sv->cv_submit.wait(lk, [&] { return sv->stop || !sv->pending.empty(); });
+        } else if (free_slot) {
+            // A batch runs and a slot is free: wake for a new request or poll the batch.
+            sv->cv_submit.wait_for(lk, std::chrono::microseconds(200));
+        } else {
+            // Every slot is busy: wait for the oldest batch.
+            Slot *oldest = nullptr;
+            for (auto &x : sv->slots)
+                if (x->busy && (!oldest || x->seq < oldest->seq)) oldest = x.get();
+            lk.unlock();
+            (void)hipEventSynchronize(oldest->ws.ev[5]);
+            lk.lock();
+        }
+    }
 }
 
 }  // namespace
@@ -1348,10 +1672,9 @@ void vt3d_session_destroy(vt3d_session *s) {
     if (!s) return;
     (void)hipSetDevice(s->device);
     if (s->stream) (void)hipStreamSynchronize(s->stream);
-    for (hipEvent_t &e : s->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (s->stream) (void)hipStreamDestroy(s->stream);
+    hipStream_t st = s->stream;
     delete s;
+    if (st) (void)hipStreamDestroy(st);
 }
 
 int32_t vt3d_session_upload_frames(vt3d_session *s, const vt3d_frame *frames, size_t n,
@@ -1412,34 +1735,14 @@ int32_t vt3d_batch_create(vt3d_session *s, const vt3d_request *reqs, size_t n,
     std::unique_ptr<vt3d_batch> b(new vt3d_batch());
     b->session = s;
     b->ws = &b->own;
+    b->stream = s->stream;
     b->use_graph = use_graph != 0;
     for (size_t i = 0; i < n; ++i) {
-        const vt3d_request &q = reqs[i];
         vt3d_sweep_params p;
         int rc;
-        if ((rc = resolve_params(q.params, &p))) return rc;
-        if ((q.n_cameras && !q.cameras) || (q.n_sparse && !q.sparse_xyz) ||
-            (q.n_frames && !q.frame_map)) {
-            set_error("null request array with nonzero length");
-            return VT3D_E_INVALID;
-        }
-        FrameSource fs{nullptr, s, {}};
-        for (size_t k = 0; k < q.n_frames; ++k) {
-            if (q.frame_map[k] >= s->views.size() || !s->views[q.frame_map[k]]) {
-                set_error("request frame_map names a frame the session does not hold");
-                return VT3D_E_INVALID;
-            }
-            fs.map.push_back((size_t)q.frame_map[k]);
-        }
+        if ((rc = check_request(s, reqs[i], &p))) return rc;
         b->prepared.emplace_back(new Prepared());
-        Control ctl{nullptr, nullptr};
-        rc = q.reference_camera_index >= 0
-                 ? prepare_reference_patch(s, fs, q.cameras, q.n_cameras, q.sparse_xyz,
-                                           q.n_sparse, q.focal, (size_t)q.reference_camera_index,
-                                           q.fine_grid != 0, p, ctl, *b->prepared.back())
-                 : prepare_sweep(s, fs, q.cameras, q.n_cameras, q.sparse_xyz, q.n_sparse, q.focal,
-                                 q.fine_grid != 0, p, ctl, *b->prepared.back());
-        if (rc) return rc;
+        if ((rc = prepare_request(s, reqs[i], p, *b->prepared.back()))) return rc;
     }
     std::vector<Prepared *> reqp;
     for (auto &u : b->prepared) reqp.push_back(u.get());
@@ -1469,8 +1772,116 @@ int32_t vt3d_batch_last_timing(const vt3d_batch *b, vt3d_timing *t) {
 
 void vt3d_batch_destroy(vt3d_batch *b) {
     if (!b) return;
-    if (b->session && b->session->stream) (void)hipStreamSynchronize(b->session->stream);
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
     delete b;
+}
+
+int32_t vt3d_server_create(vt3d_session *s, const vt3d_server_config *cfg, vt3d_server **out) {
+    if (!s || !out) return VT3D_E_INVALID;
+    *out = nullptr;
+    vt3d_server_config c{};
+    c.struct_size = sizeof(c);
+    if (cfg) {
+        if (cfg->struct_size != sizeof(vt3d_server_config)) {
+            set_error("vt3d_server_config.struct_size mismatch");
+            return VT3D_E_INVALID;
+        }
+        c = *cfg;
+    }
+    if (c.max_batch_requests < 0 || c.max_inflight_batches < 0 || c.max_inflight_batches > 8) {
+        set_error("invalid vt3d_server_config");
+        return VT3D_E_INVALID;
+    }
+    VT3D_HIP(hipSetDevice(s->device));
+    std::unique_ptr<vt3d_server> sv(new vt3d_server());
+    sv->session = s;
+    sv->max_batch = c.max_batch_requests ? c.max_batch_requests : 64;
+    const int inflight = c.max_inflight_batches ? c.max_inflight_batches : 2;
+    sv->budget = c.hbm_budget_bytes ? c.hbm_budget_bytes : (64ull << 30);
+    for (const auto &v : s->views)
+        if (v) sv->frame_bytes += v->rgba.cap + v->luma.cap + v->quad.cap;
+    if (sv->frame_bytes >= sv->budget) {
+        set_error("session frames alone exceed the HBM budget");
+        return VT3D_E_BUDGET;
+    }
+    // Each in-flight slot may hold up to its share of what the frames leave; buffers only
+    // grow to a batch admitted under that share, so frames + slots stay within the budget.
+    sv->slot_budget = (sv->budget - sv->frame_bytes) / (uint64_t)inflight;
+    for (int k = 0; k < inflight; ++k) {
+        sv->slots.emplace_back(new Slot());
+        VT3D_HIP(hipStreamCreateWithFlags(&sv->slots.back()->stream, hipStreamNonBlocking));
+        int rc = sv->slots.back()->ws.ensure_events();
+        if (rc) return rc;
+    }
+    sv->stats.frame_bytes = sv->frame_bytes;
+    sv->stats.budget_bytes = sv->budget;
+    vt3d_server *raw = sv.get();
+    sv->worker = std::thread([raw] { serve(raw); });
+    *out = sv.release();
+    return VT3D_OK;
+}
+
+int32_t vt3d_server_submit(vt3d_server *sv, const vt3d_request *q, uint64_t tag) {
+    if (!sv || !q) return VT3D_E_INVALID;
+    std::unique_ptr<ServerReq> r(new ServerReq());
+    int rc = check_request(sv->session, *q, &r->p);
+    if (rc) return rc;
+    r->tag = tag;
+    r->cams.assign(q->cameras, q->cameras + q->n_cameras);
+    r->sparse.assign(q->sparse_xyz, q->sparse_xyz + 3 * q->n_sparse);
+    r->fmap.assign(q->frame_map, q->frame_map + q->n_frames);
+    r->q = *q;
+    r->bind();
+    r->arrival = Clock::now();
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        if (sv->stop) return VT3D_E_INVALID;
+        sv->pending.push_back(std::move(r));
+        ++sv->stats.submitted;
+    }
+    sv->cv_submit.notify_one();
+    return VT3D_OK;
+}
+
+int32_t vt3d_server_poll(vt3d_server *sv, vt3d_completion *out, size_t capacity,
+                         int32_t timeout_ms, size_t *n_out) {
+    if (!sv || !n_out || (capacity && !out)) return VT3D_E_INVALID;
+    *n_out = 0;
+    std::unique_lock<std::mutex> lk(sv->mu);
+    auto ready = [&] { return !sv->done.empty(); };
+    if (timeout_ms < 0) sv->cv_done.wait(lk, ready);
+    else if (timeout_ms > 0) sv->cv_done.wait_for(lk, std::chrono::milliseconds(timeout_ms), ready);
+    size_t n = 0;
+    while (n < capacity && !sv->done.empty()) {
+        out[n++] = sv->done.front();
+        sv->done.pop_front();
+    }
+    *n_out = n;
+    return VT3D_OK;
+}
+
+int32_t vt3d_server_get_stats(vt3d_server *sv, vt3d_server_stats *st) {
+    if (!sv || !st) return VT3D_E_INVALID;
+    std::lock_guard<std::mutex> lk(sv->mu);
+    *st = sv->stats;
+    return VT3D_OK;
+}
+
+void vt3d_server_destroy(vt3d_server *sv) {
+    if (!sv) return;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->stop = true;
+    }
+    sv->cv_submit.notify_all();
+    if (sv->worker.joinable()) sv->worker.join();
+    for (vt3d_completion &c : sv->done) vt3d_dense_result_free(c.result);
+    for (auto &sl : sv->slots)
+        if (sl->stream) {
+            (void)hipStreamSynchronize(sl->stream);
+            (void)hipStreamDestroy(sl->stream);
+        }
+    delete sv;
 }
 
 }  // extern "C"

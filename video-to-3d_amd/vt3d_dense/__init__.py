@@ -97,6 +97,17 @@ def _bind(lib: C.CDLL) -> None:
     lib.vt3d_batch_last_timing.restype = C.c_int32
     lib.vt3d_batch_destroy.argtypes = [C.c_void_p]
     lib.vt3d_batch_destroy.restype = None
+    lib.vt3d_server_create.argtypes = [C.c_void_p, P(abi.ServerConfig), P(C.c_void_p)]
+    lib.vt3d_server_create.restype = C.c_int32
+    lib.vt3d_server_submit.argtypes = [C.c_void_p, P(abi.Request), C.c_uint64]
+    lib.vt3d_server_submit.restype = C.c_int32
+    lib.vt3d_server_poll.argtypes = [C.c_void_p, P(abi.Completion), C.c_size_t, C.c_int32,
+                                     P(C.c_size_t)]
+    lib.vt3d_server_poll.restype = C.c_int32
+    lib.vt3d_server_get_stats.argtypes = [C.c_void_p, P(abi.ServerStats)]
+    lib.vt3d_server_get_stats.restype = C.c_int32
+    lib.vt3d_server_destroy.argtypes = [C.c_void_p]
+    lib.vt3d_server_destroy.restype = None
 
 
 EXPORTED_SYMBOLS = (
@@ -106,7 +117,8 @@ EXPORTED_SYMBOLS = (
     "vt3d_session_destroy", "vt3d_session_upload_frames", "vt3d_session_solve",
     "vt3d_session_last_timing", "vt3d_last_error", "vt3d_plane_homography",
     "vt3d_session_set_tuning", "vt3d_batch_create", "vt3d_batch_run",
-    "vt3d_batch_last_timing", "vt3d_batch_destroy",
+    "vt3d_batch_last_timing", "vt3d_batch_destroy", "vt3d_server_create",
+    "vt3d_server_submit", "vt3d_server_poll", "vt3d_server_get_stats", "vt3d_server_destroy",
 )
 
 
@@ -253,6 +265,26 @@ class Session:
             pass
 
 
+def _fill_request(r: abi.Request, rq: dict, n_frames: int):
+    """Fills a vt3d_request from a request dict; returns the objects that must outlive it."""
+    inp = abi.Inputs([], rq["cameras"], rq["sparse_points"])
+    fmap = rq.get("frame_map")
+    fmap = np.arange(n_frames, dtype=np.uint64) if fmap is None else \
+        np.ascontiguousarray(np.asarray(fmap, dtype=np.uint64))
+    sweep = rq.get("sweep")
+    r.cameras = C.cast(inp.cameras, C.POINTER(abi.Camera))
+    r.n_cameras = inp.n_cameras
+    r.sparse_xyz = inp.sparse_ptr
+    r.n_sparse = inp.n_sparse
+    r.frame_map = fmap.ctypes.data_as(C.POINTER(C.c_uint64))
+    r.n_frames = fmap.shape[0]
+    r.focal = float(rq["focal"])
+    r.reference_camera_index = int(rq.get("reference_camera_index", -1))
+    r.fine_grid = int(bool(rq.get("fine_grid", False)))
+    r.params = C.pointer(sweep) if sweep is not None else C.POINTER(abi.SweepParams)()
+    return (inp, fmap, sweep)
+
+
 class Batch:
     """Many solves on one session's frames, run as one kernel pipeline (optionally replayed
     from a captured HIP graph). Build once, run many times."""
@@ -265,23 +297,7 @@ class Batch:
         self._reqs = (abi.Request * max(1, n))()
         n_frames = len(session._frames._rgba) if getattr(session, "_frames", None) else 0
         for i, rq in enumerate(requests):
-            inp = abi.Inputs([], rq["cameras"], rq["sparse_points"])
-            fmap = rq.get("frame_map")
-            fmap = np.arange(n_frames, dtype=np.uint64) if fmap is None else \
-                np.ascontiguousarray(np.asarray(fmap, dtype=np.uint64))
-            sweep = rq.get("sweep")
-            self._keep += [inp, fmap, sweep]
-            r = self._reqs[i]
-            r.cameras = C.cast(inp.cameras, C.POINTER(abi.Camera))
-            r.n_cameras = inp.n_cameras
-            r.sparse_xyz = inp.sparse_ptr
-            r.n_sparse = inp.n_sparse
-            r.frame_map = fmap.ctypes.data_as(C.POINTER(C.c_uint64))
-            r.n_frames = fmap.shape[0]
-            r.focal = float(rq["focal"])
-            r.reference_camera_index = int(rq.get("reference_camera_index", -1))
-            r.fine_grid = int(bool(rq.get("fine_grid", False)))
-            r.params = C.pointer(sweep) if sweep is not None else C.POINTER(abi.SweepParams)()
+            self._keep.append(_fill_request(self._reqs[i], rq, n_frames))
         self.n = n
         self._h = C.c_void_p()
         _check(lib.vt3d_batch_create(session._h, self._reqs, n, int(bool(use_graph)),
@@ -325,3 +341,55 @@ def pack_frames(frame_lists: Sequence[Sequence[np.ndarray]]):
             m.append(index[key])
         maps.append(np.asarray(m, dtype=np.uint64))
     return frames, maps
+
+
+class Server:
+    """Continuous-batching request loop on a session's resident frames (see
+    vt3d_server_create). submit() queues a request dict (as for Session.batch) under a tag;
+    poll() returns finished requests as dicts with tag, status, result (DenseAnalysis or None)
+    and latency / queue / batch device times in ms."""
+
+    def __init__(self, session: Session, max_batch: int = 0, max_inflight: int = 0,
+                 hbm_budget_bytes: int = 0):
+        lib = library()
+        self._session = session  # the session must outlive the server
+        cfg = abi.ServerConfig(C.sizeof(abi.ServerConfig), int(max_batch), int(max_inflight), 0,
+                               int(hbm_budget_bytes))
+        self._h = C.c_void_p()
+        self._n_frames = len(session._frames._rgba) if getattr(session, "_frames", None) else 0
+        _check(lib.vt3d_server_create(session._h, C.byref(cfg), C.byref(self._h)))
+
+    def submit(self, request: dict, tag: int) -> None:
+        r = abi.Request()
+        keep = _fill_request(r, request, self._n_frames)  # noqa: F841 (copied by the server)
+        _check(library().vt3d_server_submit(self._h, C.byref(r), int(tag)))
+
+    def poll(self, timeout_ms: int = 0, capacity: int = 256) -> list:
+        buf = (abi.Completion * capacity)()
+        n = C.c_size_t()
+        _check(library().vt3d_server_poll(self._h, buf, capacity, int(timeout_ms), C.byref(n)))
+        out = []
+        for i in range(n.value):
+            c = buf[i]
+            res = _take(c.result) if c.result else None
+            out.append({"tag": int(c.tag), "status": int(c.status), "result": res,
+                        "batch_requests": int(c.batch_requests), "queue_ms": c.queue_ms,
+                        "latency_ms": c.latency_ms, "batch_device_ms": c.batch_device_ms,
+                        "units": int(c.units)})
+        return out
+
+    def stats(self) -> dict:
+        st = abi.ServerStats()
+        _check(library().vt3d_server_get_stats(self._h, C.byref(st)))
+        return {name: int(getattr(st, name)) for name, _ in abi.ServerStats._fields_}
+
+    def close(self) -> None:
+        if self._h:
+            library().vt3d_server_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

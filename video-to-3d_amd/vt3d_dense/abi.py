@@ -18,6 +18,8 @@ VT3D_NO_PATCH = 1
 VT3D_E_INVALID = -1
 VT3D_E_DEVICE = -2
 VT3D_E_NO_DEVICE = -3
+VT3D_E_BUDGET = -4
+VT3D_E_BUSY = -5
 
 COST_ZSAD, COST_ZNCC = 0, 1
 AMBIGUITY_MARGIN, AMBIGUITY_RATIO = 0, 1
@@ -133,6 +135,26 @@ class Request(C.Structure):
                 ("focal", C.c_double), ("reference_camera_index", C.c_int64),
                 ("fine_grid", C.c_int32), ("pad", C.c_int32),
                 ("params", C.POINTER(SweepParams))]
+
+
+class ServerConfig(C.Structure):
+    _fields_ = [("struct_size", C.c_uint32), ("max_batch_requests", C.c_int32),
+                ("max_inflight_batches", C.c_int32), ("pad", C.c_int32),
+                ("hbm_budget_bytes", C.c_uint64)]
+
+
+class Completion(C.Structure):
+    _fields_ = [("tag", C.c_uint64), ("status", C.c_int32), ("batch_requests", C.c_int32),
+                ("result", C.POINTER(DenseResult)), ("queue_ms", C.c_double),
+                ("latency_ms", C.c_double), ("batch_device_ms", C.c_double),
+                ("units", C.c_uint64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("batches", C.c_uint64),
+                ("rejected_budget", C.c_uint64), ("max_batch_requests_seen", C.c_uint64),
+                ("frame_bytes", C.c_uint64), ("workspace_bytes", C.c_uint64),
+                ("budget_bytes", C.c_uint64)]
 
 
 POLL_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p)

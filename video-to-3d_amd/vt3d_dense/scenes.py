@@ -169,10 +169,10 @@ def render(planes: Sequence[Plane], cam: Camera, width: int, height: int, focal:
         denom = dw @ n
         with np.errstate(divide="ignore", invalid="ignore"):
             s = ((pl.origin - C) @ n) / denom
-        P = C + s[..., None] * dw
-        rel = P - pl.origin
-        a = rel @ pl.u
-        b = rel @ pl.v
+            P = C + s[..., None] * dw
+            rel = P - pl.origin
+            a = rel @ pl.u
+            b = rel @ pl.v
         ok = (np.isfinite(s) & (s > 1e-6) & (a >= pl.extent[0]) & (a <= pl.extent[1])
               & (b >= pl.extent[2]) & (b <= pl.extent[3]) & (s < best))
         if not ok.any():
@@ -193,10 +193,10 @@ def cast_points(planes: Sequence[Plane], cam: Camera, px: np.ndarray, py: np.nda
         n = pl.normal
         with np.errstate(divide="ignore", invalid="ignore"):
             s = ((pl.origin - C) @ n) / (dw @ n)
-        P = C + s[..., None] * dw
-        rel = P - pl.origin
-        a = rel @ pl.u
-        b = rel @ pl.v
+            P = C + s[..., None] * dw
+            rel = P - pl.origin
+            a = rel @ pl.u
+            b = rel @ pl.v
         ok = (np.isfinite(s) & (s > 1e-6) & (a >= pl.extent[0]) & (a <= pl.extent[1])
               & (b >= pl.extent[2]) & (b <= pl.extent[3]) & (s < best))
         best = np.where(ok, s, best)
@@ -455,6 +455,51 @@ def c4_stream(n_requests: int, seed: int = SEED_C4, rate: float = 200.0, clips: 
                      "patch_radius": int(rng.choice([2, 3, 4])),
                      "key": int(rng.integers(0, 1 << 30))})
     return clip_spec, reqs
+
+
+def c4_clips(clip_spec, n_frames: int = 16, threads: int = 0) -> List[dict]:
+    """Frames, cameras and 1024 landmarks of each serving clip: a handheld random walk of
+    `n_frames` views through a box room whose texture is seeded per clip."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    clips = []
+    jobs = []
+    for spec in clip_spec:
+        W, H = spec["size"]
+        seed = spec["seed"]
+        f = 0.86 * max(W, H)
+        cams = random_walk(n_frames, seed, sigma_t=0.02)
+        planes = room_planes(seed)
+        clips.append({"clip": spec["clip"], "size": (W, H), "focal": f, "cameras": cams,
+                      "sparse": landmarks(planes, cams, [(W, H)] * n_frames, f, 1024, seed)})
+        jobs += [(planes, c, W, H, f, seed) for c in cams]
+    if threads <= 0:
+        threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(threads) as ex:
+        imgs = list(ex.map(lambda j: render(*j)[0], jobs))
+    for k, c in enumerate(clips):
+        c["frames"] = imgs[k * n_frames:(k + 1) * n_frames]
+    return clips
+
+
+def c4_request(req: dict, clips: List[dict], frame_base: List[int]) -> dict:
+    """Request dict (for Session.batch / Server.submit) of one serving arrival: the
+    reference is the middle of a window of `sources + 1` consecutive clip frames chosen by
+    the request key; frame_map points into a session holding every clip's frames, clip c
+    starting at frame_base[c]."""
+    c = clips[req["clip"]]
+    S = req["sources"]
+    n = len(c["cameras"])
+    lo = req["key"] % (n - S)
+    idx = list(range(lo, lo + S + 1))
+    return {"cameras": [c["cameras"][i] for i in idx], "sparse_points": c["sparse"],
+            "focal": c["focal"], "reference_camera_index": S // 2,
+            "frame_map": np.array([frame_base[req["clip"]] + i for i in idx], np.uint64),
+            "settings": {"preset": "zncc", "patch_radius": req["patch_radius"],
+                         "depth_hypotheses": req["hypotheses"], "max_source_views": S,
+                         "grid_stride": 2, "depth_min": 0.5, "depth_max": 8.0,
+                         "storage": "f16", "surface_completion_passes": 0}}
 
 
 def to_rgba(gray: np.ndarray) -> np.ndarray:
