@@ -273,6 +273,98 @@ def serving_leg(V, scenes, n_requests: int, warmup: int, log):
                         "launch, 2 launches in flight"}
 
 
+def precision_leg(V, scenes, rows: int, threads: int, log):
+    """Large single solve (1920x1080 reference + 16 sources, 12 planes with 10% flat +-2
+    patches, 256 hypotheses, 11x11 ZNCC, stride 1, reciprocal + fusion) with FP32 and FP16
+    texel storage, and the error of the FP16 solve against the FP64 oracle on a band of
+    `rows` grid rows: chosen hypothesis (depth), photometric cost and fused points."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    import numpy as np
+
+    import vt3d_oracle as O
+
+    t0 = time.perf_counter()
+    p = scenes.config_c5()
+    gen_s = time.perf_counter() - t0
+    st = dict(p.settings, surface_completion_passes=0, emit_samples=1)
+    sess = V.Session(0)
+    res, out = {}, {"workload": "1920x1080 ref + 16 sources, 12 planes (10% flat +-2 patches), "
+                                "256 hyps 0.5-8 m, 11x11 ZNCC, stride 1, reciprocal + fusion",
+                    "scene_gen_s": gen_s}
+    try:
+        for storage in ("f32", "f16"):
+            sess.upload(p.frames, storage)
+            sw = V.params(st, storage=storage)
+            sess.solve(p.cameras, p.sparse, p.focal, p.reference, sweep=sw)
+            t1 = time.perf_counter()
+            r = sess.solve(p.cameras, p.sparse, p.focal, p.reference, sweep=sw)
+            wall = time.perf_counter() - t1
+            tm = sess.timing()
+            res[storage] = r
+            out[storage] = {"wall_ms": wall * 1e3, "device_ms": tm.total_ms,
+                            "sweep_ms": tm.sweep_ms, "reciprocal_ms": tm.reciprocal_ms,
+                            "units": int(tm.units), "units_per_s": tm.units / wall,
+                            "sampled_pixels": int(r.stats["sampled_pixels"]),
+                            "accepted_points": int(r.stats["accepted_points"])}
+            log(f"C5 {storage}: {wall * 1e3:.1f} ms wall, {tm.units / wall:.4g} units/s")
+    finally:
+        sess.close()
+    a, b = res["f32"], res["f16"]
+    out["f16_equals_f32_bitwise"] = bool(a.points.tobytes() == b.points.tobytes()
+                                         and a.samples.tobytes() == b.samples.tobytes())
+    # FP64 oracle on a band of grid rows in the middle of the reference
+    ny = len(range(st["patch_radius"] + 2, p.frames[p.reference].shape[0]
+                   - (st["patch_radius"] + 2), st["grid_stride"]))
+    first = max(0, ny // 2 - rows // 2)
+    O.set_row_band(first, rows)
+    try:
+        t1 = time.perf_counter()
+        o, slack = O.reference_patch(p.frames, p.cameras, p.sparse, p.focal, p.reference,
+                                     sweep=O.params(st), threads=threads)
+        oracle_s = time.perf_counter() - t1
+    finally:
+        O.set_row_band(0, 0)
+    g = b.samples[np.isin(b.samples["y"], np.unique(o.samples["y"]))]
+    os_ = o.samples
+    n = len(os_)
+    err = {"band_rows": [first, first + rows - 1], "samples": int(n),
+           "oracle_s": oracle_s, "oracle_threads": threads}
+    if len(g) == n and n > 0 and np.array_equal(g["x"], os_["x"]):
+        agree = (g["status"] == os_["status"]) & (g["best_hypothesis"] == os_["best_hypothesis"])
+        near = (slack["cost"] < 1e-4) | (slack["px"] < 1e-3) | (slack["rel"] < 1e-5)
+        err["hypothesis_agree"] = float(agree.mean())
+        err["disagree_explained_by_near_tie"] = int((~agree & near).sum())
+        err["disagree_unexplained"] = int((~agree & ~near).sum())
+        has_cost = agree & np.isfinite(os_["best_cost"]) & (os_["best_hypothesis"] >= 0)
+        ce = np.abs(g["best_cost"][has_cost].astype(np.float64)
+                    - os_["best_cost"][has_cost].astype(np.float64))
+        err["cost_abs_err"] = {"max": float(ce.max()) if ce.size else 0.0,
+                               "p99": float(np.percentile(ce, 99)) if ce.size else 0.0}
+        D = st["depth_hypotheses"]
+        inv = lambda h: (1 / st["depth_min"]) * (1 - h / (D - 1)) + (1 / st["depth_max"]) * (h / (D - 1))  # noqa: E731
+        dis = ~agree & (g["best_hypothesis"] >= 0) & (os_["best_hypothesis"] >= 0)
+        dr = np.abs(1 / inv(g["best_hypothesis"][dis]) - 1 / inv(os_["best_hypothesis"][dis])) \
+            * inv(os_["best_hypothesis"][dis])
+        err["depth_rel_err_where_hypothesis_differs"] = {
+            "count": int(dis.sum()), "max": float(dr.max()) if dr.size else 0.0}
+        gp = {(int(q["x"]), int(q["y"])): v for q, v in zip(b.grid_sites, b.points)}
+        op = {(int(q["x"]), int(q["y"])): v for q, v in zip(o.grid_sites, o.points)}
+        same = (agree & ~near & (g["status"] == 6) & (g["reciprocal_signature"]
+                                                      == os_["reciprocal_signature"]))
+        rel = []
+        for i in np.nonzero(same)[0]:
+            k = (int(g["x"][i]), int(g["y"][i]))
+            if k in gp and k in op:
+                va = np.array([gp[k]["x"], gp[k]["y"], gp[k]["z"]], np.float64)
+                vb = np.array([op[k]["x"], op[k]["y"], op[k]["z"]], np.float64)
+                rel.append(np.linalg.norm(va - vb) / max(np.linalg.norm(vb), 1e-12))
+        err["points_compared"] = len(rel)
+        err["point_rel_err"] = {"max": float(max(rel)) if rel else 0.0,
+                                "p99": float(np.percentile(rel, 99)) if rel else 0.0}
+    out["fp16_vs_fp64_oracle"] = err
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -285,6 +377,9 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="measured requests of the serving stream (0: skip the serving leg)")
     ap.add_argument("--serving-warmup", type=int, default=200)
+    ap.add_argument("--precision-rows", type=int, default=2,
+                    help="grid rows of the large solve checked against the FP64 oracle "
+                         "(0: skip the large-solve leg)")
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="grid rows in the CPU-baseline sample (0: the whole keyframe)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -313,6 +408,9 @@ def main() -> int:
     serv = None
     if args.serving_requests > 0:
         serv = serving_leg(V, scenes, args.serving_requests, args.serving_warmup, log)
+    prec = None
+    if args.precision_rows > 0:
+        prec = precision_leg(V, scenes, args.precision_rows, _host_threads(), log)
     S, P = bat.pop("S"), bat.pop("P")
     problem0 = bat.pop("problem0")
     flops_per_unit = 24 * S * P * P
@@ -350,6 +448,7 @@ def main() -> int:
         "latency_source": ("serving open loop, submit -> result on host" if serv
                            else "plain solve wall time"),
         "serving": serv,
+        "large_solve": prec,
         "served_vs_plain": (serv["saturated"]["units_per_s"] / plain["value"] if serv else None),
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,

@@ -193,12 +193,16 @@ __global__ __launch_bounds__(kBlock) void k_build_view(const uint8_t *__restrict
 
 // ---- patch statistics and cost ------------------------------------------------------------
 
-// Occupancy caps (waves per SIMD; a cap of n allows 512 / n VGPRs per lane). The direct
-// sweep and the reciprocal kernel run best at 4, where the compiler keeps fewer gathers in
-// flight per wave and the SIMD holds more waves; the tiled sweep at 3, where its warp batch
-// and reference patch fit without spilling.
-#define VT3D_WAVES __attribute__((amdgpu_waves_per_eu(4)))
-#define VT3D_TILE_WAVES __attribute__((amdgpu_waves_per_eu(3)))
+// Occupancy caps (waves per SIMD; a cap of n allows 512 / n VGPRs per lane), by patch radius.
+// A lane holds its reference patch (P * P values) and one row of gathers in registers. Up
+// to 7 x 7 the direct sweep and the reciprocal kernel run best at 4 waves, where the compiler
+// keeps fewer gathers in flight per wave and the SIMD holds more waves, and the tiled sweep
+// at 3, where its warp batch fits beside the patch; larger patches take fewer waves rather
+// than spill.
+template <int R> constexpr int direct_waves() { return R >= 5 ? 2 : R >= 4 ? 3 : 4; }
+template <int R> constexpr int tile_waves() { return R >= 4 ? 2 : 3; }
+#define VT3D_WAVES __attribute__((amdgpu_waves_per_eu(direct_waves<R>())))
+#define VT3D_TILE_WAVES __attribute__((amdgpu_waves_per_eu(tile_waves<R>())))
 
 struct RefStats {
     float mean;     // mean of the raw reference values (the ZNCC accumulation pivot)
@@ -422,28 +426,33 @@ __device__ __forceinline__ void load_h(const double *__restrict__ H, double (&h)
     for (int k = 0; k < 9; ++k) h[k] = g[k];
 }
 
-// Warps kWarpBatch pixels of a tile's region per thread and stores their bilinear source
+// Warps NB pixels of a tile's region per thread and stores their bilinear source
 // values into LDS, exactly as patch_error evaluates one tap (FP64 homography, FP64 division
 // of the homogeneous coordinates; dense_legacy.rs:1052-1062); NaN marks an invalid tap
 // (source depth <= 1e-4 or outside [0, w-1) x [0, h-1)). Region pixel q (row-major, pitch
 // RW) sits at reference pixel (X0 + q % RW, Y0 + q / RW); thread t handles q = t + j * kBlock.
 // All of a batch's gathers are issued before any is consumed, and an invalid pixel gathers
-// texel 0 (always in bounds) and discards it, so the loop has no branches.
-constexpr int kWarpBatch = 6;
+// texel 0 (always in bounds) and discards it, so the loop has no branches. The batch is
+// smaller for large patches, whose reference values already fill most registers.
+template <int R> constexpr int warp_batch() { return R >= 5 ? 4 : 6; }
 
-template <int ST, bool BUF>
+// Hypotheses per chunk of the tiled sweep: the chunk's sorted source lists (HC * SMAX
+// values) stay within 16 registers (measured: 32 is no faster, 64 spills).
+template <int SMAX> constexpr int hyp_chunk() { return 16 / SMAX > 1 ? 16 / SMAX : 1; }
+
+template <int NB, int ST, bool BUF>
 __device__ __forceinline__ void warp_region(float *__restrict__ L, const double (&h)[9], int X0,
                                             int Y0, int RW, int npx, int px0, int py0,
                                             int step_x, int step_y, const TexRef<ST, BUF> &tex) {
     const uint32_t sw = tex.w;
     const double wlim = (double)(tex.w - 1), hlim = (double)(tex.h - 1);
     int px = px0, py = py0;
-    for (int q0 = (int)threadIdx.x; q0 < npx; q0 += kWarpBatch * kBlock) {
-        uint32_t idx[kWarpBatch];
-        float tx[kWarpBatch], ty[kWarpBatch];
-        bool ok[kWarpBatch];
+    for (int q0 = (int)threadIdx.x; q0 < npx; q0 += NB * kBlock) {
+        uint32_t idx[NB];
+        float tx[NB], ty[NB];
+        bool ok[NB];
 #pragma unroll
-        for (int j = 0; j < kWarpBatch; ++j) {
+        for (int j = 0; j < NB; ++j) {
             const bool in = q0 + j * kBlock < npx;
             const double u = (double)(X0 + px), v = (double)(Y0 + py);
             const double n = fma(h[0], u, fma(h[1], v, h[2]));
@@ -464,12 +473,12 @@ __device__ __forceinline__ void warp_region(float *__restrict__ L, const double 
             px = wrap ? px - RW : px;
             py = wrap ? py + 1 : py;
         }
-        float4 k[kWarpBatch];
+        float4 k[NB];
 #pragma unroll
-        for (int j = 0; j < kWarpBatch; ++j) k[j] = tex.coef(idx[j]);
+        for (int j = 0; j < NB; ++j) k[j] = tex.coef(idx[j]);
         const float nan = __builtin_nanf("");
 #pragma unroll
-        for (int j = 0; j < kWarpBatch; ++j) {
+        for (int j = 0; j < NB; ++j) {
             const int q = q0 + j * kBlock;
             if (q < npx) L[q] = ok[j] ? bicoef(k[j], tx[j], ty[j]) : nan;
         }
@@ -855,26 +864,42 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
     const float maxc = pd.max_cost;
     const float *mine = lds + (ly * stride) * RW + lx * stride;
     Top2 t{-1, 0, kInfF, -1, 0, kInfF};
+    // Sources outermost within a chunk of HC hypotheses: consecutive steps warp the same
+    // source at neighbouring depths, whose footprints nearly coincide (the epipolar shift per
+    // hypothesis is a fraction of a pixel), so a step's texels are mostly still in L1 / L2
+    // from the step before. Each hypothesis of the chunk keeps its own sorted source list.
+    constexpr int HC = hyp_chunk<SMAX>();
     int buf = 0;
-    for (int hyp = h0; hyp < h1; ++hyp) {
-        float l[SMAX];
+    for (int hc0 = h0; hc0 < h1; hc0 += HC) {
+        float l[HC][SMAX];
 #pragma unroll
-        for (int k = 0; k < SMAX; ++k) l[k] = kInfF;
+        for (int j = 0; j < HC; ++j)
+#pragma unroll
+            for (int k = 0; k < SMAX; ++k) l[j][k] = kInfF;
         for (int s = 0; s < S; ++s) {
             const SourceDev &src = pd.src[s];
-            double h[9];
-            load_h(pd.Hp + ((size_t)s * D + hyp) * 9, h);
-            float *L = lds + buf * npx;
             const TexRef<ST, false> tex(src.tex.quad, src.tex.w, src.tex.h);
-            warp_region<ST, false>(L, h, X0, Y0, RW, npx, px0, py0, step_x, step_y, tex);
-            __syncthreads();
-            float c = kInfF;
-            if (active) c = lds_patch_cost<R, COST>(mine + buf * npx, RW, rp, st);
-            c = c <= maxc ? c : kInfF;
-            sorted_insert<SMAX>(l, c);
-            buf ^= 1;
+#pragma unroll
+            for (int j = 0; j < HC; ++j) {
+                const int hyp = hc0 + j;
+                if (hyp < h1) {  // uniform across the workgroup
+                    double h[9];
+                    load_h(pd.Hp + ((size_t)s * D + hyp) * 9, h);
+                    float *L = lds + buf * npx;
+                    warp_region<warp_batch<R>(), ST, false>(L, h, X0, Y0, RW, npx, px0, py0,
+                                                            step_x, step_y, tex);
+                    __syncthreads();
+                    float c = kInfF;
+                    if (active) c = lds_patch_cost<R, COST>(mine + buf * npx, RW, rp, st);
+                    c = c <= maxc ? c : kInfF;
+                    sorted_insert<SMAX>(l[j], c);
+                    buf ^= 1;
+                }
+            }
         }
-        fold_hypothesis<SMAX>(t, l, hyp);
+#pragma unroll
+        for (int j = 0; j < HC; ++j)
+            if (hc0 + j < h1) fold_hypothesis<SMAX>(t, l[j], hc0 + j);
     }
     if (active) a.partial[pd.partial_base + (size_t)hb * pd.cap + sidx] = t;
 }

@@ -363,16 +363,22 @@ def config_c2(seed: int = SEED_C2) -> Problem:
 C3_SIZES = ((640, 360), (360, 640), (480, 270))
 
 
-def _render_many(planes, jobs, seed: int, threads: int):
-    """Renders (camera, W, H, f) jobs on a thread pool; numpy releases the GIL in the large
-    array operations of render(), so threads scale on the host cores."""
+def _render_pool(jobs, threads: int = 0):
+    """render(*job) for each job on a thread pool; numpy releases the GIL in the large array
+    operations of render(), so threads scale on the host cores."""
     import os
     from concurrent.futures import ThreadPoolExecutor
 
     if threads <= 0:
         threads = max(1, min(16, len(os.sched_getaffinity(0))))
     with ThreadPoolExecutor(threads) as ex:
-        return list(ex.map(lambda j: render(planes, j[0], j[1], j[2], j[3], seed)[0], jobs))
+        return list(ex.map(lambda j: render(*j), jobs))
+
+
+def _render_many(planes, jobs, seed: int, threads: int):
+    """Images of (camera, W, H, f) jobs over one scene."""
+    return [img for img, _ in _render_pool([(planes, c, W, H, f, seed) for c, W, H, f in jobs],
+                                           threads)]
 
 
 def config_c3(count: int = 120, seed: int = SEED_C3, sources: int = 8,
@@ -425,7 +431,7 @@ def config_c5(seed: int = SEED_C5, width: int = 1920, height: int = 1080,
         u = np.array([math.cos(th), 0.0, -math.sin(th)])
         planes.append(Plane(c, u, ey, (-0.35, 0.35, -0.3, 0.3), 21 + k, 0.05, 0.10))
     cams = random_walk(sources + 1, seed)
-    frames, depths = zip(*[render(planes, c, W, H, f, seed) for c in cams])
+    frames, depths = zip(*_render_pool([(planes, c, W, H, f, seed) for c in cams]))
     sparse = landmarks(planes, cams, [(W, H)] * len(cams), f, 1024, seed)
     return Problem(f"c5-{W}x{H}", list(frames), cams, sparse, f, sources // 2,
                    {"preset": "zncc", "patch_radius": 5, "depth_hypotheses": 256,
@@ -460,9 +466,6 @@ def c4_stream(n_requests: int, seed: int = SEED_C4, rate: float = 200.0, clips: 
 def c4_clips(clip_spec, n_frames: int = 16, threads: int = 0) -> List[dict]:
     """Frames, cameras and 1024 landmarks of each serving clip: a handheld random walk of
     `n_frames` views through a box room whose texture is seeded per clip."""
-    import os
-    from concurrent.futures import ThreadPoolExecutor
-
     clips = []
     jobs = []
     for spec in clip_spec:
@@ -474,10 +477,7 @@ def c4_clips(clip_spec, n_frames: int = 16, threads: int = 0) -> List[dict]:
         clips.append({"clip": spec["clip"], "size": (W, H), "focal": f, "cameras": cams,
                       "sparse": landmarks(planes, cams, [(W, H)] * n_frames, f, 1024, seed)})
         jobs += [(planes, c, W, H, f, seed) for c in cams]
-    if threads <= 0:
-        threads = max(1, min(16, len(os.sched_getaffinity(0))))
-    with ThreadPoolExecutor(threads) as ex:
-        imgs = list(ex.map(lambda j: render(*j)[0], jobs))
+    imgs = [img for img, _ in _render_pool(jobs, threads)]
     for k, c in enumerate(clips):
         c["frames"] = imgs[k * n_frames:(k + 1) * n_frames]
     return clips
