@@ -26,19 +26,23 @@
 
 namespace vt3d {
 
-bool launch_group(int storage, int cost, int radius, int smax, bool recip, bool tiled,
-                  size_t lds, const Args &a, hipStream_t s);
+bool launch_group(int storage, int cost, int radius, int smax, int kind, bool tiled,
+                  size_t lds, int pass, const Args &a, hipStream_t s);
 bool launch_build_view(int storage, const uint8_t *rgba, uint32_t w, uint32_t h, uint8_t *luma,
                        void *quad, hipStream_t s);
 size_t texel_bytes(int storage);
 size_t tile_lds_bytes(int tile_w, int stride, int radius);
 void launch_gate(const Args &a, hipStream_t s);
-void launch_scan(const Args &a, int gate, hipStream_t s);
+void launch_scan(const Args &a, int stage, uint32_t *count_out, hipStream_t s);
 void launch_problem_offsets(const Args &a, int which, hipStream_t s);
 void launch_compact_sites(const Args &a, hipStream_t s);
 void launch_decide(const Args &a, hipStream_t s);
 void launch_fuse(const Args &a, vt3d_sample_diag *diag, hipStream_t s);
 void launch_compact_points(const Args &a, const Outputs &o, hipStream_t s);
+void launch_occ_init(const Args &a, hipStream_t s);
+void launch_compact_props(const Args &a, hipStream_t s);
+void launch_complete_fuse(const Args &a, int pass, hipStream_t s);
+void launch_compact_additions(const Args &a, const Outputs &o, int pass, hipStream_t s);
 
 namespace {
 
@@ -127,6 +131,8 @@ struct Workspace {
     DevBuf site_flag, samples, site_map, partial, decision, recip, out;
     DevBuf block_counts, block_offsets, meta;
     DevBuf points, sites, attrs, errors, supports, fobs, diag;
+    DevBuf site_pt, prop_site, prop, prop_flag;                 // surface completion
+    DevBuf c_points, c_sites, c_attrs, c_errors, c_supports;    // completed points, per pass
     HostBuf meta_host;
     HostBuf out_host;  // staging of the packed outputs
     hipEvent_t ev[8] = {};
@@ -143,7 +149,9 @@ struct Workspace {
         const DevBuf *all[] = {&prob, &src, &Hp, &Hr, &depth_p, &depth_r, &block_problem,
                                &site_flag, &samples, &site_map, &partial, &decision, &recip,
                                &out, &block_counts, &block_offsets, &meta, &points, &sites,
-                               &attrs, &errors, &supports, &fobs, &diag};
+                               &attrs, &errors, &supports, &fobs, &diag, &site_pt, &prop_site,
+                               &prop, &prop_flag, &c_points, &c_sites, &c_attrs, &c_errors,
+                               &c_supports};
         size_t t = 0;
         for (const DevBuf *b : all) t += b->cap;
         return t;
@@ -635,6 +643,7 @@ struct Group {
     uint32_t p0, p1, items;
     int storage, cost, radius, smax, tiled;
     size_t lds;
+    uint32_t prop_items;  // completion-proposal work items (sweep groups)
 };
 
 int smax_of(int S) { return S <= 4 ? 4 : S <= 8 ? 8 : S <= 16 ? 16 : 32; }
@@ -651,6 +660,7 @@ struct Plan {
     uint32_t slots = 0, n_blocks = 0;
     uint64_t partial_slots = 0, recip_slots = 0;
     bool emit = false;
+    int max_passes = 0;  // surface-completion passes of the batch (largest request)
 };
 
 bool wants_tiled(const Prepared &pr, int tile_w) {
@@ -734,6 +744,8 @@ int build_plan(const std::vector<Prepared *> &reqs, const vt3d_tuning &tuning, P
         const uint32_t ty = (uint32_t)(kBlock / tile_w);
         pd.tiles = std::max<uint32_t>(1, (uint32_t)pd.tiles_x * ((pr.ny + ty - 1) / ty));
         pd.emit_samples = pr.p.emit_samples ? 1 : 0;
+        pd.passes = pr.p.surface_completion_passes;
+        pl.max_passes = std::max(pl.max_passes, pd.passes);
         pl.emit = pl.emit || pd.emit_samples;
         for (uint32_t b = 0; b < pd.cap / kBlock; ++b) pl.block_problem.push_back(k);
         pl.prob.push_back(pd);
@@ -743,7 +755,8 @@ int build_plan(const std::vector<Prepared *> &reqs, const vt3d_tuning &tuning, P
     // Sweep groups (one instantiation each) and their work items.
     for (uint32_t k = 0; k < pl.prob.size();) {
         const Prepared &pr = *pl.dev[k];
-        Group g{k, k, 0, pr.p.storage, pr.p.cost_kind, pr.R, smax_of(pr.S), pl.prob[k].tiled, 0};
+        Group g{k, k, 0, pr.p.storage, pr.p.cost_kind, pr.R, smax_of(pr.S), pl.prob[k].tiled, 0,
+                0};
         uint32_t e = k;
         while (e < pl.prob.size()) {
             const Prepared &q = *pl.dev[e];
@@ -754,6 +767,8 @@ int build_plan(const std::vector<Prepared *> &reqs, const vt3d_tuning &tuning, P
             if (g.tiled) g.lds = std::max(g.lds, tile_lds_bytes(tile_w, (int)q.stride, q.R));
             pd.sweep_item0 = g.items;
             g.items += (g.tiled ? pd.tiles : pd.cap / kBlock) * (uint32_t)pd.n_hb;
+            pd.prop_item0 = g.prop_items;
+            g.prop_items += pd.cap / kBlock;
             ++e;
         }
         g.p1 = e;
@@ -763,7 +778,7 @@ int build_plan(const std::vector<Prepared *> &reqs, const vt3d_tuning &tuning, P
     // Reciprocal groups (storage, cost, radius) and their work items.
     for (uint32_t k = 0; k < pl.prob.size();) {
         const Prepared &pr = *pl.dev[k];
-        Group g{k, k, 0, pr.p.storage, pr.p.cost_kind, pr.R, 0, 0, 0};
+        Group g{k, k, 0, pr.p.storage, pr.p.cost_kind, pr.R, 0, 0, 0, 0};
         uint32_t e = k;
         while (e < pl.prob.size()) {
             const Prepared &q = *pl.dev[e];
@@ -837,7 +852,11 @@ struct vt3d_batch {
     hipGraphExec_t exec = nullptr;
     vt3d::Args args{};
     vt3d::Outputs outs{};
+    vt3d::Outputs comp{};          // completed points, pass k at k * plan.slots
     size_t meta_words = 0;
+    int run_passes = -1;           // completion passes the launch enqueues (-1: the plan's)
+    int row_limit = 1 << 30;       // grid rows of the last pass (cancellation mid-pass)
+    int discard_pass = -1;         // pass whose completions are counted but dropped
     vt3d_timing timing{};
     ~vt3d_batch() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -847,6 +866,25 @@ struct vt3d_batch {
 
 namespace vt3d {
 namespace {
+
+// Layout of the per-problem words copied back after a launch:
+//   totals [stages][n][kCounters] | n_samples [n] | n_props [passes][n] | out_offset [n + 1]
+//   | diag_offset [n + 1] | comp_offset [passes][n + 1]
+struct MetaLayout {
+    size_t n, passes, stages;
+    size_t totals() const { return 0; }
+    size_t n_samples() const { return stages * n * kCounters; }
+    size_t n_props() const { return n_samples() + n; }
+    size_t out_offset() const { return n_props() + passes * n; }
+    size_t diag_offset() const { return out_offset() + n + 1; }
+    size_t comp_offset() const { return diag_offset() + n + 1; }
+    size_t words() const { return comp_offset() + passes * (n + 1); }
+};
+
+MetaLayout meta_layout(const Plan &pl) {
+    const size_t passes = (size_t)pl.max_passes;
+    return MetaLayout{pl.prob.size(), passes, 2 + 2 * passes};
+}
 
 // Uploads the plan's tables into the workspace and fills the launch arguments.
 int upload_plan(vt3d_batch &b) {
@@ -875,8 +913,19 @@ int upload_plan(vt3d_batch &b) {
         (rc = ws.fobs.ensure(4ull * pl.slots)) ||
         (pl.emit && (rc = ws.diag.ensure(sizeof(vt3d_sample_diag) * pl.slots))))
         return rc;
-    // meta: totals [n][kCounters] | n_samples [n] | out_offset [n + 1] | diag_offset [n + 1]
-    b.meta_words = n * kCounters + n + 2 * (n + 1);
+    const size_t cs = (size_t)pl.max_passes * pl.slots;
+    if (pl.max_passes > 0 &&
+        ((rc = ws.site_pt.ensure(sizeof(SitePoint) * pl.slots)) ||
+         (rc = ws.prop_site.ensure(sizeof(Proposal) * pl.slots)) ||
+         (rc = ws.prop.ensure(sizeof(Proposal) * pl.slots)) ||
+         (rc = ws.prop_flag.ensure(pl.slots)) ||
+         (rc = ws.c_points.ensure(sizeof(vt3d_point3) * cs)) ||
+         (rc = ws.c_sites.ensure(sizeof(vt3d_grid_site) * cs)) ||
+         (rc = ws.c_attrs.ensure(sizeof(vt3d_point_attributes) * cs)) ||
+         (rc = ws.c_errors.ensure(4 * cs)) || (rc = ws.c_supports.ensure(4 * cs))))
+        return rc;
+    const MetaLayout ml = meta_layout(pl);
+    b.meta_words = ml.words();
     if ((rc = ws.meta.ensure(4 * b.meta_words)) || (rc = ws.meta_host.ensure(4 * b.meta_words)))
         return rc;
     for (ProblemDev &pd : pl.prob) {
@@ -915,10 +964,20 @@ int upload_plan(vt3d_batch &b) {
     a.block_counts = ws.block_counts.as<uint32_t>();
     a.block_offsets = ws.block_offsets.as<uint32_t>();
     uint32_t *meta = ws.meta.as<uint32_t>();
-    a.totals = meta;
-    a.n_samples = meta + n * kCounters;
-    a.out_offset = a.n_samples + n;
-    a.diag_offset = a.out_offset + n + 1;
+    a.totals = meta + ml.totals();
+    a.n_samples = meta + ml.n_samples();
+    a.n_props = meta + ml.n_props();
+    a.out_offset = meta + ml.out_offset();
+    a.diag_offset = meta + ml.diag_offset();
+    a.comp_offset = meta + ml.comp_offset();
+    if (pl.max_passes > 0) {
+        a.site_pt = ws.site_pt.as<SitePoint>();
+        a.prop_site = ws.prop_site.as<Proposal>();
+        a.prop = ws.prop.as<Proposal>();
+        a.prop_flag = ws.prop_flag.as<uint8_t>();
+    }
+    a.comp_row_limit = 1 << 30;
+    a.comp_discard_pass = -1;
     Outputs &o = b.outs;
     o.points = ws.points.as<vt3d_point3>();
     o.sites = ws.sites.as<vt3d_grid_site>();
@@ -927,16 +986,84 @@ int upload_plan(vt3d_batch &b) {
     o.supports = ws.supports.as<float>();
     o.fused_obs = ws.fobs.as<float>();
     o.diag = pl.emit ? ws.diag.as<vt3d_sample_diag>() : nullptr;
+    Outputs &c = b.comp;
+    c = Outputs{};
+    if (pl.max_passes > 0) {
+        c.points = ws.c_points.as<vt3d_point3>();
+        c.sites = ws.c_sites.as<vt3d_grid_site>();
+        c.attrs = ws.c_attrs.as<vt3d_point_attributes>();
+        c.errors = ws.c_errors.as<float>();
+        c.supports = ws.c_supports.as<float>();
+    }
+    if (b.run_passes < 0) b.run_passes = pl.max_passes;
     b.uploaded = true;
     return VT3D_OK;
 }
 
-// The batch's launches on the session stream, with events between the stages (ev may be
-// null, as under graph capture).
+// Launch arguments of a batch with its current completion limits.
+Args batch_args(const vt3d_batch &b) {
+    Args a = b.args;
+    a.comp_row_limit = b.row_limit;
+    a.comp_discard_pass = b.discard_pass;
+    return a;
+}
+
+// Surface-completion pass k of every problem: proposals, their compaction, the reciprocal
+// check of the proposals, fusion and footprint, and the compaction of the completed points.
+int enqueue_pass(vt3d_batch &b, int k) {
+    hipStream_t st = b.stream;
+    const Plan &pl = b.plan;
+    const Args a = batch_args(b);
+    const uint32_t n = a.n_prob;
+    for (const Group &g : pl.sweep_groups) {
+        Args ga = a;
+        ga.group_p0 = g.p0;
+        ga.group_p1 = g.p1;
+        ga.group_items = g.prop_items;
+        if (!launch_group(g.storage, g.cost, g.radius, g.smax, 2, false, 0, k, ga, st)) {
+            set_error("no kernel for this cost / radius / storage");
+            return VT3D_E_INVALID;
+        }
+    }
+    launch_scan(a, stage_propose(k), a.n_props + (size_t)k * n, st);
+    launch_compact_props(a, st);
+    Args ra = a;
+    ra.n_samples = a.n_props + (size_t)k * n;
+    for (const Group &g : pl.recip_groups) {
+        Args ga = ra;
+        ga.group_p0 = g.p0;
+        ga.group_p1 = g.p1;
+        ga.group_items = g.items;
+        launch_group(g.storage, g.cost, g.radius, 0, 1, false, 0, k, ga, st);
+    }
+    launch_complete_fuse(a, k, st);
+    launch_scan(a, stage_complete(k), nullptr, st);
+    launch_problem_offsets(a, 2 + k, st);
+    Outputs co = b.comp;
+    const size_t off = (size_t)k * pl.slots;
+    co.points += off;
+    co.sites += off;
+    co.attrs += off;
+    co.errors += off;
+    co.supports += off;
+    launch_compact_additions(a, co, k, st);
+    VT3D_HIP(hipGetLastError());
+    return VT3D_OK;
+}
+
+// Copies the per-problem words to pinned memory (the last step of every launch).
+int enqueue_meta(vt3d_batch &b) {
+    VT3D_HIP(hipMemcpyAsync(b.ws->meta_host.p, b.ws->meta.p, 4 * b.meta_words,
+                            hipMemcpyDeviceToHost, b.stream));
+    return VT3D_OK;
+}
+
+// The batch's launches on its stream, with events between the stages (ev may be null, as
+// under graph capture).
 int enqueue(vt3d_batch &b, hipEvent_t *ev) {
     hipStream_t st = b.stream;
     const Plan &pl = b.plan;
-    const Args a = b.args;
+    const Args a = batch_args(b);
     auto mark = [&](int k) -> int {
         if (ev) VT3D_HIP(hipEventRecord(ev[k], st));
         return VT3D_OK;
@@ -944,7 +1071,7 @@ int enqueue(vt3d_batch &b, hipEvent_t *ev) {
     int rc;
     if ((rc = mark(1))) return rc;
     launch_gate(a, st);
-    launch_scan(a, 1, st);
+    launch_scan(a, kStageGate, a.n_samples, st);
     launch_compact_sites(a, st);
     launch_problem_offsets(a, 0, st);
     VT3D_HIP(hipGetLastError());
@@ -954,7 +1081,7 @@ int enqueue(vt3d_batch &b, hipEvent_t *ev) {
         ga.group_p0 = g.p0;
         ga.group_p1 = g.p1;
         ga.group_items = g.items;
-        if (!launch_group(g.storage, g.cost, g.radius, g.smax, false, g.tiled != 0, g.lds, ga,
+        if (!launch_group(g.storage, g.cost, g.radius, g.smax, 0, g.tiled != 0, g.lds, 0, ga,
                           st)) {
             set_error("no kernel for this cost / radius / storage");
             return VT3D_E_INVALID;
@@ -968,17 +1095,22 @@ int enqueue(vt3d_batch &b, hipEvent_t *ev) {
         ga.group_p0 = g.p0;
         ga.group_p1 = g.p1;
         ga.group_items = g.items;
-        launch_group(g.storage, g.cost, g.radius, 0, true, false, 0, ga, st);
+        launch_group(g.storage, g.cost, g.radius, 0, 1, false, 0, 0, ga, st);
     }
     VT3D_HIP(hipGetLastError());
     if ((rc = mark(4))) return rc;
     launch_fuse(a, b.outs.diag, st);
-    launch_scan(a, 0, st);
+    launch_scan(a, kStageFuse, nullptr, st);
     launch_problem_offsets(a, 1, st);
     launch_compact_points(a, b.outs, st);
     VT3D_HIP(hipGetLastError());
-    VT3D_HIP(hipMemcpyAsync(b.ws->meta_host.p, b.ws->meta.p, 4 * b.meta_words,
-                            hipMemcpyDeviceToHost, st));
+    if ((rc = mark(7))) return rc;
+    if (b.run_passes > 0) {
+        launch_occ_init(a, st);
+        for (int k = 0; k < b.run_passes; ++k)
+            if ((rc = enqueue_pass(b, k))) return rc;
+    }
+    if ((rc = enqueue_meta(b))) return rc;
     return mark(5);
 }
 
@@ -1008,12 +1140,12 @@ int launch_batch(vt3d_batch &b) {
                 VT3D_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
             }
             VT3D_HIP(hipGraphLaunch(b.exec, st));
-            for (int k = 1; k <= 5; ++k) VT3D_HIP(hipEventRecord(ws.ev[k], st));
+            for (int k : {1, 2, 3, 4, 7, 5}) VT3D_HIP(hipEventRecord(ws.ev[k], st));
         } else if ((rc = enqueue(b, ws.ev))) {
             return rc;
         }
     } else {
-        for (int k = 1; k <= 5; ++k) VT3D_HIP(hipEventRecord(ws.ev[k], st));
+        for (int k : {1, 2, 3, 4, 7, 5}) VT3D_HIP(hipEventRecord(ws.ev[k], st));
     }
     return VT3D_OK;
 }
@@ -1030,42 +1162,76 @@ int finish_batch(vt3d_batch &b, vt3d_dense_result **results) {
     VT3D_HIP(hipStreamSynchronize(st));
 
     // Packed outputs: one copy per array into pinned staging memory.
+    const MetaLayout ml = meta_layout(b.plan);
     const uint32_t *meta = b.ws->meta_host.as<uint32_t>();
-    const uint32_t *totals = meta;
-    const uint32_t *n_samples = meta + n * kCounters;
-    const uint32_t *out_offset = n_samples + n;
-    const uint32_t *diag_offset = out_offset + n + 1;
+    const uint32_t *totals = meta + ml.totals();
+    const uint32_t *n_samples = meta + ml.n_samples();
+    const uint32_t *out_offset = meta + ml.out_offset();
+    const uint32_t *diag_offset = meta + ml.diag_offset();
+    const uint32_t *comp_offset = meta + ml.comp_offset();
+    const int passes = std::max(0, b.run_passes);
+    auto tot = [&](int stage, uint32_t k, int c) {
+        return totals[((size_t)stage * n + k) * kCounters + c];
+    };
     const uint32_t n_acc = n ? out_offset[n] : 0;
     const uint32_t n_diag = n ? diag_offset[n] : 0;
-    const size_t off_sites = sizeof(vt3d_point3) * (size_t)n_acc;
-    const size_t off_attrs = off_sites + sizeof(vt3d_grid_site) * (size_t)n_acc;
-    const size_t off_errs = off_attrs + sizeof(vt3d_point_attributes) * (size_t)n_acc;
-    const size_t off_sups = off_errs + 4ull * n_acc;
-    const size_t off_fobs = off_sups + 4ull * n_acc;
-    const size_t off_diag = (off_fobs + 4ull * n_acc + 15) & ~(size_t)15;
-    const size_t host_bytes = off_diag + sizeof(vt3d_sample_diag) * (size_t)n_diag;
-    if ((rc = b.ws->out_host.ensure(host_bytes))) return rc;
+    size_t off = 0;
+    auto place = [&](size_t bytes) {
+        const size_t o = off;
+        off = (off + bytes + 15) & ~(size_t)15;
+        return o;
+    };
+    const size_t off_pts = place(sizeof(vt3d_point3) * (size_t)n_acc);
+    const size_t off_sites = place(sizeof(vt3d_grid_site) * (size_t)n_acc);
+    const size_t off_attrs = place(sizeof(vt3d_point_attributes) * (size_t)n_acc);
+    const size_t off_errs = place(4ull * n_acc);
+    const size_t off_sups = place(4ull * n_acc);
+    const size_t off_fobs = place(4ull * n_acc);
+    const size_t off_diag = place(sizeof(vt3d_sample_diag) * (size_t)n_diag);
+    std::vector<uint32_t> c_tot(passes);
+    std::vector<size_t> c_pts(passes), c_sites(passes), c_attrs(passes), c_errs(passes),
+        c_sups(passes);
+    for (int k = 0; k < passes; ++k) {
+        c_tot[k] = n ? comp_offset[(size_t)k * (n + 1) + n] : 0;
+        c_pts[k] = place(sizeof(vt3d_point3) * (size_t)c_tot[k]);
+        c_sites[k] = place(sizeof(vt3d_grid_site) * (size_t)c_tot[k]);
+        c_attrs[k] = place(sizeof(vt3d_point_attributes) * (size_t)c_tot[k]);
+        c_errs[k] = place(4ull * c_tot[k]);
+        c_sups[k] = place(4ull * c_tot[k]);
+    }
+    if ((rc = b.ws->out_host.ensure(off))) return rc;
     uint8_t *hb = b.ws->out_host.as<uint8_t>();
-    auto d2h = [&](size_t off, const void *src, size_t bytes) -> int {
-        if (bytes) VT3D_HIP(hipMemcpyAsync(hb + off, src, bytes, hipMemcpyDeviceToHost, st));
+    auto d2h = [&](size_t o, const void *src, size_t bytes) -> int {
+        if (bytes) VT3D_HIP(hipMemcpyAsync(hb + o, src, bytes, hipMemcpyDeviceToHost, st));
         return VT3D_OK;
     };
-    if ((rc = d2h(0, b.outs.points, off_sites)) ||
-        (rc = d2h(off_sites, b.outs.sites, off_attrs - off_sites)) ||
-        (rc = d2h(off_attrs, b.outs.attrs, off_errs - off_attrs)) ||
+    if ((rc = d2h(off_pts, b.outs.points, sizeof(vt3d_point3) * (size_t)n_acc)) ||
+        (rc = d2h(off_sites, b.outs.sites, sizeof(vt3d_grid_site) * (size_t)n_acc)) ||
+        (rc = d2h(off_attrs, b.outs.attrs, sizeof(vt3d_point_attributes) * (size_t)n_acc)) ||
         (rc = d2h(off_errs, b.outs.errors, 4ull * n_acc)) ||
         (rc = d2h(off_sups, b.outs.supports, 4ull * n_acc)) ||
         (rc = d2h(off_fobs, b.outs.fused_obs, 4ull * n_acc)) ||
         (rc = d2h(off_diag, b.outs.diag, sizeof(vt3d_sample_diag) * (size_t)n_diag)))
         return rc;
+    for (int k = 0; k < passes; ++k) {
+        const size_t base = (size_t)k * b.plan.slots;
+        if ((rc = d2h(c_pts[k], b.comp.points + base, sizeof(vt3d_point3) * (size_t)c_tot[k])) ||
+            (rc = d2h(c_sites[k], b.comp.sites + base,
+                      sizeof(vt3d_grid_site) * (size_t)c_tot[k])) ||
+            (rc = d2h(c_attrs[k], b.comp.attrs + base,
+                      sizeof(vt3d_point_attributes) * (size_t)c_tot[k])) ||
+            (rc = d2h(c_errs[k], b.comp.errors + base, 4ull * c_tot[k])) ||
+            (rc = d2h(c_sups[k], b.comp.supports + base, 4ull * c_tot[k])))
+            return rc;
+    }
     VT3D_HIP(hipEventRecord(ws.ev[6], st));
     VT3D_HIP(hipStreamSynchronize(st));
-    const vt3d_point3 *points = reinterpret_cast<const vt3d_point3 *>(hb);
+    const vt3d_point3 *points = reinterpret_cast<const vt3d_point3 *>(hb + off_pts);
     const vt3d_grid_site *sites = reinterpret_cast<const vt3d_grid_site *>(hb + off_sites);
     const vt3d_point_attributes *attrs =
         reinterpret_cast<const vt3d_point_attributes *>(hb + off_attrs);
-    float *errs = reinterpret_cast<float *>(hb + off_errs);
-    float *sups = reinterpret_cast<float *>(hb + off_sups);
+    const float *errs = reinterpret_cast<const float *>(hb + off_errs);
+    const float *sups = reinterpret_cast<const float *>(hb + off_sups);
     float *fob = reinterpret_cast<float *>(hb + off_fobs);
     const vt3d_sample_diag *diag = reinterpret_cast<const vt3d_sample_diag *>(hb + off_diag);
 
@@ -1074,13 +1240,14 @@ int finish_batch(vt3d_batch &b, vt3d_dense_result **results) {
     tm.gate_ms = elapsed(ws.ev[1], ws.ev[2]);
     tm.sweep_ms = elapsed(ws.ev[2], ws.ev[3]);
     tm.reciprocal_ms = elapsed(ws.ev[3], ws.ev[4]);
-    tm.fuse_ms = elapsed(ws.ev[4], ws.ev[5]);
+    tm.fuse_ms = elapsed(ws.ev[4], ws.ev[7]);
+    tm.completion_ms = elapsed(ws.ev[7], ws.ev[5]);
     tm.download_ms = elapsed(ws.ev[5], ws.ev[6]);
     tm.total_ms = elapsed(ws.ev[0], ws.ev[6]);
 
     // Medians (median_option, dense_legacy.rs:1176-1182): the sorted value at index
-    // round((n - 1) / 2), i.e. n / 2. The staging copies are scratch, so selection runs in
-    // place.
+    // round((n - 1) / 2), i.e. n / 2. Errors and supports include the completed points
+    // (dense_legacy.rs:728-734), fusion observation counts only the primary ones.
     auto median = [](float *v, size_t cnt, float *m) {
         if (cnt == 0) return 0;
         std::nth_element(v, v + cnt / 2, v + cnt);
@@ -1099,20 +1266,45 @@ int finish_batch(vt3d_batch &b, vt3d_dense_result **results) {
             return VT3D_OK;
         }
         const uint32_t k = b.plan.slot_of[i];
-        const uint32_t *tot = totals + (size_t)k * kCounters;
-        const uint32_t o0 = out_offset[k], na = out_offset[k + 1] - o0;
+        const uint32_t o0 = out_offset[k], na0 = out_offset[k + 1] - o0;
+        std::vector<uint32_t> ck0(passes), nak(passes);
+        size_t na = na0;
+        for (int q = 0; q < passes; ++q) {
+            ck0[q] = comp_offset[(size_t)q * (n + 1) + k];
+            nak[q] = comp_offset[(size_t)q * (n + 1) + k + 1] - ck0[q];
+            na += nak[q];
+        }
         vt3d_dense_result *r = new_result();
         if (!r) return VT3D_E_DEVICE;
         results[i] = r;
+        std::vector<float> e_all(na), s_all(na);
         if (na) {
             r->points = (vt3d_point3 *)std::malloc(sizeof(vt3d_point3) * na);
             r->grid_sites = (vt3d_grid_site *)std::malloc(sizeof(vt3d_grid_site) * na);
             r->attributes =
                 (vt3d_point_attributes *)std::malloc(sizeof(vt3d_point_attributes) * na);
             if (!r->points || !r->grid_sites || !r->attributes) return VT3D_E_DEVICE;
-            std::memcpy(r->points, points + o0, sizeof(vt3d_point3) * na);
-            std::memcpy(r->grid_sites, sites + o0, sizeof(vt3d_grid_site) * na);
-            std::memcpy(r->attributes, attrs + o0, sizeof(vt3d_point_attributes) * na);
+            std::memcpy(r->points, points + o0, sizeof(vt3d_point3) * na0);
+            std::memcpy(r->grid_sites, sites + o0, sizeof(vt3d_grid_site) * na0);
+            std::memcpy(r->attributes, attrs + o0, sizeof(vt3d_point_attributes) * na0);
+            std::memcpy(e_all.data(), errs + o0, 4ull * na0);
+            std::memcpy(s_all.data(), sups + o0, 4ull * na0);
+            size_t at = na0;
+            for (int q = 0; q < passes; ++q) {
+                if (!nak[q]) continue;
+                const auto *cp = reinterpret_cast<const vt3d_point3 *>(hb + c_pts[q]);
+                const auto *cs = reinterpret_cast<const vt3d_grid_site *>(hb + c_sites[q]);
+                const auto *ca = reinterpret_cast<const vt3d_point_attributes *>(hb + c_attrs[q]);
+                const auto *ce = reinterpret_cast<const float *>(hb + c_errs[q]);
+                const auto *cu = reinterpret_cast<const float *>(hb + c_sups[q]);
+                std::memcpy(r->points + at, cp + ck0[q], sizeof(vt3d_point3) * nak[q]);
+                std::memcpy(r->grid_sites + at, cs + ck0[q], sizeof(vt3d_grid_site) * nak[q]);
+                std::memcpy(r->attributes + at, ca + ck0[q],
+                            sizeof(vt3d_point_attributes) * nak[q]);
+                std::memcpy(e_all.data() + at, ce + ck0[q], 4ull * nak[q]);
+                std::memcpy(s_all.data() + at, cu + ck0[q], 4ull * nak[q]);
+                at += nak[q];
+            }
         }
         const uint32_t ns = n_samples[k];
         if (pr.p.emit_samples && ns) {
@@ -1132,20 +1324,31 @@ int finish_batch(vt3d_batch &b, vt3d_dense_result **results) {
         sx.accepted_points = na;
         sx.grid_stride = pr.stride;
         sx.grid_border = pr.border;
-        sx.reciprocal_checked_points = tot[1];
-        sx.reciprocal_rejected_points = tot[2];
-        sx.reciprocal_consistent_points = tot[3];
-        sx.fusion_input_observations = tot[4];
-        sx.fusion_rejected_observations = tot[5];
-        sx.fusion_rejected_points = tot[6];
-        sx.has_median_fusion_observations = median(fob + o0, na, &sx.median_fusion_observations);
-        sx.has_median_supporting_views = median(sups + o0, na, &sx.median_supporting_views);
-        sx.has_median_photometric_error = median(errs + o0, na, &sx.median_photometric_error);
+        sx.reciprocal_checked_points = tot(kStageFuse, k, 1);
+        sx.reciprocal_rejected_points = tot(kStageFuse, k, 2);
+        sx.reciprocal_consistent_points = tot(kStageFuse, k, 3);
+        sx.fusion_input_observations = tot(kStageFuse, k, 4);
+        sx.fusion_rejected_observations = tot(kStageFuse, k, 5);
+        sx.fusion_rejected_points = tot(kStageFuse, k, 6);
+        for (int q = 0; q < passes; ++q) {
+            sx.surface_completion_proposals += tot(stage_propose(q), k, 1);
+            sx.surface_completion_rejected_texture += tot(stage_propose(q), k, 2);
+            sx.surface_completion_rejected_cross_view += tot(stage_propose(q), k, 3);
+            sx.surface_completed_points += tot(stage_complete(q), k, 0);
+            sx.surface_completion_rejected_reciprocal += tot(stage_complete(q), k, 1);
+            sx.surface_completion_rejected_fusion += tot(stage_complete(q), k, 2);
+            sx.surface_completion_rejected_footprint += tot(stage_complete(q), k, 3);
+        }
+        sx.has_median_fusion_observations =
+            median(fob + o0, na0, &sx.median_fusion_observations);
+        sx.has_median_supporting_views = median(s_all.data(), na, &sx.median_supporting_views);
+        sx.has_median_photometric_error =
+            median(e_all.data(), na, &sx.median_photometric_error);
         sx.has_search_depth = 1;
         sx.search_min_depth = (float)pr.sweep_min;
         sx.search_max_depth = (float)pr.sweep_max;
         r->n_points = na;
-        r->primary_points = na;
+        r->primary_points = na0;
         return VT3D_OK;
     };
     // Large batches build their results on host threads (requests are independent).
@@ -1206,8 +1409,8 @@ int resolve_params(const vt3d_sweep_params *params, vt3d_sweep_params *resolved)
         set_error("invalid vt3d_sweep_params");
         return VT3D_E_INVALID;
     }
-    if (resolved->surface_completion_passes > 0) {
-        set_error("surface completion is not available on the device path");
+    if (resolved->surface_completion_passes > 64) {
+        set_error("surface_completion_passes above 64");
         return VT3D_E_INVALID;
     }
     return VT3D_OK;
@@ -1241,7 +1444,46 @@ int solve_one(vt3d_session *s, FrameSource &fs, const vt3d_camera *cameras, size
     std::vector<Prepared *> reqs{&pr};
     if ((rc = build_plan(reqs, s->tuning, b.plan))) return rc;
     vt3d_dense_result *res = nullptr;
-    if ((rc = run_batch(b, &res))) {
+    const int passes = pr.p.surface_completion_passes;
+    if (!poll || passes == 0) {
+        rc = run_batch(b, &res);
+    } else {
+        // complete_surface_samples_with_control polls once per grid row of each pass it runs
+        // (dense_legacy.rs:552-555), and a pass that completes nothing ends completion. A
+        // cancel at row r of pass k keeps that pass's counters for rows < r and drops its
+        // completions; the analysis is returned as it stands.
+        auto poll_rows = [&](int *limit) {
+            for (uint32_t r = 0; r < pr.ny; ++r)
+                if (ctl.is_canceled()) {
+                    *limit = (int)r;
+                    return true;
+                }
+            *limit = 1 << 30;
+            return false;
+        };
+        int limit = 0;
+        bool canceled = poll_rows(&limit);
+        b.run_passes = 1;
+        b.row_limit = limit;
+        b.discard_pass = canceled ? 0 : -1;
+        rc = launch_batch(b);
+        int k = 1;
+        while (!rc && !canceled && k < passes) {
+            VT3D_HIP(hipStreamSynchronize(b.stream));
+            const MetaLayout ml = meta_layout(b.plan);
+            const uint32_t added =
+                b.ws->meta_host.as<uint32_t>()[ml.totals() + (size_t)stage_complete(k - 1) *
+                                                                  kCounters];
+            if (added == 0) break;
+            canceled = poll_rows(&limit);
+            b.row_limit = limit;
+            b.discard_pass = canceled ? k : -1;
+            if (!(rc = enqueue_pass(b, k))) rc = enqueue_meta(b);
+            b.run_passes = ++k;
+        }
+        if (!rc) rc = finish_batch(b, &res);
+    }
+    if (rc) {
         vt3d_dense_result_free(res);
         return rc;
     }

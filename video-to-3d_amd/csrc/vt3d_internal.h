@@ -75,7 +75,9 @@ struct ProblemDev {
     int tiled, tile_w, tiles_x;
     uint32_t tiles;
     int emit_samples;
-    int pad1;
+    int passes;              // surface completion passes
+    uint32_t prop_item0;     // first work item of the completion proposal kernel
+    uint32_t pad1;
 };
 
 // Best and comparable-second candidate of one sample over a block of hypotheses, in the
@@ -123,6 +125,35 @@ struct SampleOut {
     int support;
 };
 
+// The point a grid site holds during surface completion: the f32 output values the
+// reference reads back from its point list (position, confidence, depth margin).
+struct SitePoint {
+    float x, y, z, conf, margin;
+    int occupied;
+};
+
+// One surface-completion proposal (complete_surface_samples_with_control,
+// dense_legacy.rs:549-695): site, predicted depth and position, its re-projection, and the
+// direct check's outcome.
+struct Proposal {
+    uint32_t site;           // grid site index within the problem (row-major)
+    int direct_count;        // sources whose direct error passes the photometric gate
+    float direct_error;      // median of those errors
+    int proj_ok;
+    double nconf, nmargin;   // neighbour confidence and depth-margin means
+    double depth;            // predicted depth
+    double pos[3];           // unproject(reference, x, y, depth)
+    double xr, yr, dr;       // project(reference, pos)
+};
+
+// Counter stages of a batch's compaction scans: totals[stage][problem][kCounters].
+constexpr int kStageGate = 0;      // [0] texture-gated samples
+constexpr int kStageFuse = 1;      // [0] accepted, [1..6] reciprocal / fusion counters
+__host__ __device__ constexpr int stage_propose(int pass) { return 2 + 2 * pass; }
+// [0] sent to the reciprocal check, [1] proposals, [2] rejected texture, [3] rejected cross view
+__host__ __device__ constexpr int stage_complete(int pass) { return 3 + 2 * pass; }
+// [0] completed, [1] rejected reciprocal, [2] rejected fusion, [3] rejected footprint
+
 // Kernel arguments of a batch.
 struct Args {
     const ProblemDev *prob;
@@ -143,10 +174,19 @@ struct Args {
     // compaction scans
     uint32_t *block_counts;         // [n_blocks][kCounters]
     uint32_t *block_offsets;        // [n_blocks][kCounters], offsets within the problem
-    uint32_t *totals;               // [n_prob][kCounters]
+    uint32_t *totals;               // [n_stages][n_prob][kCounters]
     uint32_t *n_samples;            // [n_prob]
     uint32_t *out_offset;           // [n_prob + 1] packed offsets of accepted points
     uint32_t *diag_offset;          // [n_prob + 1] packed offsets of sample diagnostics
+    // surface completion
+    SitePoint *site_pt;             // per slot (site-indexed)
+    Proposal *prop_site;            // per slot (site-indexed), this pass's proposals
+    Proposal *prop;                 // per slot, compacted in site order
+    uint8_t *prop_flag;             // per slot (site-indexed): proposal goes to the reciprocal check
+    uint32_t *n_props;              // [passes][n_prob]
+    uint32_t *comp_offset;          // [passes][n_prob + 1] packed offsets of completed points
+    int comp_row_limit;             // grid rows a pass considers (cancellation mid-pass)
+    int comp_discard_pass;          // pass whose completions are counted but dropped, or -1
 };
 
 // Packed outputs of a batch (problem p's entries start at out_offset[p] / diag_offset[p]).

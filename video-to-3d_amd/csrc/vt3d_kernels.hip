@@ -561,8 +561,10 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 }
 
 // Per-block category counts: each thread contributes c[k]; the block's totals land in
-// block_counts[blockIdx.x][k]. Integer sums, so the result does not depend on order.
-__device__ __forceinline__ void block_count(const uint32_t (&c)[kCounters], uint32_t *dst) {
+// block_counts[block][k] (block: the block's index over all problems' slots). Integer sums,
+// so the result does not depend on order.
+__device__ __forceinline__ void block_count(const uint32_t (&c)[kCounters], uint32_t *dst,
+                                            uint32_t block) {
     __shared__ uint32_t red[kBlock / 64][kCounters];
     const int wave = threadIdx.x >> 6;
 #pragma unroll
@@ -576,7 +578,7 @@ __device__ __forceinline__ void block_count(const uint32_t (&c)[kCounters], uint
     if (threadIdx.x < kCounters) {
         uint32_t s = 0;
         for (int wv = 0; wv < kBlock / 64; ++wv) s += red[wv][threadIdx.x];
-        dst[blockIdx.x * kCounters + threadIdx.x] = s;
+        dst[block * kCounters + threadIdx.x] = s;
     }
 }
 
@@ -593,13 +595,15 @@ __device__ __forceinline__ uint32_t block_rank(bool flag) {
 }
 
 // Problem of a work item within the launched group: the last problem of the group whose
-// first item is <= item. RECIP selects the reciprocal kernel's item numbering.
-template <bool RECIP>
+// first item is <= item. KIND selects the item numbering: 0 primary sweep, 1 reciprocal
+// kernel, 2 surface-completion proposals.
+template <int KIND>
 __device__ __forceinline__ uint32_t problem_of_item(const Args &a, uint32_t item) {
     uint32_t lo = a.group_p0, hi = a.group_p1;
     while (hi - lo > 1) {
         const uint32_t mid = (lo + hi) >> 1;
-        const uint32_t first = RECIP ? a.prob[mid].recip_item0 : a.prob[mid].sweep_item0;
+        const ProblemDev &q = a.prob[mid];
+        const uint32_t first = KIND == 0 ? q.sweep_item0 : KIND == 1 ? q.recip_item0 : q.prop_item0;
         if (first <= item) lo = mid;
         else hi = mid;
     }
@@ -631,19 +635,21 @@ __global__ __launch_bounds__(kBlock) void k_gate(Args a) {
         a.site_flag[pd.base + j] = pass ? 1 : 0;
         c[0] = pass ? 1 : 0;
     }
-    block_count(c, a.block_counts);
+    block_count(c, a.block_counts, blockIdx.x);
 }
 
 // Exclusive scan of block_counts over each problem's blocks, per category; one workgroup of
-// 1024 threads per problem. gate != 0: the first category is the texture gate, whose total
-// is the problem's sample count.
-__global__ __launch_bounds__(1024) void k_scan(Args a, int gate) {
+// 1024 threads per problem. The problem's totals land in counter stage `stage`; count_out,
+// if given, receives the total of the first category (the problem's item count for the
+// next stage).
+__global__ __launch_bounds__(1024) void k_scan(Args a, int stage, uint32_t *count_out) {
     __shared__ uint32_t wave_sum[16];
     __shared__ uint32_t carry_s;
     const uint32_t p = blockIdx.x;
     const ProblemDev &pd = a.prob[p];
     const uint32_t b0 = pd.block0, nb = pd.cap / kBlock;
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
+    uint32_t *totals = a.totals + ((size_t)stage * a.n_prob + p) * kCounters;
     for (int k = 0; k < kCounters; ++k) {
         if (threadIdx.x == 0) carry_s = 0;
         __syncthreads();
@@ -667,21 +673,25 @@ __global__ __launch_bounds__(1024) void k_scan(Args a, int gate) {
             __syncthreads();
         }
         if (threadIdx.x == 0) {
-            a.totals[p * kCounters + k] = carry_s;
-            if (gate && k == 0) a.n_samples[p] = carry_s;
+            totals[k] = carry_s;
+            if (count_out && k == 0) count_out[p] = carry_s;
         }
         __syncthreads();
     }
 }
 
-// Exclusive scan over problems: which == 0, sample diagnostics (n_samples of problems that
-// emit them) into diag_offset; which == 1, accepted points (totals[p][0]) into out_offset.
-// Entry n_prob holds the batch total. One workgroup of 1024 threads.
+// Exclusive scan over problems into a packed-output offset table (entry n_prob holds the
+// batch total): which == 0, sample diagnostics (n_samples of problems that emit them) into
+// diag_offset; which == 1, accepted points into out_offset; which == 2 + k, the points
+// completed by surface-completion pass k into comp_offset[k]. One workgroup of 1024 threads.
 __global__ __launch_bounds__(1024) void k_problem_offsets(Args a, int which) {
     __shared__ uint32_t wave_sum[16];
     __shared__ uint32_t carry_s;
     const int lane = __lane_id(), wave = threadIdx.x >> 6;
-    uint32_t *dst = which == 0 ? a.diag_offset : a.out_offset;
+    uint32_t *dst = which == 0   ? a.diag_offset
+                    : which == 1 ? a.out_offset
+                                 : a.comp_offset + (size_t)(which - 2) * (a.n_prob + 1);
+    const int stage = which == 1 ? kStageFuse : stage_complete(which - 2);
     if (threadIdx.x == 0) carry_s = 0;
     __syncthreads();
     for (uint32_t base = 0; base < a.n_prob; base += 1024) {
@@ -689,7 +699,7 @@ __global__ __launch_bounds__(1024) void k_problem_offsets(Args a, int which) {
         uint32_t v = 0;
         if (p < a.n_prob)
             v = which == 0 ? (a.prob[p].emit_samples ? a.n_samples[p] : 0u)
-                           : a.totals[p * kCounters];
+                           : a.totals[((size_t)stage * a.n_prob + p) * kCounters];
         uint32_t incl = v;
 #pragma unroll
         for (int off = 1; off < 64; off <<= 1) {
@@ -766,7 +776,7 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep(Args a) {
     constexpr int T = P * P;
     const uint32_t item = xcd_chunk(blockIdx.x, gridDim.x);
     if (item >= a.group_items) return;
-    const uint32_t p = problem_of_item<false>(a, item);
+    const uint32_t p = problem_of_item<0>(a, item);
     const ProblemDev &pd = a.prob[p];
     const uint32_t local = item - pd.sweep_item0;
     const int hb = (int)(local % (uint32_t)pd.n_hb);
@@ -825,7 +835,7 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
     constexpr int T = P * P;
     const uint32_t item = xcd_chunk(blockIdx.x, gridDim.x);
     if (item >= a.group_items) return;  // padding: the whole block leaves before any barrier
-    const uint32_t p = problem_of_item<false>(a, item);
+    const uint32_t p = problem_of_item<0>(a, item);
     const ProblemDev &pd = a.prob[p];
     const uint32_t local = item - pd.sweep_item0;
     const int tile = (int)(local / (uint32_t)pd.n_hb);
@@ -999,7 +1009,7 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     constexpr int T = P * P;
     const uint32_t item = xcd_chunk(blockIdx.x, gridDim.x);
     if (item >= a.group_items) return;
-    const uint32_t p = problem_of_item<true>(a, item);
+    const uint32_t p = problem_of_item<1>(a, item);
     const ProblemDev &pd = a.prob[p];
     const uint32_t local = item - pd.recip_item0;
     const int s = (int)(local % (uint32_t)pd.S);
@@ -1171,7 +1181,7 @@ __global__ __launch_bounds__(kBlock) void k_fuse(Args a, vt3d_sample_diag *diag)
             diag[a.diag_offset[p] + i] = g;
         }
     }
-    block_count(c, a.block_counts);
+    block_count(c, a.block_counts, blockIdx.x);
 }
 
 // Accepted samples -> packed outputs in reference order (problem p's points start at
@@ -1204,6 +1214,304 @@ __global__ __launch_bounds__(kBlock) void k_compact_points(Args a, Outputs o) {
     o.errors[dst] = s.error;
     o.supports[dst] = (float)s.support;
     o.fused_obs[dst] = (float)s.fused_obs;
+}
+
+// ---- surface completion --------------------------------------------------------------------
+//
+// complete_surface_samples_with_control (dense_legacy.rs:519-737): up to `passes` passes
+// over the grid sites without a point; a site whose 8-neighbourhood (grid offsets +-1) holds
+// at least 3 points with depths within a 6% spread proposes the median depth, which must
+// pass the texture gate, the direct check against at least one source, the reciprocal
+// check, fusion and a half-stride footprint test. A pass sees the points as they were at
+// its start; its completions are appended (in row-major order) before the next pass, and a
+// pass that completes nothing ends the problem's completion.
+
+constexpr int kMinSurfaceNeighbors = 3;                // dense_legacy.rs:17
+constexpr double kMaxSurfaceNeighborSpread = 0.06;     // dense_legacy.rs:18
+constexpr double kSurfaceConfidenceScale = 0.85;       // dense_legacy.rs:19
+
+// Grid sites -> their current points (the accepted primary samples).
+__global__ __launch_bounds__(kBlock) void k_occ_init(Args a) {
+    const uint32_t b = blockIdx.x;
+    const ProblemDev &pd = a.prob[a.block_problem[b]];
+    const uint32_t nsites = pd.grid_nx * pd.grid_ny;
+    const uint32_t j = (b - pd.block0) * kBlock + threadIdx.x;
+    if (j >= nsites || pd.passes <= 0) return;
+    SitePoint sp{0.f, 0.f, 0.f, 0.f, 0.f, 0};
+    const int i = a.sample_of_site[pd.base + j];
+    if (i >= 0) {
+        const SampleOut &o = a.out[pd.base + i];
+        if (o.status == VT3D_SAMPLE_ACCEPTED)
+            sp = SitePoint{o.px, o.py, o.pz, o.conf, o.depth_margin, 1};
+    }
+    a.site_pt[pd.base + j] = sp;
+}
+
+template <int N>
+__device__ __forceinline__ void sorted_insert_d(double (&l)[N], double c) {
+#pragma unroll
+    for (int k = 0; k < N; ++k) {
+        const double lo = fmin(l[k], c);
+        c = fmax(l[k], c);
+        l[k] = lo;
+    }
+}
+
+// One lane per grid site: neighbour prediction (surface_depth_prediction,
+// dense_legacy.rs:739-753), texture gate, predicted position and the direct check against
+// every source (a one-hypothesis sweep at the predicted depth); counters of stage
+// stage_propose(pass). A work item is (problem, block of 256 sites).
+template <int R, int COST, int ST, int SMAX>
+__global__ __launch_bounds__(kBlock) VT3D_WAVES void k_propose(Args a, int pass) {
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    const uint32_t item = xcd_chunk(blockIdx.x, gridDim.x);
+    if (item >= a.group_items) return;  // padding: the whole block leaves before any barrier
+    const uint32_t p = problem_of_item<2>(a, item);
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t blk = item - pd.prop_item0;
+    const uint32_t j = blk * kBlock + threadIdx.x;
+    const uint32_t nx = pd.grid_nx, nsites = nx * pd.grid_ny;
+    uint32_t c[kCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
+    uint8_t flag = 0;
+    bool live = j < nsites && pass < pd.passes;
+    if (live && pass > 0)
+        live = a.totals[((size_t)stage_complete(pass - 1) * a.n_prob + p) * kCounters] > 0;
+    const int gx = (int)(j % nx), gy = (int)(j / nx);
+    if (live) live = gy < a.comp_row_limit && !a.site_pt[pd.base + j].occupied;
+    if (live) {
+        // neighbours in the reference's order (offset_y outer, offset_x inner)
+        double dep[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) dep[k] = __builtin_huge_val();
+        int n = 0;
+        double csum = 0.0, msum = 0.0;
+#pragma unroll
+        for (int oy = -1; oy <= 1; ++oy)
+#pragma unroll
+            for (int ox = -1; ox <= 1; ++ox) {
+                if (ox == 0 && oy == 0) continue;
+                const int qx = gx + ox, qy = gy + oy;
+                if (qx < 0 || qy < 0 || qx >= (int)nx || qy >= (int)pd.grid_ny) continue;
+                const SitePoint q = a.site_pt[pd.base + (uint32_t)qy * nx + (uint32_t)qx];
+                if (!q.occupied) continue;
+                const double pos[3] = {(double)q.x, (double)q.y, (double)q.z};
+                double xx, yy, zz;
+                if (!project_d(pd.Rr, pd.tr, pos, pd.ref.w, pd.ref.h, pd.focal, &xx, &yy, &zz))
+                    continue;
+                sorted_insert_d<8>(dep, zz);
+                csum += (double)q.conf;
+                msum += (double)q.margin;
+                ++n;
+            }
+        bool pred = n >= kMinSurfaceNeighbors;
+        double mn = dep[0], mx = dep[0], depth = dep[0];
+#pragma unroll
+        for (int k = 1; k < 8; ++k) {
+            mx = k == n - 1 ? dep[k] : mx;
+            depth = k == (n >> 1) ? dep[k] : depth;  // quantile 0.5: index round((n - 1) / 2)
+        }
+        pred = pred && isfinite(mn) && isfinite(mx) && mn > 0.0 && mx > 0.0 &&
+               !((mx - mn) / mx > kMaxSurfaceNeighborSpread);
+        if (pred) {
+            c[1] = 1;  // a proposal
+            const uint32_t x = pd.border + (uint32_t)gx * pd.stride;
+            const uint32_t y = pd.border + (uint32_t)gy * pd.stride;
+            // patch_contrast at integer taps (dense_legacy.rs:1097-1111)
+            float rp[T];
+            {
+                const uint8_t *l = pd.ref.luma;
+                const uint32_t w = pd.ref.w;
+#pragma unroll
+                for (int dy = -R; dy <= R; ++dy)
+#pragma unroll
+                    for (int dx = -R; dx <= R; ++dx)
+                        rp[(dy + R) * P + (dx + R)] = (float)l[(size_t)(y + dy) * w + (x + dx)];
+            }
+            float lo = rp[0], hi = rp[0];
+#pragma unroll
+            for (int k = 1; k < T; ++k) {
+                lo = fminf(lo, rp[k]);
+                hi = fmaxf(hi, rp[k]);
+            }
+            if (!((hi - lo) >= pd.min_contrast)) {
+                c[2] = 1;
+            } else {
+                const RefStats st = ref_stats<COST, T>(rp);
+                float l[SMAX];
+#pragma unroll
+                for (int k = 0; k < SMAX; ++k) l[k] = kInfF;
+                for (int s = 0; s < pd.S; ++s) {
+                    const SourceDev &src = pd.src[s];
+                    double h[9];
+#pragma unroll
+                    for (int k = 0; k < 9; ++k) h[k] = fma(depth, src.A[k], src.B[k]);
+                    const TexRef<ST, false> tex(src.tex.quad, src.tex.w, src.tex.h);
+                    float cst = patch_cost<R, COST, ST, false>(rp, st, h, (double)x, (double)y, tex);
+                    sorted_insert<SMAX>(l, cst <= pd.max_cost ? cst : kInfF);
+                }
+                int cnt = 0;
+#pragma unroll
+                for (int k = 0; k < SMAX; ++k) cnt += l[k] < kInfF ? 1 : 0;
+                if (cnt == 0) {
+                    c[3] = 1;
+                } else {
+                    float med = l[0];
+#pragma unroll
+                    for (int k = 1; k < SMAX; ++k) med = k == (cnt >> 1) ? l[k] : med;
+                    Proposal q;
+                    q.site = j;
+                    q.direct_count = cnt;
+                    q.direct_error = med;
+                    q.nconf = csum / (double)n;
+                    q.nmargin = msum / (double)n;
+                    q.depth = depth;
+                    unproject_d(pd.Rr, pd.tr, (double)x, (double)y, depth, pd.ref.w, pd.ref.h,
+                                pd.focal, q.pos);
+                    q.proj_ok = project_d(pd.Rr, pd.tr, q.pos, pd.ref.w, pd.ref.h, pd.focal, &q.xr,
+                                          &q.yr, &q.dr)
+                                    ? 1
+                                    : 0;
+                    a.prop_site[pd.base + j] = q;
+                    flag = 1;
+                    c[0] = 1;
+                }
+            }
+        }
+    }
+    if (j < pd.cap) a.prop_flag[pd.base + j] = flag;
+    block_count(c, a.block_counts, pd.block0 + blk);
+}
+
+// Proposals -> a row-major list per problem, with the Decision records the reciprocal kernel
+// reads (block offsets of stage stage_propose(pass)).
+__global__ __launch_bounds__(kBlock) void k_compact_props(Args a) {
+    const uint32_t b = blockIdx.x;
+    const ProblemDev &pd = a.prob[a.block_problem[b]];
+    const uint32_t nsites = pd.grid_nx * pd.grid_ny;
+    const uint32_t j = (b - pd.block0) * kBlock + threadIdx.x;
+    const bool flag = j < nsites && a.prop_flag[pd.base + j] != 0;
+    const uint32_t r = block_rank(flag);
+    if (!flag) return;
+    const uint32_t k = a.block_offsets[b * kCounters] + r;
+    const Proposal q = a.prop_site[pd.base + j];
+    a.prop[pd.base + k] = q;
+    Decision d{};
+    d.status = 0;
+    d.depth = q.depth;
+    d.pos[0] = q.pos[0];
+    d.pos[1] = q.pos[1];
+    d.pos[2] = q.pos[2];
+    d.xr = q.xr;
+    d.yr = q.yr;
+    d.dr = q.dr;
+    d.proj_ok = q.proj_ok;
+    a.decision[pd.base + k] = d;
+}
+
+// Reciprocal outcome of each proposal -> fusion, footprint test, confidence
+// (dense_legacy.rs:638-719); counters of stage stage_complete(pass). A completed site's point
+// is written back for the next pass unless the pass is the one being dropped.
+__global__ __launch_bounds__(kBlock) void k_complete_fuse(Args a, int pass) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t p = a.block_problem[b];
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t i = (b - pd.block0) * kBlock + threadIdx.x;
+    uint32_t c[kCounters] = {0, 0, 0, 0, 0, 0, 0, 0};
+    if (i < a.n_props[(size_t)pass * a.n_prob + p]) {
+        const Proposal q = a.prop[pd.base + i];
+        SampleOut o{};
+        o.status = VT3D_SAMPLE_REJECT_RECIPROCAL;
+        const double radius = fmax(fabs(q.depth) * pd.fusion_rel, 1.0e-4);
+        double sx = q.pos[0], sy = q.pos[1], sz = q.pos[2];
+        uint32_t obs = 1, cnt = 0;
+        for (int s = 0; s < pd.S; ++s) {
+            const Recip r = a.recip[pd.recip_base + (size_t)s * pd.cap + i];
+            if (!r.ok) continue;
+            ++cnt;
+            const double ex = r.pos[0] - q.pos[0], ey = r.pos[1] - q.pos[1],
+                         ez = r.pos[2] - q.pos[2];
+            if (!(sqrt(ex * ex + ey * ey + ez * ez) <= radius)) continue;
+            sx += r.pos[0];
+            sy += r.pos[1];
+            sz += r.pos[2];
+            ++obs;
+        }
+        if (cnt == 0) {
+            c[1] = 1;
+        } else if (obs < (uint32_t)pd.min_obs) {
+            c[2] = 1;
+            o.status = VT3D_SAMPLE_REJECT_FUSION;
+        } else {
+            const double fp[3] = {sx / (double)obs, sy / (double)obs, sz / (double)obs};
+            double xx, yy, zz;
+            const uint32_t x = pd.border + (q.site % pd.grid_nx) * pd.stride;
+            const uint32_t y = pd.border + (q.site / pd.grid_nx) * pd.stride;
+            const double lim = (double)pd.stride * 0.5;
+            if (!project_d(pd.Rr, pd.tr, fp, pd.ref.w, pd.ref.h, pd.focal, &xx, &yy, &zz) ||
+                fabs(xx - (double)x) >= lim || fabs(yy - (double)y) >= lim) {
+                c[3] = 1;
+                o.status = VT3D_SAMPLE_REJECT_FUSION;  // footprint
+            } else if (pass != a.comp_discard_pass) {
+                const double S = (double)pd.S;
+                const double support_conf = (double)q.direct_count / S;
+                const double recip_conf = fmin(fmax((double)(obs - 1) / S, 0.0), 1.0);
+                const double err_conf = 1.0 / (1.0 + (double)q.direct_error / pd.err_scale);
+                const double conf = fmin(fmax(q.nconf * support_conf * recip_conf * err_conf *
+                                                  kSurfaceConfidenceScale,
+                                              0.05),
+                                         0.9);
+                o.status = VT3D_SAMPLE_ACCEPTED;
+                o.conf = (float)conf;
+                o.recip_conf = (float)recip_conf;
+                o.depth_margin = (float)fmin(fmax(q.nmargin, 0.0), 1.0);
+                o.px = (float)fp[0];
+                o.py = (float)fp[1];
+                o.pz = (float)fp[2];
+                o.error = q.direct_error;
+                o.support = q.direct_count;
+                c[0] = 1;
+                a.site_pt[pd.base + q.site] =
+                    SitePoint{o.px, o.py, o.pz, o.conf, o.depth_margin, 1};
+            }
+        }
+        a.out[pd.base + i] = o;
+    }
+    block_count(c, a.block_counts, b);
+}
+
+// Completed points -> the pass's packed outputs in row-major order (problem p's start at
+// comp_offset[pass][p]; block offsets of stage stage_complete(pass)).
+__global__ __launch_bounds__(kBlock) void k_compact_additions(Args a, Outputs o, int pass) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t p = a.block_problem[b];
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t i = (b - pd.block0) * kBlock + threadIdx.x;
+    const bool flag = i < a.n_props[(size_t)pass * a.n_prob + p] &&
+                      a.out[pd.base + i].status == VT3D_SAMPLE_ACCEPTED;
+    const uint32_t r = block_rank(flag);
+    if (!flag) return;
+    const uint32_t dst = a.comp_offset[(size_t)pass * (a.n_prob + 1) + p] +
+                         a.block_offsets[b * kCounters] + r;
+    const SampleOut s = a.out[pd.base + i];
+    const uint32_t site = a.prop[pd.base + i].site;
+    const uint32_t x = pd.border + (site % pd.grid_nx) * pd.stride;
+    const uint32_t y = pd.border + (site / pd.grid_nx) * pd.stride;
+    const uint8_t *px = pd.ref_rgba + ((size_t)y * pd.ref.w + x) * 4;
+    vt3d_point3 pt;
+    pt.x = s.px;
+    pt.y = s.py;
+    pt.z = s.pz;
+    pt.confidence = s.conf;
+    pt.r = px[0];
+    pt.g = px[1];
+    pt.b = px[2];
+    pt.pad = 0;
+    o.points[dst] = pt;
+    o.sites[dst] = vt3d_grid_site{x, y};
+    o.attrs[dst] = vt3d_point_attributes{s.recip_conf, s.depth_margin};
+    o.errors[dst] = s.error;
+    o.supports[dst] = (float)s.support;
 }
 
 // ---- launchers ------------------------------------------------------------------------------
@@ -1242,12 +1550,23 @@ void launch_recip_t(const Args &a, hipStream_t s) {
                        s, a);
 }
 
+template <int R, int COST, int ST>
+void launch_propose_t(const Args &a, int smax, int pass, hipStream_t s) {
+    const dim3 grid(xcd_grid(a.group_items));
+    if (smax <= 4) hipLaunchKernelGGL((k_propose<R, COST, ST, 4>), grid, dim3(kBlock), 0, s, a, pass);
+    else if (smax <= 8) hipLaunchKernelGGL((k_propose<R, COST, ST, 8>), grid, dim3(kBlock), 0, s, a, pass);
+    else if (smax <= 16) hipLaunchKernelGGL((k_propose<R, COST, ST, 16>), grid, dim3(kBlock), 0, s, a, pass);
+    else hipLaunchKernelGGL((k_propose<R, COST, ST, 32>), grid, dim3(kBlock), 0, s, a, pass);
+}
+
+// kind: 0 primary sweep, 1 reciprocal kernel, 2 surface-completion proposals.
 template <int COST, int ST>
-bool dispatch_r(int radius, const Args &a, int smax, bool recip, bool tiled, size_t lds,
+bool dispatch_r(int radius, const Args &a, int smax, int kind, bool tiled, size_t lds, int pass,
                 hipStream_t s) {
 #define VT3D_CASE(RR)                                                           \
     case RR:                                                                    \
-        if (recip) launch_recip_t<RR, COST, ST>(a, s);                          \
+        if (kind == 1) launch_recip_t<RR, COST, ST>(a, s);                      \
+        else if (kind == 2) launch_propose_t<RR, COST, ST>(a, smax, pass, s);   \
         else launch_sweep_t<RR, COST, ST>(a, smax, tiled, lds, s);              \
         return true;
     if constexpr (COST == VT3D_COST_ZSAD) {
@@ -1271,24 +1590,24 @@ bool dispatch_r(int radius, const Args &a, int smax, bool recip, bool tiled, siz
 }
 
 template <int ST>
-bool dispatch_cost(int cost, int radius, const Args &a, int smax, bool recip, bool tiled,
-                   size_t lds, hipStream_t s) {
+bool dispatch_cost(int cost, int radius, const Args &a, int smax, int kind, bool tiled,
+                   size_t lds, int pass, hipStream_t s) {
     if (cost == VT3D_COST_ZSAD)
-        return dispatch_r<VT3D_COST_ZSAD, ST>(radius, a, smax, recip, tiled, lds, s);
-    return dispatch_r<VT3D_COST_ZNCC, ST>(radius, a, smax, recip, tiled, lds, s);
+        return dispatch_r<VT3D_COST_ZSAD, ST>(radius, a, smax, kind, tiled, lds, pass, s);
+    return dispatch_r<VT3D_COST_ZNCC, ST>(radius, a, smax, kind, tiled, lds, pass, s);
 }
 
-// One launch of the primary sweep (recip == false) or reciprocal kernel over the problem
-// group a.group_p0 .. a.group_p1.
-bool launch_group(int storage, int cost, int radius, int smax, bool recip, bool tiled,
-                  size_t lds, const Args &a, hipStream_t s) {
+// One launch of the primary sweep (kind 0), the reciprocal kernel (1) or the completion
+// proposals of pass `pass` (2) over the problem group a.group_p0 .. a.group_p1.
+bool launch_group(int storage, int cost, int radius, int smax, int kind, bool tiled,
+                  size_t lds, int pass, const Args &a, hipStream_t s) {
     switch (storage) {
     case VT3D_STORAGE_F32:
-        return dispatch_cost<VT3D_STORAGE_F32>(cost, radius, a, smax, recip, tiled, lds, s);
+        return dispatch_cost<VT3D_STORAGE_F32>(cost, radius, a, smax, kind, tiled, lds, pass, s);
     case VT3D_STORAGE_F16:
-        return dispatch_cost<VT3D_STORAGE_F16>(cost, radius, a, smax, recip, tiled, lds, s);
+        return dispatch_cost<VT3D_STORAGE_F16>(cost, radius, a, smax, kind, tiled, lds, pass, s);
     case VT3D_STORAGE_BF16:
-        return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, recip, tiled, lds, s);
+        return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, kind, tiled, lds, pass, s);
     default: return false;
     }
 }
@@ -1317,8 +1636,8 @@ size_t tile_lds_bytes(int tile_w, int stride, int radius) {
 void launch_gate(const Args &a, hipStream_t s) {
     hipLaunchKernelGGL(k_gate, dim3(a.n_blocks), dim3(kBlock), 0, s, a);
 }
-void launch_scan(const Args &a, int gate, hipStream_t s) {
-    hipLaunchKernelGGL(k_scan, dim3(a.n_prob), dim3(1024), 0, s, a, gate);
+void launch_scan(const Args &a, int stage, uint32_t *count_out, hipStream_t s) {
+    hipLaunchKernelGGL(k_scan, dim3(a.n_prob), dim3(1024), 0, s, a, stage, count_out);
 }
 void launch_problem_offsets(const Args &a, int which, hipStream_t s) {
     hipLaunchKernelGGL(k_problem_offsets, dim3(1), dim3(1024), 0, s, a, which);
@@ -1334,6 +1653,18 @@ void launch_fuse(const Args &a, vt3d_sample_diag *diag, hipStream_t s) {
 }
 void launch_compact_points(const Args &a, const Outputs &o, hipStream_t s) {
     hipLaunchKernelGGL(k_compact_points, dim3(a.n_blocks), dim3(kBlock), 0, s, a, o);
+}
+void launch_occ_init(const Args &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_occ_init, dim3(a.n_blocks), dim3(kBlock), 0, s, a);
+}
+void launch_compact_props(const Args &a, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_props, dim3(a.n_blocks), dim3(kBlock), 0, s, a);
+}
+void launch_complete_fuse(const Args &a, int pass, hipStream_t s) {
+    hipLaunchKernelGGL(k_complete_fuse, dim3(a.n_blocks), dim3(kBlock), 0, s, a, pass);
+}
+void launch_compact_additions(const Args &a, const Outputs &o, int pass, hipStream_t s) {
+    hipLaunchKernelGGL(k_compact_additions, dim3(a.n_blocks), dim3(kBlock), 0, s, a, o, pass);
 }
 
 }  // namespace vt3d
