@@ -263,6 +263,86 @@ int32_t vt3d_dense_reference_patch(const vt3d_frame *frames, size_t n_frames,
 
 void vt3d_dense_result_free(vt3d_dense_result *result);
 
+/* ---- multi-reference dense estimate --------------------------------------------------- */
+
+#define VT3D_MAX_REFERENCE_VIEWS 3     /* MAX_REFERENCE_VIEWS, dense.rs:8 */
+#define VT3D_MAX_REFERENCE_ATTEMPTS 6  /* MAX_REFERENCE_ATTEMPTS, dense.rs:9 */
+#define VT3D_MIN_REFERENCE_PATCH_POINTS 3  /* dense.rs:10 */
+
+/* DenseReferenceAttemptStats, dense.rs:63-81 (skip_reason "" when accepted). */
+typedef struct vt3d_reference_attempt {
+    uint64_t reference_frame;
+    int32_t attempted;
+    int32_t accepted;
+    char skip_reason[256];
+    uint64_t sampled_pixels;
+    uint64_t accepted_points;
+    uint64_t reciprocal_checked_points;
+    uint64_t reciprocal_rejected_points;
+    uint64_t reciprocal_consistent_points;
+    uint64_t surface_completion_proposals;
+    uint64_t surface_completed_points;
+    uint64_t surface_completion_rejected_texture;
+    uint64_t surface_completion_rejected_cross_view;
+    uint64_t surface_completion_rejected_reciprocal;
+    uint64_t surface_completion_rejected_fusion;
+    uint64_t surface_completion_rejected_footprint;
+} vt3d_reference_attempt;
+
+/* DenseReferencePatchStats, dense.rs:83-98. */
+typedef struct vt3d_reference_patch_stats {
+    uint64_t reference_frame;
+    uint64_t n_source_frames;
+    uint64_t source_frames[32];
+    uint64_t primary_start;
+    uint64_t primary_points;
+    uint64_t completion_start;
+    uint64_t completed_points;
+    uint64_t sampled_pixels;
+    uint64_t accepted_points;
+    uint64_t reciprocal_consistent_points;
+    uint64_t grid_stride;
+    uint32_t grid_border;
+    int32_t has_search_depth;
+    float search_min_depth;
+    float search_max_depth;
+} vt3d_reference_patch_stats;
+
+/* The multi-reference DenseAnalysis of dense.rs (DenseStats, dense.rs:100-146, without the
+ * working-set fields). `combined` holds the merged stats and points: every patch's primary
+ * points in patch order, then every patch's completed points (combine_patches,
+ * dense.rs:842-995); combined.stats.source_frames lists the participating frames other than
+ * the primary reference; the medians and search depths are set only for a single patch. */
+typedef struct vt3d_multi_result {
+    vt3d_dense_result combined;
+    size_t n_reference_frames;
+    uint64_t reference_frames[VT3D_MAX_REFERENCE_VIEWS];
+    size_t n_attempts;
+    vt3d_reference_attempt attempts[VT3D_MAX_REFERENCE_ATTEMPTS + 1];
+    size_t n_patches;
+    vt3d_reference_patch_stats patches[VT3D_MAX_REFERENCE_VIEWS];
+    int32_t omitted_references; /* the view limit stopped the run while candidates remained */
+    int32_t fine_grid;          /* the grid spacing that ran */
+} vt3d_multi_result;
+
+/* Replaces estimate_depth_points_limited (dense.rs:244-330): reference candidates in the
+ * order of reference_candidate_order (dense.rs:620-672), up to 6 attempts through the
+ * reference remap, keeping the first `reference_views` (1..3) patches with at least 3
+ * points, the legacy single-reference sweep as the fallback, and the patches combined.
+ * fine_grid < 0 decides the grid as estimate_depth_points_with_budget_and_control does
+ * (dense.rs:366-372, without the working-set budget): finer only where the stride rule gives
+ * frames[0] a smaller stride and has_distributed_landmarks (dense.rs:161-213) holds. With a
+ * poll function the candidates run one at a time with the reference's polling; without one,
+ * in batches of as many candidates as patches are still wanted (each patch depends only on
+ * its own inputs, so the kept patches are those of the sequential run). */
+int32_t vt3d_dense_estimate(const vt3d_frame *frames, size_t n_frames,
+                            const vt3d_camera *cameras, size_t n_cameras,
+                            const double *sparse_xyz, size_t n_sparse, double focal,
+                            int32_t reference_views, int32_t fine_grid,
+                            const vt3d_sweep_params *params, vt3d_poll_fn poll, void *poll_user,
+                            vt3d_multi_result **out);
+void vt3d_multi_result_free(vt3d_multi_result *result);
+
 /* ---- device-resident sessions (frames uploaded once, solved many times) -------------- */
 /* No counterpart in the reference: the reference re-reads host frames every call. A
  * session keeps luma textures in HBM so repeated solves (benchmarks, batching, serving)

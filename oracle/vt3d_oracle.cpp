@@ -14,6 +14,8 @@
 #include <cstring>
 #include <limits>
 #include <map>
+#include <set>
+#include <string>
 #include <vector>
 
 #ifdef _OPENMP
@@ -1187,14 +1189,16 @@ int32_t vt3d_oracle_plane_sweep(const vt3d_frame *frames, size_t n_frames,
     return *out ? VT3D_OK : VT3D_E_INVALID;
 }
 
-int32_t vt3d_oracle_reference_patch(const vt3d_frame *frames, size_t n_frames,
-                                    const vt3d_camera *cameras, size_t n_cameras,
-                                    const double *sparse_xyz, size_t n_sparse, double focal,
-                                    size_t ref_idx, int32_t fine_grid,
-                                    const vt3d_sweep_params *params, vt3d_poll_fn poll,
-                                    void *poll_user, int32_t threads, vt3d_dense_result **out,
-                                    vt3d_oracle_slack **slack_out) {
-    // reconstruct_reference_patch, dense.rs:674-747.
+}  // extern "C"
+
+namespace {
+
+// reconstruct_reference_patch, dense.rs:674-747, with the caller's control.
+int32_t reference_patch_ctl(const vt3d_frame *frames, size_t n_frames, const vt3d_camera *cameras,
+                            size_t n_cameras, const double *sparse_xyz, size_t n_sparse,
+                            double focal, size_t ref_idx, int32_t fine_grid,
+                            const vt3d_sweep_params *params, Control &ctl, int32_t threads,
+                            vt3d_dense_result **out, vt3d_oracle_slack **slack_out) {
     if (!out) return VT3D_E_INVALID;
     *out = nullptr;
     if (slack_out) *slack_out = nullptr;
@@ -1230,7 +1234,6 @@ int32_t vt3d_oracle_reference_patch(const vt3d_frame *frames, size_t n_frames,
     order.insert(order.end(), src_idx.begin(), src_idx.end());
     std::vector<vt3d_frame> rf;
     std::vector<vt3d_camera> rcams;
-    Control ctl{poll, poll_user};
     for (size_t k = 0; k < order.size(); ++k) {
         if (ctl.is_canceled()) return VT3D_NO_PATCH;
         rf.push_back(frames[cameras[order[k]].frame_index]);
@@ -1253,6 +1256,22 @@ int32_t vt3d_oracle_reference_patch(const vt3d_frame *frames, size_t n_frames,
         r->stats.source_frames[i] = cameras[order[r->stats.source_frames[i]]].frame_index;
     *out = r;
     return VT3D_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vt3d_oracle_reference_patch(const vt3d_frame *frames, size_t n_frames,
+                                    const vt3d_camera *cameras, size_t n_cameras,
+                                    const double *sparse_xyz, size_t n_sparse, double focal,
+                                    size_t ref_idx, int32_t fine_grid,
+                                    const vt3d_sweep_params *params, vt3d_poll_fn poll,
+                                    void *poll_user, int32_t threads, vt3d_dense_result **out,
+                                    vt3d_oracle_slack **slack_out) {
+    Control ctl{poll, poll_user};
+    return reference_patch_ctl(frames, n_frames, cameras, n_cameras, sparse_xyz, n_sparse, focal,
+                               ref_idx, fine_grid, params, ctl, threads, out, slack_out);
 }
 
 void vt3d_oracle_set_row_band(int32_t first, int32_t count) {
@@ -1374,6 +1393,400 @@ int32_t vt3d_oracle_surface_depth_prediction(double *depths, size_t n, double *p
     bool ok = surface_depth_prediction(d, predicted);
     std::copy(d.begin(), d.end(), depths);
     return ok ? 1 : 0;
+}
+
+}  // extern "C"
+
+// ---- multi-reference dense estimate (dense.rs:161-330, 620-672, 749-1082) --------------------
+
+namespace {
+
+constexpr size_t kMaxReferenceViews = 3;     // MAX_REFERENCE_VIEWS, dense.rs:8
+constexpr size_t kMaxReferenceAttempts = 6;  // MAX_REFERENCE_ATTEMPTS, dense.rs:9
+constexpr size_t kMinReferencePatchPoints = 3;  // MIN_REFERENCE_PATCH_POINTS, dense.rs:10
+
+// has_distributed_landmarks, dense.rs:161-213.
+bool distributed_landmarks(const vt3d_frame *frames, size_t n_frames, const vt3d_camera *cams,
+                           size_t n_cams, const double *xyz, size_t n, double f, Control &ctl) {
+    const size_t majority = n_cams / 2 + 1;
+    size_t covered_views = 0;
+    for (size_t ci = 0; ci < n_cams; ++ci) {
+        if (cams[ci].frame_index >= n_frames) continue;
+        const vt3d_frame &fr = frames[cams[ci].frame_index];
+        const Cam c = to_cam(cams[ci]);
+        unsigned tiles[4][4] = {};
+        size_t supported = 0;
+        bool done = false;
+        for (size_t chunk = 0; chunk < n && !done; chunk += 256) {
+            if (ctl.is_canceled()) return false;
+            const size_t end = std::min(n, chunk + 256);
+            for (size_t i = chunk; i < end; ++i) {
+                double x, y, z;
+                if (!project(c, {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}, fr.width, fr.height,
+                             f, &x, &y, &z))
+                    continue;
+                if (!(0.0 <= x && x < (double)fr.width) || !(0.0 <= y && y < (double)fr.height))
+                    continue;
+                // x < width, so 4x/width rounds to 4.0 only for x within an ulp of the edge
+                unsigned &t = tiles[std::min<size_t>(3, (size_t)(4.0 * y / (double)fr.height))]
+                                   [std::min<size_t>(3, (size_t)(4.0 * x / (double)fr.width))];
+                t = std::min(t + 1u, 255u);  // u8::saturating_add
+                if (t == 2) {
+                    ++supported;
+                    if (supported >= 8) {
+                        ++covered_views;
+                        if (covered_views == majority) return true;
+                        done = true;
+                        break;
+                    }
+                }
+            }
+        }
+    }
+    return false;
+}
+
+// reference_candidate_order, dense.rs:620-672.
+std::vector<size_t> candidate_order(const vt3d_frame *frames, size_t n_frames,
+                                    const vt3d_camera *cams, size_t n_cams, const double *xyz,
+                                    size_t n, double f) {
+    std::vector<std::pair<size_t, size_t>> cand;  // (camera index, visible landmarks)
+    for (size_t ci = 0; ci < n_cams; ++ci) {
+        if (cams[ci].frame_index >= n_frames) continue;
+        const vt3d_frame &fr = frames[cams[ci].frame_index];
+        const Cam c = to_cam(cams[ci]);
+        size_t visible = 0;
+        for (size_t i = 0; i < n; ++i) {
+            double x, y, z;
+            if (project(c, {xyz[3 * i], xyz[3 * i + 1], xyz[3 * i + 2]}, fr.width, fr.height, f,
+                        &x, &y, &z) &&
+                in_image_bounds(x, y, fr.width, fr.height))
+                ++visible;
+        }
+        if (visible >= kMinVisibleSparsePoints) cand.push_back({ci, visible});
+    }
+    std::stable_sort(cand.begin(), cand.end(), [&](const auto &l, const auto &r) {
+        if (l.second != r.second) return l.second > r.second;
+        return cams[l.first].frame_index < cams[r.first].frame_index;
+    });
+    if (cand.empty()) return {};
+    const V3 pc = camera_center(to_cam(cams[cand[0].first]));
+    auto d2 = [&](size_t ci) {
+        const V3 d = camera_center(to_cam(cams[ci])) - pc;
+        return d.x * d.x + d.y * d.y + d.z * d.z;
+    };
+    std::stable_sort(cand.begin() + 1, cand.end(), [&](const auto &l, const auto &r) {
+        const double dl = d2(l.first), dr = d2(r.first);
+        if (dl != dr) return dl > dr;
+        if (l.second != r.second) return l.second > r.second;
+        return cams[l.first].frame_index < cams[r.first].frame_index;
+    });
+    std::vector<size_t> out;
+    for (const auto &c : cand) out.push_back(c.first);
+    return out;
+}
+
+// reference_attempt_stats, dense.rs:749-785.
+vt3d_reference_attempt attempt_of(uint64_t frame, const vt3d_dense_result &r, bool accepted) {
+    vt3d_reference_attempt a{};
+    a.reference_frame = frame;
+    a.attempted = r.stats.attempted;
+    a.accepted = accepted;
+    if (!accepted) {
+        std::string reason;
+        if (r.skip_code) reason = r.skip_reason;
+        else if (!r.stats.attempted) reason = "dense estimation did not run for this reference";
+        else
+            reason = "reference produced " + std::to_string(r.n_points) +
+                     " accepted dense points; at least 3 are required for a surface patch";
+        std::strncpy(a.skip_reason, reason.c_str(), sizeof(a.skip_reason) - 1);
+    }
+    const vt3d_dense_stats &s = r.stats;
+    a.sampled_pixels = s.sampled_pixels;
+    a.accepted_points = r.n_points;
+    a.reciprocal_checked_points = s.reciprocal_checked_points;
+    a.reciprocal_rejected_points = s.reciprocal_rejected_points;
+    a.reciprocal_consistent_points = s.reciprocal_consistent_points;
+    a.surface_completion_proposals = s.surface_completion_proposals;
+    a.surface_completed_points = s.surface_completed_points;
+    a.surface_completion_rejected_texture = s.surface_completion_rejected_texture;
+    a.surface_completion_rejected_cross_view = s.surface_completion_rejected_cross_view;
+    a.surface_completion_rejected_reciprocal = s.surface_completion_rejected_reciprocal;
+    a.surface_completion_rejected_fusion = s.surface_completion_rejected_fusion;
+    a.surface_completion_rejected_footprint = s.surface_completion_rejected_footprint;
+    return a;
+}
+
+size_t completion_count(const vt3d_dense_result &r) {
+    return std::min<size_t>(r.stats.surface_completed_points, r.n_points);  // split_patch
+}
+
+vt3d_reference_patch_stats patch_of(const vt3d_dense_result &r, size_t primary_start,
+                                    size_t completion_start) {
+    vt3d_reference_patch_stats ps{};
+    const vt3d_dense_stats &s = r.stats;
+    ps.reference_frame = s.reference_frame;
+    ps.n_source_frames = std::min<uint64_t>(s.source_views, 32);
+    std::copy(s.source_frames, s.source_frames + ps.n_source_frames, ps.source_frames);
+    ps.primary_start = primary_start;
+    ps.primary_points = r.n_points - completion_count(r);
+    ps.completion_start = completion_start;
+    ps.completed_points = completion_count(r);
+    ps.sampled_pixels = s.sampled_pixels;
+    ps.accepted_points = r.n_points;
+    ps.reciprocal_consistent_points = s.reciprocal_consistent_points;
+    ps.grid_stride = s.grid_stride;
+    ps.grid_border = s.grid_border;
+    ps.has_search_depth = s.has_search_depth;
+    ps.search_min_depth = s.search_min_depth;
+    ps.search_max_depth = s.search_max_depth;
+    return ps;
+}
+
+// merge_fallback_attempts, dense.rs:787-804.
+void merge_attempts(std::vector<vt3d_reference_attempt> &attempts,
+                    const vt3d_reference_attempt *fallback, size_t nf) {
+    for (size_t k = 0; k < nf; ++k) {
+        const vt3d_reference_attempt &fa = fallback[k];
+        auto it = std::find_if(attempts.begin(), attempts.end(), [&](const auto &a) {
+            return a.reference_frame == fa.reference_frame;
+        });
+        if (it == attempts.end()) attempts.push_back(fa);
+        else if (fa.accepted && !it->accepted) *it = fa;
+    }
+}
+
+// from_legacy, dense.rs:997-1082 (takes r).
+void legacy_into(vt3d_dense_result *r, vt3d_multi_result *m) {
+    m->combined = *r;
+    std::free(r);
+    const vt3d_dense_result &c = m->combined;
+    if (!c.stats.has_reference_frame) return;
+    m->n_reference_frames = 1;
+    m->reference_frames[0] = c.stats.reference_frame;
+    m->n_attempts = 1;
+    m->attempts[0] = attempt_of(c.stats.reference_frame, c,
+                                c.stats.attempted && c.n_points >= kMinReferencePatchPoints);
+    if (c.stats.attempted) {
+        m->n_patches = 1;
+        m->patches[0] = patch_of(c, 0, c.n_points - completion_count(c));
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vt3d_oracle_estimate(const vt3d_frame *frames, size_t n_frames,
+                             const vt3d_camera *cameras, size_t n_cameras,
+                             const double *sparse_xyz, size_t n_sparse, double focal,
+                             int32_t reference_views, int32_t fine_grid,
+                             const vt3d_sweep_params *params, vt3d_poll_fn poll, void *poll_user,
+                             int32_t threads, vt3d_multi_result **out) {
+    // estimate_depth_points_limited, dense.rs:244-330 (estimate_depth_points decides the grid
+    // with has_distributed_landmarks when fine_grid < 0).
+    if (!out) return VT3D_E_INVALID;
+    *out = nullptr;
+    if (reference_views < 1 || reference_views > (int32_t)kMaxReferenceViews) return VT3D_E_INVALID;
+    vt3d_sweep_params def;
+    vt3d_oracle_params_reference(&def);
+    const vt3d_sweep_params &pp = params ? *params : def;
+    if (!valid_params(pp)) return VT3D_E_INVALID;
+    const Params p = to_params(pp);
+    Control ctl{poll, poll_user};
+    // estimate_depth_points_with_budget_and_control, dense.rs:366-372 (no working-set budget):
+    // finer only where the rule gives frame 0 a smaller stride, and then only with landmarks
+    // spread over the views.
+    const bool finer = n_frames > 0 && grid_stride_rule(frames[0].width, frames[0].height, true) <
+                                           grid_stride_rule(frames[0].width, frames[0].height,
+                                                            false);
+    const bool fine =
+        fine_grid < 0 ? finer && distributed_landmarks(frames, n_frames, cameras, n_cameras,
+                                                       sparse_xyz, n_sparse, focal, ctl)
+                      : fine_grid != 0;
+    vt3d_multi_result *m = (vt3d_multi_result *)std::calloc(1, sizeof(vt3d_multi_result));
+    m->fine_grid = fine;
+    auto legacy = [&]() {
+        return plane_sweep(frames, n_frames, cameras, n_cameras, sparse_xyz, n_sparse, focal, fine,
+                           p, ctl, threads, nullptr);
+    };
+    const std::vector<size_t> order =
+        candidate_order(frames, n_frames, cameras, n_cameras, sparse_xyz, n_sparse, focal);
+    if (order.empty()) {
+        legacy_into(legacy(), m);
+        *out = m;
+        return VT3D_OK;
+    }
+    std::vector<vt3d_dense_result *> patches;
+    std::vector<vt3d_reference_attempt> attempts;
+    const size_t n_try = std::min(order.size(), kMaxReferenceAttempts);
+    for (size_t k = 0; k < n_try; ++k) {
+        if (ctl.is_canceled()) {  // (DenseAnalysis::default(), false)
+            for (auto *q : patches) vt3d_oracle_result_free(q);
+            *m = vt3d_multi_result{};
+            m->fine_grid = fine;
+            *out = m;
+            return VT3D_OK;
+        }
+        vt3d_dense_result *r = nullptr;
+        const int32_t rc = reference_patch_ctl(frames, n_frames, cameras, n_cameras, sparse_xyz,
+                                               n_sparse, focal, order[k], fine, &pp, ctl, threads,
+                                               &r, nullptr);
+        if (rc == VT3D_NO_PATCH) continue;
+        if (rc) {
+            for (auto *q : patches) vt3d_oracle_result_free(q);
+            std::free(m);
+            return rc;
+        }
+        const bool accepted = r->stats.attempted && r->n_points >= kMinReferencePatchPoints;
+        attempts.push_back(attempt_of(cameras[order[k]].frame_index, *r, accepted));
+        if (!accepted) {
+            vt3d_oracle_result_free(r);
+            continue;
+        }
+        patches.push_back(r);
+        if (patches.size() >= (size_t)reference_views) {
+            m->omitted_references = k + 1 < n_try;
+            break;
+        }
+    }
+    if (patches.empty()) {
+        // the legacy fallback, its attempt merged into the remapped ones
+        vt3d_multi_result fb{};
+        legacy_into(legacy(), &fb);
+        merge_attempts(attempts, fb.attempts, fb.n_attempts);
+        *m = fb;
+        m->fine_grid = fine;
+        m->n_attempts = std::min(attempts.size(), kMaxReferenceAttempts + 1);
+        std::copy(attempts.begin(), attempts.begin() + m->n_attempts, m->attempts);
+        *out = m;
+        return VT3D_OK;
+    }
+    // combine_patches, dense.rs:842-995
+    vt3d_dense_result &c = m->combined;
+    std::vector<vt3d_point3> pts;
+    std::vector<vt3d_grid_site> sites;
+    std::vector<vt3d_point_attributes> attrs;
+    std::vector<size_t> pstart, cstart;
+    for (auto *q : patches) {
+        pstart.push_back(pts.size());
+        const size_t prim = q->n_points - completion_count(*q);
+        pts.insert(pts.end(), q->points, q->points + prim);
+        sites.insert(sites.end(), q->grid_sites, q->grid_sites + prim);
+        attrs.insert(attrs.end(), q->attributes, q->attributes + prim);
+    }
+    for (auto *q : patches) {
+        cstart.push_back(pts.size());
+        const size_t prim = q->n_points - completion_count(*q);
+        pts.insert(pts.end(), q->points + prim, q->points + q->n_points);
+        sites.insert(sites.end(), q->grid_sites + prim, q->grid_sites + q->n_points);
+        attrs.insert(attrs.end(), q->attributes + prim, q->attributes + q->n_points);
+    }
+    std::set<uint64_t> participating;
+    for (auto *q : patches) {
+        participating.insert(q->stats.reference_frame);
+        participating.insert(q->stats.source_frames,
+                             q->stats.source_frames + std::min<uint64_t>(q->stats.source_views, 32));
+    }
+    const vt3d_dense_stats &first = patches[0]->stats;
+    participating.erase(first.reference_frame);
+    vt3d_dense_stats &s = c.stats;
+    s.attempted = 1;
+    s.has_reference_frame = 1;
+    s.reference_frame = first.reference_frame;
+    s.source_views = participating.size();
+    size_t i = 0;
+    for (uint64_t fr : participating)
+        if (i < 32) s.source_frames[i++] = fr;
+    size_t total_completion = 0;
+    for (size_t k = 0; k < patches.size(); ++k) {
+        const vt3d_dense_stats &q = patches[k]->stats;
+        m->reference_frames[k] = q.reference_frame;
+        m->patches[k] = patch_of(*patches[k], pstart[k], cstart[k]);
+        total_completion += completion_count(*patches[k]);
+        s.sampled_pixels += q.sampled_pixels;
+        s.depth_hypotheses = std::max(s.depth_hypotheses, q.depth_hypotheses);
+        s.surface_completion_proposals += q.surface_completion_proposals;
+        s.surface_completion_rejected_texture += q.surface_completion_rejected_texture;
+        s.surface_completion_rejected_cross_view += q.surface_completion_rejected_cross_view;
+        s.surface_completion_rejected_reciprocal += q.surface_completion_rejected_reciprocal;
+        s.surface_completion_rejected_fusion += q.surface_completion_rejected_fusion;
+        s.surface_completion_rejected_footprint += q.surface_completion_rejected_footprint;
+        s.reciprocal_checked_points += q.reciprocal_checked_points;
+        s.reciprocal_rejected_points += q.reciprocal_rejected_points;
+        s.reciprocal_consistent_points += q.reciprocal_consistent_points;
+        s.fusion_input_observations += q.fusion_input_observations;
+        s.fusion_rejected_observations += q.fusion_rejected_observations;
+        s.fusion_rejected_points += q.fusion_rejected_points;
+    }
+    s.surface_completed_points = total_completion;
+    s.accepted_points = pts.size();
+    s.grid_stride = first.grid_stride;
+    s.grid_border = first.grid_border;
+    if (patches.size() == 1) {
+        s.has_median_fusion_observations = first.has_median_fusion_observations;
+        s.median_fusion_observations = first.median_fusion_observations;
+        s.has_median_supporting_views = first.has_median_supporting_views;
+        s.median_supporting_views = first.median_supporting_views;
+        s.has_median_photometric_error = first.has_median_photometric_error;
+        s.median_photometric_error = first.median_photometric_error;
+        s.has_search_depth = first.has_search_depth;
+        s.search_min_depth = first.search_min_depth;
+        s.search_max_depth = first.search_max_depth;
+    }
+    c.n_points = pts.size();
+    c.primary_points = cstart[0];
+    c.points = copy_out(pts);
+    c.grid_sites = copy_out(sites);
+    c.attributes = copy_out(attrs);
+    m->n_reference_frames = m->n_patches = patches.size();
+    m->n_attempts = std::min(attempts.size(), kMaxReferenceAttempts + 1);
+    std::copy(attempts.begin(), attempts.begin() + m->n_attempts, m->attempts);
+    for (auto *q : patches) vt3d_oracle_result_free(q);
+    *out = m;
+    return VT3D_OK;
+}
+
+void vt3d_oracle_multi_result_free(vt3d_multi_result *m) {
+    if (!m) return;
+    std::free(m->combined.points);
+    std::free(m->combined.grid_sites);
+    std::free(m->combined.attributes);
+    std::free(m->combined.samples);
+    std::free(m);
+}
+
+int32_t vt3d_oracle_distributed_landmarks(const vt3d_frame *frames, size_t n_frames,
+                                          const vt3d_camera *cameras, size_t n_cameras,
+                                          const double *sparse_xyz, size_t n_sparse,
+                                          double focal, vt3d_poll_fn poll, void *poll_user) {
+    Control ctl{poll, poll_user};
+    return distributed_landmarks(frames, n_frames, cameras, n_cameras, sparse_xyz, n_sparse,
+                                 focal, ctl);
+}
+
+size_t vt3d_oracle_reference_order(const vt3d_frame *frames, size_t n_frames,
+                                   const vt3d_camera *cameras, size_t n_cameras,
+                                   const double *sparse_xyz, size_t n_sparse, double focal,
+                                   size_t *order) {
+    const std::vector<size_t> o =
+        candidate_order(frames, n_frames, cameras, n_cameras, sparse_xyz, n_sparse, focal);
+    std::copy(o.begin(), o.end(), order);
+    return o.size();
+}
+
+void vt3d_oracle_reference_attempt(uint64_t reference_frame, const vt3d_dense_result *r,
+                                   int32_t accepted, vt3d_reference_attempt *out) {
+    *out = attempt_of(reference_frame, *r, accepted != 0);
+}
+
+size_t vt3d_oracle_merge_attempts(const vt3d_reference_attempt *attempts, size_t n,
+                                  const vt3d_reference_attempt *fallback, size_t n_fallback,
+                                  vt3d_reference_attempt *out) {
+    std::vector<vt3d_reference_attempt> v(attempts, attempts + n);
+    merge_attempts(v, fallback, n_fallback);
+    std::copy(v.begin(), v.end(), out);
+    return v.size();
 }
 
 }  // extern "C"

@@ -51,6 +51,16 @@ int32_t vt3d_oracle_reference_patch(const vt3d_frame *frames, size_t n_frames,
                                     void *poll_user, int32_t threads, vt3d_dense_result **out,
                                     vt3d_oracle_slack **slack_out);
 
+/* Same contract as vt3d_dense_estimate (dense.rs:244-330): up to reference_views reference
+ * patches, tried sequentially with one shared poll. Free with vt3d_oracle_multi_result_free. */
+int32_t vt3d_oracle_estimate(const vt3d_frame *frames, size_t n_frames,
+                             const vt3d_camera *cameras, size_t n_cameras,
+                             const double *sparse_xyz, size_t n_sparse, double focal,
+                             int32_t reference_views, int32_t fine_grid,
+                             const vt3d_sweep_params *params, vt3d_poll_fn poll, void *poll_user,
+                             int32_t threads, vt3d_multi_result **out);
+void vt3d_oracle_multi_result_free(vt3d_multi_result *result);
+
 /* Restricts plane_sweep to grid rows [first, first + count) (count 0: all rows), for
  * bounded timing samples. Process-wide; not thread-safe. */
 void vt3d_oracle_set_row_band(int32_t first, int32_t count);
@@ -116,6 +126,28 @@ void vt3d_oracle_fuse(const double *primary, const double *reciprocal, size_t n,
 /* surface_depth_prediction (dense_legacy.rs:739-753): 1 + depth or 0. depths is sorted
  * in place. */
 int32_t vt3d_oracle_surface_depth_prediction(double *depths, size_t n, double *predicted);
+
+/* has_distributed_landmarks (dense.rs:161-213): 1 or 0; polls once per 256 landmarks of
+ * each registered view. */
+int32_t vt3d_oracle_distributed_landmarks(const vt3d_frame *frames, size_t n_frames,
+                                          const vt3d_camera *cameras, size_t n_cameras,
+                                          const double *sparse_xyz, size_t n_sparse,
+                                          double focal, vt3d_poll_fn poll, void *poll_user);
+/* reference_candidate_order (dense.rs:620-672): writes camera indices to order (capacity
+ * n_cameras), returns the count. */
+size_t vt3d_oracle_reference_order(const vt3d_frame *frames, size_t n_frames,
+                                   const vt3d_camera *cameras, size_t n_cameras,
+                                   const double *sparse_xyz, size_t n_sparse, double focal,
+                                   size_t *order);
+
+/* reference_attempt_stats (dense.rs:749-785), accepted_points = result->n_points. */
+void vt3d_oracle_reference_attempt(uint64_t reference_frame, const vt3d_dense_result *result,
+                                   int32_t accepted, vt3d_reference_attempt *out);
+/* merge_fallback_attempts (dense.rs:787-804): out has room for n + n_fallback; returns the
+ * merged count. */
+size_t vt3d_oracle_merge_attempts(const vt3d_reference_attempt *attempts, size_t n,
+                                  const vt3d_reference_attempt *fallback, size_t n_fallback,
+                                  vt3d_reference_attempt *out);
 
 #ifdef __cplusplus
 }

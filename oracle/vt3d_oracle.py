@@ -54,6 +54,23 @@ def library() -> C.CDLL:
                                                          C.c_void_p, C.c_int32,
                                                          P(P(abi.DenseResult)), P(C.c_void_p)]
     lib.vt3d_oracle_reference_patch.restype = C.c_int32
+    lib.vt3d_oracle_estimate.argtypes = common + [C.c_int32, C.c_int32, P(abi.SweepParams),
+                                                  abi.POLL_FN, C.c_void_p, C.c_int32,
+                                                  P(P(abi.MultiResult))]
+    lib.vt3d_oracle_estimate.restype = C.c_int32
+    lib.vt3d_oracle_multi_result_free.argtypes = [P(abi.MultiResult)]
+    lib.vt3d_oracle_multi_result_free.restype = None
+    lib.vt3d_oracle_distributed_landmarks.argtypes = common + [abi.POLL_FN, C.c_void_p]
+    lib.vt3d_oracle_distributed_landmarks.restype = C.c_int32
+    lib.vt3d_oracle_reference_order.argtypes = common + [P(C.c_size_t)]
+    lib.vt3d_oracle_reference_order.restype = C.c_size_t
+    lib.vt3d_oracle_reference_attempt.argtypes = [C.c_uint64, P(abi.DenseResult), C.c_int32,
+                                                   P(abi.ReferenceAttempt)]
+    lib.vt3d_oracle_reference_attempt.restype = None
+    lib.vt3d_oracle_merge_attempts.argtypes = [P(abi.ReferenceAttempt), C.c_size_t,
+                                               P(abi.ReferenceAttempt), C.c_size_t,
+                                               P(abi.ReferenceAttempt)]
+    lib.vt3d_oracle_merge_attempts.restype = C.c_size_t
     lib.vt3d_oracle_set_row_band.argtypes = [C.c_int32, C.c_int32]
     lib.vt3d_oracle_set_row_band.restype = None
     lib.vt3d_oracle_result_free.argtypes = [P(abi.DenseResult)]
@@ -169,6 +186,46 @@ def reference_patch(frames, cameras, sparse_points, focal: float, reference_came
     if rc < 0:
         raise ValueError(f"oracle rejected the input ({rc})")
     return _take(out, slack.value)
+
+
+def estimate(frames, cameras, sparse_points, focal: float, reference_views: int = 3,
+             fine_grid: Optional[bool] = None, control: Optional[Callable[[], bool]] = None,
+             sweep: Optional[abi.SweepParams] = None, threads: int = 1) -> abi.MultiAnalysis:
+    """Oracle of estimate_depth_points_limited (fine_grid None: decided as the budgeted entry
+    decides it, without a working-set budget)."""
+    lib = library()
+    inp = abi.Inputs(_rgba(frames), cameras, sparse_points)
+    out = C.POINTER(abi.MultiResult)()
+    fine = -1 if fine_grid is None else int(bool(fine_grid))
+    rc = lib.vt3d_oracle_estimate(inp.frames, inp.n_frames, inp.cameras, inp.n_cameras,
+                                  inp.sparse_ptr, inp.n_sparse, float(focal),
+                                  int(reference_views), fine,
+                                  C.byref(sweep) if sweep is not None else None, _poll(control),
+                                  None, int(threads), C.byref(out))
+    if rc < 0:
+        raise ValueError(f"oracle rejected the input ({rc})")
+    m = abi.unpack_multi(out.contents)
+    lib.vt3d_oracle_multi_result_free(out)
+    return m
+
+
+def distributed_landmarks(frames, cameras, sparse_points, focal: float,
+                          control: Optional[Callable[[], bool]] = None) -> bool:
+    """Oracle of has_distributed_landmarks (frames may be (H, W) arrays; only sizes count)."""
+    inp = abi.Inputs(_rgba(frames), cameras, sparse_points)
+    return bool(library().vt3d_oracle_distributed_landmarks(
+        inp.frames, inp.n_frames, inp.cameras, inp.n_cameras, inp.sparse_ptr, inp.n_sparse,
+        float(focal), _poll(control), None))
+
+
+def reference_order(frames, cameras, sparse_points, focal: float):
+    """Oracle of reference_candidate_order: camera indices."""
+    inp = abi.Inputs(_rgba(frames), cameras, sparse_points)
+    buf = (C.c_size_t * max(1, inp.n_cameras))()
+    n = library().vt3d_oracle_reference_order(inp.frames, inp.n_frames, inp.cameras,
+                                              inp.n_cameras, inp.sparse_ptr, inp.n_sparse,
+                                              float(focal), buf)
+    return [int(buf[i]) for i in range(n)]
 
 
 def camera_struct(R, t, frame_index: int = 0) -> abi.Camera:

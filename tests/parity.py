@@ -89,3 +89,36 @@ def stats_equal(a: dict, b: dict, keys=None):
                     "skip_reason"]
     for k in keys:
         assert a[k] == b[k], (k, a[k], b[k])
+
+
+COMPLETION_KEYS = ("accepted_points", "surface_completion_proposals", "surface_completed_points",
+                   "surface_completion_rejected_texture",
+                   "surface_completion_rejected_cross_view",
+                   "surface_completion_rejected_reciprocal",
+                   "surface_completion_rejected_fusion",
+                   "surface_completion_rejected_footprint")
+
+
+def compare_completed(g, o, what):
+    """Counters, sites and order exactly; points and attributes within FP32 error of the
+    oracle. For fixtures without near ties (surface completion appends points in a fixed
+    order, so the arrays are compared element by element)."""
+    for k in COMPLETION_KEYS:
+        assert g.stats[k] == o.stats[k], (what, k, g.stats[k], o.stats[k])
+    assert g.primary_points == o.primary_points, what
+    assert np.array_equal(g.grid_sites, o.grid_sites), what
+    gp = np.stack([g.points["x"], g.points["y"], g.points["z"]], 1).astype(np.float64)
+    op = np.stack([o.points["x"], o.points["y"], o.points["z"]], 1).astype(np.float64)
+    if len(gp):
+        rel = np.linalg.norm(gp - op, axis=1) / np.maximum(np.linalg.norm(op, axis=1), 1e-12)
+        assert rel.max() <= 1e-4, (what, rel.max())
+        assert np.abs(g.points["confidence"] - o.points["confidence"]).max() <= 1e-4, what
+        assert np.array_equal(g.points[["r", "g", "b"]], o.points[["r", "g", "b"]]), what
+        # depth_margin is (second - best cost) / saturation: FP32 costs (|error| ~1e-5)
+        # over the ZNCC saturation 0.15
+        for f in ("reciprocal_consistency", "depth_margin"):
+            assert np.abs(g.point_attributes[f] - o.point_attributes[f]).max() <= 1e-4, (what, f)
+    for k in ("median_supporting_views", "median_fusion_observations"):
+        assert g.stats[k] == o.stats[k], (what, k)
+    ge, oe = g.stats["median_photometric_error"], o.stats["median_photometric_error"]
+    assert (ge is None) == (oe is None) and (ge is None or abs(ge - oe) <= 1e-3), (what, ge, oe)

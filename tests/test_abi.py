@@ -46,7 +46,10 @@ STRUCTS = {
     "vt3d_point3": abi.Point3, "vt3d_grid_site": abi.GridSite,
     "vt3d_point_attributes": abi.PointAttributes, "vt3d_dense_stats": abi.DenseStats,
     "vt3d_sample_diag": abi.SampleDiag, "vt3d_dense_result": abi.DenseResult,
-    "vt3d_timing": abi.Timing,
+    "vt3d_timing": abi.Timing, "vt3d_tuning": abi.Tuning, "vt3d_request": abi.Request,
+    "vt3d_server_config": abi.ServerConfig, "vt3d_completion": abi.Completion,
+    "vt3d_server_stats": abi.ServerStats, "vt3d_reference_attempt": abi.ReferenceAttempt,
+    "vt3d_reference_patch_stats": abi.ReferencePatchStats, "vt3d_multi_result": abi.MultiResult,
 }
 
 

@@ -1417,10 +1417,11 @@ int resolve_params(const vt3d_sweep_params *params, vt3d_sweep_params *resolved)
 }
 
 // Solves one request as a batch of one on the session's workspace.
-int solve_one(vt3d_session *s, FrameSource &fs, const vt3d_camera *cameras, size_t n_cameras,
+int solve_ctl(vt3d_session *s, FrameSource &fs, const vt3d_camera *cameras, size_t n_cameras,
               const double *sparse_xyz, size_t n_sparse, double focal, int64_t ref_idx,
-              bool fine_grid, const vt3d_sweep_params &p, vt3d_poll_fn poll, void *poll_user,
+              bool fine_grid, const vt3d_sweep_params &p, Control &ctl,
               vt3d_dense_result **out) {
+    const vt3d_poll_fn poll = ctl.poll;
     VT3D_HIP(hipSetDevice(s->device));
     vt3d_batch b;
     b.session = s;
@@ -1428,7 +1429,6 @@ int solve_one(vt3d_session *s, FrameSource &fs, const vt3d_camera *cameras, size
     b.stream = s->stream;
     b.prepared.emplace_back(new Prepared());
     Prepared &pr = *b.prepared.back();
-    Control ctl{poll, poll_user};
     int rc = ref_idx >= 0
                  ? prepare_reference_patch(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal,
                                            (size_t)ref_idx, fine_grid, p, ctl, pr)
@@ -1490,6 +1490,16 @@ int solve_one(vt3d_session *s, FrameSource &fs, const vt3d_camera *cameras, size
     s->timing = b.timing;
     *out = res;
     return VT3D_OK;
+}
+
+// Solves one request as a batch of one, polling `poll` as the reference polls its control.
+int solve_one(vt3d_session *s, FrameSource &fs, const vt3d_camera *cameras, size_t n_cameras,
+              const double *sparse_xyz, size_t n_sparse, double focal, int64_t ref_idx,
+              bool fine_grid, const vt3d_sweep_params &p, vt3d_poll_fn poll, void *poll_user,
+              vt3d_dense_result **out) {
+    Control ctl{poll, poll_user};
+    return solve_ctl(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal, ref_idx, fine_grid,
+                     p, ctl, out);
 }
 
 int check_out(vt3d_dense_result **out, const vt3d_camera *cameras, size_t n_cameras,
@@ -1555,6 +1565,388 @@ uint64_t request_bytes(const Prepared &pr, const vt3d_tuning &tuning) {
                             2ull * pr.S * pr.D * 9 * 8 + 8ull * pr.D * (1 + pr.S);
     return cap * per_slot + cap * n_hb * sizeof(Top2) + cap * (uint64_t)pr.S * sizeof(Recip) +
            blocks * kCounters * 8 + tables + 64 * 4;
+}
+
+// ---- multi-reference estimate (dense.rs) -------------------------------------------------------
+
+// has_distributed_landmarks, dense.rs:161-213: the finer grid only when the landmarks cover at
+// least half of a 4x4 tiling (two landmarks per tile) in a majority of the views. Polls once
+// per chunk of 256 landmarks.
+bool has_distributed_landmarks(const FrameSource &fs, const std::vector<Cam> &cams,
+                               const std::vector<V3> &sparse, double f, Control &ctl) {
+    const size_t majority = cams.size() / 2 + 1;
+    size_t covered = 0;
+    for (const Cam &c : cams) {
+        if (c.frame_index >= fs.count()) continue;
+        const uint32_t w = fs.w(c.frame_index), h = fs.h(c.frame_index);
+        uint8_t tiles[4][4] = {};
+        int supported = 0;
+        bool next_camera = false;
+        for (size_t c0 = 0; c0 < sparse.size() && !next_camera; c0 += 256) {
+            if (ctl.is_canceled()) return false;
+            for (size_t i = c0; i < std::min(c0 + 256, sparse.size()); ++i) {
+                double x, y, z;
+                if (!project(c, sparse[i], w, h, f, &x, &y, &z)) continue;
+                if (!(x >= 0.0 && x < (double)w) || !(y >= 0.0 && y < (double)h)) continue;
+                // x < w, so 4x/w rounds up to 4.0 only within an ulp of the edge
+                const size_t col = std::min<size_t>(3, (size_t)(4.0 * x / (double)w));
+                const size_t row = std::min<size_t>(3, (size_t)(4.0 * y / (double)h));
+                uint8_t &t = tiles[row][col];
+                if (t < 255) ++t;
+                if (t == 2 && ++supported >= 8) {
+                    if (++covered == majority) return true;
+                    next_camera = true;
+                    break;
+                }
+            }
+        }
+    }
+    return false;
+}
+
+// reference_candidate_order, dense.rs:620-672: cameras with at least 8 landmarks inside their
+// image, the most-seeing first (lower frame index on ties); the rest farthest from it first,
+// then more landmarks, then lower frame index.
+std::vector<size_t> reference_candidate_order(const FrameSource &fs, const std::vector<Cam> &cams,
+                                              const std::vector<V3> &sparse, double f) {
+    struct Cand {
+        size_t idx, visible;
+    };
+    std::vector<Cand> cand;
+    for (size_t i = 0; i < cams.size(); ++i) {
+        if (cams[i].frame_index >= fs.count()) continue;
+        const uint32_t w = fs.w(cams[i].frame_index), h = fs.h(cams[i].frame_index);
+        size_t visible = 0;
+        for (const V3 &q : sparse) {
+            double x, y, z;
+            if (project(cams[i], q, w, h, f, &x, &y, &z) && in_image(x, y, w, h)) ++visible;
+        }
+        if (visible >= kMinVisibleSparsePoints) cand.push_back({i, visible});
+    }
+    std::stable_sort(cand.begin(), cand.end(), [&](const Cand &a, const Cand &b) {
+        if (a.visible != b.visible) return a.visible > b.visible;
+        return cams[a.idx].frame_index < cams[b.idx].frame_index;
+    });
+    if (cand.empty()) return {};
+    const V3 pc = center(cams[cand[0].idx]);
+    auto dist2 = [&](size_t i) {
+        const V3 d = sub(center(cams[i]), pc);
+        return (d.x * d.x + d.y * d.y) + d.z * d.z;
+    };
+    std::stable_sort(cand.begin() + 1, cand.end(), [&](const Cand &a, const Cand &b) {
+        const double da = dist2(a.idx), db = dist2(b.idx);
+        if (da != db) return da > db;
+        if (a.visible != b.visible) return a.visible > b.visible;
+        return cams[a.idx].frame_index < cams[b.idx].frame_index;
+    });
+    std::vector<size_t> out;
+    for (const Cand &c : cand) out.push_back(c.idx);
+    return out;
+}
+
+// reference_attempt_stats, dense.rs:749-785.
+vt3d_reference_attempt attempt_stats(uint64_t reference_frame, const vt3d_dense_result &r,
+                                     bool accepted) {
+    vt3d_reference_attempt a{};
+    const vt3d_dense_stats &st = r.stats;
+    a.reference_frame = reference_frame;
+    a.attempted = st.attempted;
+    a.accepted = accepted ? 1 : 0;
+    if (!accepted) {
+        if (r.skip_code)
+            std::snprintf(a.skip_reason, sizeof(a.skip_reason), "%s", r.skip_reason);
+        else if (!st.attempted)
+            std::snprintf(a.skip_reason, sizeof(a.skip_reason),
+                          "dense estimation did not run for this reference");
+        else
+            std::snprintf(a.skip_reason, sizeof(a.skip_reason),
+                          "reference produced %zu accepted dense points; at least %d are "
+                          "required for a surface patch",
+                          r.n_points, VT3D_MIN_REFERENCE_PATCH_POINTS);
+    }
+    a.sampled_pixels = st.sampled_pixels;
+    a.accepted_points = r.n_points;
+    a.reciprocal_checked_points = st.reciprocal_checked_points;
+    a.reciprocal_rejected_points = st.reciprocal_rejected_points;
+    a.reciprocal_consistent_points = st.reciprocal_consistent_points;
+    a.surface_completion_proposals = st.surface_completion_proposals;
+    a.surface_completed_points = st.surface_completed_points;
+    a.surface_completion_rejected_texture = st.surface_completion_rejected_texture;
+    a.surface_completion_rejected_cross_view = st.surface_completion_rejected_cross_view;
+    a.surface_completion_rejected_reciprocal = st.surface_completion_rejected_reciprocal;
+    a.surface_completion_rejected_fusion = st.surface_completion_rejected_fusion;
+    a.surface_completion_rejected_footprint = st.surface_completion_rejected_footprint;
+    return a;
+}
+
+vt3d_reference_patch_stats patch_stats(const vt3d_dense_result &r, size_t primary_start,
+                                       size_t completion_start) {
+    vt3d_reference_patch_stats ps{};
+    const vt3d_dense_stats &st = r.stats;
+    const size_t completed = std::min<size_t>(st.surface_completed_points, r.n_points);
+    ps.reference_frame = st.reference_frame;
+    ps.n_source_frames = std::min<uint64_t>(st.source_views, 32);
+    for (size_t k = 0; k < ps.n_source_frames; ++k) ps.source_frames[k] = st.source_frames[k];
+    ps.primary_start = primary_start;
+    ps.primary_points = r.n_points - completed;
+    ps.completion_start = completion_start;
+    ps.completed_points = completed;
+    ps.sampled_pixels = st.sampled_pixels;
+    ps.accepted_points = r.n_points;
+    ps.reciprocal_consistent_points = st.reciprocal_consistent_points;
+    ps.grid_stride = st.grid_stride;
+    ps.grid_border = st.grid_border;
+    ps.has_search_depth = st.has_search_depth;
+    ps.search_min_depth = st.search_min_depth;
+    ps.search_max_depth = st.search_max_depth;
+    return ps;
+}
+
+// from_legacy, dense.rs:997-1082: a single-reference analysis as a multi-reference one. Takes
+// ownership of r's arrays.
+void from_legacy(vt3d_dense_result *r, vt3d_multi_result *m) {
+    m->combined = *r;
+    std::free(r);
+    const vt3d_dense_result &c = m->combined;
+    if (!c.stats.has_reference_frame) return;
+    const uint64_t ref = c.stats.reference_frame;
+    m->n_reference_frames = 1;
+    m->reference_frames[0] = ref;
+    m->n_attempts = 1;
+    m->attempts[0] = attempt_stats(
+        ref, c, c.stats.attempted && c.n_points >= (size_t)VT3D_MIN_REFERENCE_PATCH_POINTS);
+    if (c.stats.attempted) {
+        const size_t completed = std::min<size_t>(c.stats.surface_completed_points, c.n_points);
+        m->n_patches = 1;
+        m->patches[0] = patch_stats(c, 0, c.n_points - completed);
+    }
+}
+
+// merge_fallback_attempts, dense.rs:787-804.
+void merge_fallback_attempts(std::vector<vt3d_reference_attempt> &attempts,
+                             const vt3d_multi_result &fallback) {
+    for (size_t k = 0; k < fallback.n_attempts; ++k) {
+        const vt3d_reference_attempt &fa = fallback.attempts[k];
+        bool found = false;
+        for (vt3d_reference_attempt &e : attempts)
+            if (e.reference_frame == fa.reference_frame) {
+                if (fa.accepted && !e.accepted) e = fa;
+                found = true;
+                break;
+            }
+        if (!found) attempts.push_back(fa);
+    }
+}
+
+// combine_patches, dense.rs:842-995. Takes ownership of the patch results.
+int combine_patches(std::vector<vt3d_dense_result *> &patches,
+                    const std::vector<vt3d_reference_attempt> &attempts, vt3d_multi_result *m) {
+    size_t total = 0;
+    for (vt3d_dense_result *p : patches) total += p->n_points;
+    vt3d_dense_result &c = m->combined;
+    if (total) {
+        c.points = (vt3d_point3 *)std::malloc(sizeof(vt3d_point3) * total);
+        c.grid_sites = (vt3d_grid_site *)std::malloc(sizeof(vt3d_grid_site) * total);
+        c.attributes = (vt3d_point_attributes *)std::malloc(sizeof(vt3d_point_attributes) * total);
+        if (!c.points || !c.grid_sites || !c.attributes) return VT3D_E_DEVICE;
+    }
+    std::vector<size_t> primary_start, completion_start;
+    size_t at = 0;
+    for (int part = 0; part < 2; ++part)
+        for (vt3d_dense_result *p : patches) {
+            const size_t completed = std::min<size_t>(p->stats.surface_completed_points, p->n_points);
+            const size_t prim = p->n_points - completed;
+            const size_t from = part == 0 ? 0 : prim, count = part == 0 ? prim : completed;
+            (part == 0 ? primary_start : completion_start).push_back(at);
+            if (count) {
+                std::memcpy(c.points + at, p->points + from, sizeof(vt3d_point3) * count);
+                std::memcpy(c.grid_sites + at, p->grid_sites + from, sizeof(vt3d_grid_site) * count);
+                std::memcpy(c.attributes + at, p->attributes + from,
+                            sizeof(vt3d_point_attributes) * count);
+            }
+            at += count;
+        }
+    c.n_points = total;
+    vt3d_dense_stats &st = c.stats;
+    const vt3d_dense_stats &first = patches[0]->stats;
+    const uint64_t primary = first.reference_frame;
+    std::vector<uint64_t> part;  // participating frames other than the primary, ascending
+    for (vt3d_dense_result *p : patches) {
+        part.push_back(p->stats.reference_frame);
+        for (uint64_t k = 0; k < std::min<uint64_t>(p->stats.source_views, 32); ++k)
+            part.push_back(p->stats.source_frames[k]);
+    }
+    std::sort(part.begin(), part.end());
+    part.erase(std::unique(part.begin(), part.end()), part.end());
+    part.erase(std::remove(part.begin(), part.end(), primary), part.end());
+    st.attempted = 1;
+    st.has_reference_frame = 1;
+    st.reference_frame = primary;
+    st.source_views = part.size();
+    for (size_t k = 0; k < part.size() && k < 32; ++k) st.source_frames[k] = part[k];
+    const bool single = patches.size() == 1;
+    for (size_t k = 0; k < patches.size(); ++k) {
+        const vt3d_dense_stats &q = patches[k]->stats;
+        m->reference_frames[k] = q.reference_frame;
+        m->patches[k] = patch_stats(*patches[k], primary_start[k], completion_start[k]);
+        st.sampled_pixels += q.sampled_pixels;
+        st.depth_hypotheses = std::max(st.depth_hypotheses, q.depth_hypotheses);
+        st.surface_completion_proposals += q.surface_completion_proposals;
+        st.surface_completed_points += m->patches[k].completed_points;
+        st.surface_completion_rejected_texture += q.surface_completion_rejected_texture;
+        st.surface_completion_rejected_cross_view += q.surface_completion_rejected_cross_view;
+        st.surface_completion_rejected_reciprocal += q.surface_completion_rejected_reciprocal;
+        st.surface_completion_rejected_fusion += q.surface_completion_rejected_fusion;
+        st.surface_completion_rejected_footprint += q.surface_completion_rejected_footprint;
+        st.reciprocal_checked_points += q.reciprocal_checked_points;
+        st.reciprocal_rejected_points += q.reciprocal_rejected_points;
+        st.reciprocal_consistent_points += q.reciprocal_consistent_points;
+        st.fusion_input_observations += q.fusion_input_observations;
+        st.fusion_rejected_observations += q.fusion_rejected_observations;
+        st.fusion_rejected_points += q.fusion_rejected_points;
+    }
+    st.accepted_points = total;
+    st.grid_stride = first.grid_stride;
+    st.grid_border = first.grid_border;
+    if (single) {
+        st.has_median_fusion_observations = first.has_median_fusion_observations;
+        st.median_fusion_observations = first.median_fusion_observations;
+        st.has_median_supporting_views = first.has_median_supporting_views;
+        st.median_supporting_views = first.median_supporting_views;
+        st.has_median_photometric_error = first.has_median_photometric_error;
+        st.median_photometric_error = first.median_photometric_error;
+        st.has_search_depth = first.has_search_depth;
+        st.search_min_depth = first.search_min_depth;
+        st.search_max_depth = first.search_max_depth;
+    }
+    c.primary_points = completion_start[0];
+    m->n_reference_frames = patches.size();
+    m->n_patches = patches.size();
+    m->n_attempts = std::min<size_t>(attempts.size(), VT3D_MAX_REFERENCE_ATTEMPTS + 1);
+    for (size_t k = 0; k < m->n_attempts; ++k) m->attempts[k] = attempts[k];
+    for (vt3d_dense_result *p : patches) vt3d_dense_result_free(p);
+    patches.clear();
+    return VT3D_OK;
+}
+
+// estimate_depth_points_limited, dense.rs:244-330, on frames resident in the session.
+int dense_estimate(vt3d_session *s, const vt3d_camera *cams_in, size_t n_cams,
+                   const double *sparse_xyz, size_t n_sparse, double f, int reference_views,
+                   int fine_req, const vt3d_sweep_params &p, vt3d_poll_fn poll, void *user,
+                   vt3d_multi_result *m) {
+    FrameSource fs{nullptr, s, {}};
+    for (size_t i = 0; i < s->views.size(); ++i) fs.map.push_back(i);
+    std::vector<Cam> cams(n_cams);
+    for (size_t i = 0; i < n_cams; ++i) cams[i] = to_cam(cams_in[i]);
+    std::vector<V3> sparse(n_sparse);
+    for (size_t i = 0; i < n_sparse; ++i)
+        sparse[i] = {sparse_xyz[3 * i], sparse_xyz[3 * i + 1], sparse_xyz[3 * i + 2]};
+    Control ctl{poll, user};
+    // dense.rs:366-372: the finer grid only where it is finer for the first frame, and the
+    // landmark check (which polls) only then.
+    const bool finer = fs.count() > 0 && grid_stride_rule(fs.w(0), fs.h(0), true) <
+                                             grid_stride_rule(fs.w(0), fs.h(0), false);
+    const bool fine = fine_req < 0 ? finer && has_distributed_landmarks(fs, cams, sparse, f, ctl)
+                                   : fine_req != 0;
+    m->fine_grid = fine ? 1 : 0;
+    // The legacy single-reference sweep (fallback and empty candidate order). The same
+    // control carries on: a cancel seen here makes it return the canceled analysis.
+    auto legacy = [&](vt3d_multi_result *out) -> int {
+        vt3d_dense_result *r = nullptr;
+        int rc = solve_ctl(s, fs, cams_in, n_cams, sparse_xyz, n_sparse, f, -1, fine, p, ctl, &r);
+        if (rc) return rc;
+        from_legacy(r, out);
+        return VT3D_OK;
+    };
+    const std::vector<size_t> order = reference_candidate_order(fs, cams, sparse, f);
+    if (order.empty()) return legacy(m);
+    const size_t n_cand = std::min<size_t>(order.size(), VT3D_MAX_REFERENCE_ATTEMPTS);
+    // With a poll function the candidates run one at a time in order, with the reference's
+    // polling. Without one they run in waves of one batch each, a wave holding as many
+    // candidates as patches are still wanted: each patch depends only on its own inputs, so
+    // the kept patches are those of the sequential run, and no candidate past the last one
+    // the sequential run would reach is solved.
+    std::vector<vt3d_dense_result *> patches;
+    std::vector<vt3d_reference_attempt> attempts;
+    auto free_patches = [&]() {
+        for (vt3d_dense_result *q : patches) vt3d_dense_result_free(q);
+        patches.clear();
+    };
+    int omitted = 0;
+    bool done = false;
+    for (size_t k = 0; k < n_cand && !done;) {
+        const size_t wave =
+            poll ? 1 : std::min(n_cand - k, (size_t)reference_views - patches.size());
+        std::vector<vt3d_dense_result *> res(wave, nullptr);
+        if (poll) {
+            if (ctl.is_canceled()) {  // DenseAnalysis::default()
+                free_patches();
+                *m = vt3d_multi_result{};
+                m->fine_grid = fine ? 1 : 0;
+                return VT3D_OK;
+            }
+            const int rc = solve_ctl(s, fs, cams_in, n_cams, sparse_xyz, n_sparse, f,
+                                     (int64_t)order[k], fine, p, ctl, &res[0]);
+            if (rc && rc != VT3D_NO_PATCH) {
+                free_patches();
+                return rc;
+            }
+        } else {
+            vt3d_batch b;
+            b.session = s;
+            b.ws = &s->ws;
+            b.stream = s->stream;
+            int rc = VT3D_OK;
+            for (size_t j = 0; j < wave && !rc; ++j) {
+                b.prepared.emplace_back(new Prepared());
+                Control none{nullptr, nullptr};
+                rc = prepare_reference_patch(s, fs, cams_in, n_cams, sparse_xyz, n_sparse, f,
+                                             order[k + j], fine, p, none, *b.prepared.back());
+            }
+            std::vector<Prepared *> reqs;
+            for (auto &u : b.prepared) reqs.push_back(u.get());
+            if (!rc) rc = build_plan(reqs, s->tuning, b.plan);
+            if (!rc) rc = run_batch(b, res.data());
+            if (rc) {
+                for (vt3d_dense_result *r : res) vt3d_dense_result_free(r);
+                free_patches();
+                return rc;
+            }
+            s->timing = b.timing;
+        }
+        for (size_t j = 0; j < wave; ++j, ++k) {
+            vt3d_dense_result *r = res[j];
+            if (!r) continue;  // the remap returned None
+            const uint64_t ref_frame = cams_in[order[k]].frame_index;
+            const bool accepted =
+                r->stats.attempted && r->n_points >= (size_t)VT3D_MIN_REFERENCE_PATCH_POINTS;
+            attempts.push_back(attempt_stats(ref_frame, *r, accepted));
+            if (!accepted) {
+                vt3d_dense_result_free(r);
+                continue;
+            }
+            patches.push_back(r);
+            // a wave never holds more candidates than patches are wanted, so this is its last
+            if ((int)patches.size() >= reference_views) {
+                omitted = k + 1 < n_cand ? 1 : 0;
+                done = true;
+            }
+        }
+    }
+    if (patches.empty()) {
+        vt3d_multi_result fb{};
+        int rc = legacy(&fb);
+        if (rc) return rc;
+        merge_fallback_attempts(attempts, fb);
+        *m = fb;
+        m->fine_grid = fine ? 1 : 0;
+        m->n_attempts = std::min<size_t>(attempts.size(), VT3D_MAX_REFERENCE_ATTEMPTS + 1);
+        for (size_t k = 0; k < m->n_attempts; ++k) m->attempts[k] = attempts[k];
+        return VT3D_OK;
+    }
+    int rc = combine_patches(patches, attempts, m);
+    m->omitted_references = omitted;
+    return rc;
 }
 
 // ---- serving ---------------------------------------------------------------------------------
@@ -2124,6 +2516,50 @@ void vt3d_server_destroy(vt3d_server *sv) {
             (void)hipStreamDestroy(sl->stream);
         }
     delete sv;
+}
+
+int32_t vt3d_dense_estimate(const vt3d_frame *frames, size_t n_frames,
+                            const vt3d_camera *cameras, size_t n_cameras,
+                            const double *sparse_xyz, size_t n_sparse, double focal,
+                            int32_t reference_views, int32_t fine_grid,
+                            const vt3d_sweep_params *params, vt3d_poll_fn poll, void *poll_user,
+                            vt3d_multi_result **out) {
+    if (!out) return VT3D_E_INVALID;
+    *out = nullptr;
+    vt3d_sweep_params p;
+    int rc;
+    if ((rc = resolve_params(params, &p))) return rc;
+    if ((n_frames && !frames) || (n_cameras && !cameras) || (n_sparse && !sparse_xyz)) {
+        set_error("null input array with nonzero length");
+        return VT3D_E_INVALID;
+    }
+    if (reference_views < 1 || reference_views > VT3D_MAX_REFERENCE_VIEWS) {
+        set_error("reference_views must be 1..3");
+        return VT3D_E_INVALID;
+    }
+    vt3d_session *s = thread_session(&rc);
+    if (!s) return rc;
+    // The frames go to the device once and every candidate patch reads them there.
+    if ((rc = vt3d_session_upload_frames(s, frames, n_frames, p.storage))) return rc;
+    vt3d_multi_result *m = (vt3d_multi_result *)std::calloc(1, sizeof(vt3d_multi_result));
+    if (!m) return VT3D_E_DEVICE;
+    rc = dense_estimate(s, cameras, n_cameras, sparse_xyz, n_sparse, focal, reference_views,
+                        fine_grid, p, poll, poll_user, m);
+    if (rc) {
+        vt3d_multi_result_free(m);
+        return rc;
+    }
+    *out = m;
+    return VT3D_OK;
+}
+
+void vt3d_multi_result_free(vt3d_multi_result *m) {
+    if (!m) return;
+    std::free(m->combined.points);
+    std::free(m->combined.grid_sites);
+    std::free(m->combined.attributes);
+    std::free(m->combined.samples);
+    std::free(m);
 }
 
 }  // extern "C"

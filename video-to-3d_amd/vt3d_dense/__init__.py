@@ -6,8 +6,10 @@ The functions below mirror the reference's Rust interface for this path:
   control)` -- crates/video-to-3d-core/src/dense_legacy.rs:149-156
 * `reconstruct_reference_patch(frames, cameras, sparse_points, focal, reference_camera_index,
   fine_grid, control)` -- crates/video-to-3d-core/src/dense.rs:674-747
+* `estimate_depth_points_limited(frames, cameras, sparse_points, focal, reference_views,
+  fine_grid, control)` -- crates/video-to-3d-core/src/dense.rs:244-330
 
-Both run the HIP kernels of libvt3d_dense.so on the current GPU. There is no CPU path: if the
+They run the HIP kernels of libvt3d_dense.so on the current GPU. There is no CPU path: if the
 library is missing or no GPU is present, they raise.
 """
 from __future__ import annotations
@@ -73,6 +75,11 @@ def _bind(lib: C.CDLL) -> None:
     lib.vt3d_dense_reference_patch.restype = C.c_int32
     lib.vt3d_dense_result_free.argtypes = [P(abi.DenseResult)]
     lib.vt3d_dense_result_free.restype = None
+    lib.vt3d_dense_estimate.argtypes = common + [C.c_int32, C.c_int32, P(abi.SweepParams),
+                                                 abi.POLL_FN, C.c_void_p, P(P(abi.MultiResult))]
+    lib.vt3d_dense_estimate.restype = C.c_int32
+    lib.vt3d_multi_result_free.argtypes = [P(abi.MultiResult)]
+    lib.vt3d_multi_result_free.restype = None
     lib.vt3d_session_create.argtypes = [C.c_int32, P(C.c_void_p)]
     lib.vt3d_session_create.restype = C.c_int32
     lib.vt3d_session_destroy.argtypes = [C.c_void_p]
@@ -119,6 +126,7 @@ EXPORTED_SYMBOLS = (
     "vt3d_session_set_tuning", "vt3d_batch_create", "vt3d_batch_run",
     "vt3d_batch_last_timing", "vt3d_batch_destroy", "vt3d_server_create",
     "vt3d_server_submit", "vt3d_server_poll", "vt3d_server_get_stats", "vt3d_server_destroy",
+    "vt3d_dense_estimate", "vt3d_multi_result_free",
 )
 
 
@@ -203,6 +211,36 @@ def reconstruct_reference_patch(frames, cameras, sparse_points, focal: float,
         return None
     _check(rc)
     return _take(out)
+
+
+def estimate_depth_points_limited(frames, cameras, sparse_points, focal: float,
+                                  reference_views: int = abi.MAX_REFERENCE_VIEWS,
+                                  fine_grid: Optional[bool] = None,
+                                  control: Optional[Callable[[], bool]] = None,
+                                  sweep: Optional[abi.SweepParams] = None):
+    """dense.rs:244-330 on the GPU: returns (MultiAnalysis, omitted_references). fine_grid None
+    decides the grid as the budgeted entry does (dense.rs:366-372, no working-set budget)."""
+    lib = library()
+    inp = abi.Inputs(_rgba(frames), cameras, sparse_points)
+    out = C.POINTER(abi.MultiResult)()
+    poll = _poll(control)
+    fine = -1 if fine_grid is None else int(bool(fine_grid))
+    rc = lib.vt3d_dense_estimate(inp.frames, inp.n_frames, inp.cameras, inp.n_cameras,
+                                 inp.sparse_ptr, inp.n_sparse, float(focal), int(reference_views),
+                                 fine, C.byref(sweep) if sweep is not None else None, poll, None,
+                                 C.byref(out))
+    _check(rc)
+    owner = _ResultOwner(out, lib.vt3d_multi_result_free)
+    m = abi.unpack_multi(out.contents, owner)
+    return m, m.omitted_references
+
+
+def estimate_depth_points(frames, cameras, sparse_points, focal: float,
+                          control: Optional[Callable[[], bool]] = None,
+                          sweep: Optional[abi.SweepParams] = None) -> "abi.MultiAnalysis":
+    """dense.rs:222-244: all three reference views, the grid decided from the landmarks."""
+    return estimate_depth_points_limited(frames, cameras, sparse_points, focal,
+                                         abi.MAX_REFERENCE_VIEWS, None, control, sweep)[0]
 
 
 class Session:

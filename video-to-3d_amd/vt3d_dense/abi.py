@@ -157,6 +157,44 @@ class ServerStats(C.Structure):
                 ("budget_bytes", C.c_uint64)]
 
 
+MAX_REFERENCE_VIEWS = 3      # dense.rs:8
+MAX_REFERENCE_ATTEMPTS = 6   # dense.rs:9
+MIN_REFERENCE_PATCH_POINTS = 3  # dense.rs:10
+
+
+class ReferenceAttempt(C.Structure):
+    _fields_ = [("reference_frame", C.c_uint64), ("attempted", C.c_int32),
+                ("accepted", C.c_int32), ("skip_reason", C.c_char * 256)] + [
+        (n, C.c_uint64) for n in (
+            "sampled_pixels", "accepted_points", "reciprocal_checked_points",
+            "reciprocal_rejected_points", "reciprocal_consistent_points",
+            "surface_completion_proposals", "surface_completed_points",
+            "surface_completion_rejected_texture", "surface_completion_rejected_cross_view",
+            "surface_completion_rejected_reciprocal", "surface_completion_rejected_fusion",
+            "surface_completion_rejected_footprint")]
+
+
+class ReferencePatchStats(C.Structure):
+    _fields_ = [("reference_frame", C.c_uint64), ("n_source_frames", C.c_uint64),
+                ("source_frames", C.c_uint64 * 32)] + [
+        (n, C.c_uint64) for n in (
+            "primary_start", "primary_points", "completion_start", "completed_points",
+            "sampled_pixels", "accepted_points", "reciprocal_consistent_points",
+            "grid_stride")] + [
+        ("grid_border", C.c_uint32), ("has_search_depth", C.c_int32),
+        ("search_min_depth", C.c_float), ("search_max_depth", C.c_float)]
+
+
+class MultiResult(C.Structure):
+    _fields_ = [("combined", DenseResult), ("n_reference_frames", C.c_size_t),
+                ("reference_frames", C.c_uint64 * MAX_REFERENCE_VIEWS),
+                ("n_attempts", C.c_size_t),
+                ("attempts", ReferenceAttempt * (MAX_REFERENCE_ATTEMPTS + 1)),
+                ("n_patches", C.c_size_t),
+                ("patches", ReferencePatchStats * MAX_REFERENCE_VIEWS),
+                ("omitted_references", C.c_int32), ("fine_grid", C.c_int32)]
+
+
 POLL_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p)
 
 POINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("confidence", "<f4"),
@@ -225,6 +263,49 @@ def unpack_result(res: DenseResult, owner=None) -> DenseAnalysis:
                          _array(res.grid_sites, n, SITE_DTYPE, owner),
                          _array(res.attributes, n, ATTR_DTYPE, owner), int(res.primary_points),
                          samples, int(res.skip_code))
+
+
+@dataclasses.dataclass
+class MultiAnalysis:
+    """The multi-reference DenseAnalysis of crates/video-to-3d-core/src/dense.rs:100-146
+    (without the working-set fields): `combined` with stats["reference_frames"],
+    stats["reference_attempts"] and stats["reference_patches"] filled in."""
+    combined: DenseAnalysis
+    omitted_references: bool
+    fine_grid: bool
+
+    @property
+    def stats(self) -> Dict[str, object]:
+        return self.combined.stats
+
+
+def _attempt(a: ReferenceAttempt) -> Dict[str, object]:
+    d = {n: int(getattr(a, n)) for n, _ in ReferenceAttempt._fields_
+         if n not in ("attempted", "accepted", "skip_reason")}
+    d["attempted"] = bool(a.attempted)
+    d["accepted"] = bool(a.accepted)
+    d["skip_reason"] = None if a.accepted else a.skip_reason.decode()
+    return d
+
+
+def _patch(p: ReferencePatchStats) -> Dict[str, object]:
+    d = {n: int(getattr(p, n)) for n, _ in ReferencePatchStats._fields_
+         if n not in ("n_source_frames", "source_frames", "has_search_depth",
+                      "search_min_depth", "search_max_depth")}
+    d["source_frames"] = [int(p.source_frames[i]) for i in range(min(int(p.n_source_frames), 32))]
+    d["search_min_depth"] = float(p.search_min_depth) if p.has_search_depth else None
+    d["search_max_depth"] = float(p.search_max_depth) if p.has_search_depth else None
+    return d
+
+
+def unpack_multi(m: MultiResult, owner=None) -> MultiAnalysis:
+    """MultiAnalysis of a C multi result (views of its buffers with `owner`, else copies)."""
+    combined = unpack_result(m.combined, owner)
+    st = combined.stats
+    st["reference_frames"] = [int(m.reference_frames[i]) for i in range(int(m.n_reference_frames))]
+    st["reference_attempts"] = [_attempt(m.attempts[i]) for i in range(int(m.n_attempts))]
+    st["reference_patches"] = [_patch(m.patches[i]) for i in range(int(m.n_patches))]
+    return MultiAnalysis(combined, bool(m.omitted_references), bool(m.fine_grid))
 
 
 class Inputs:
