@@ -343,6 +343,28 @@ int32_t vt3d_dense_estimate(const vt3d_frame *frames, size_t n_frames,
                             vt3d_multi_result **out);
 void vt3d_multi_result_free(vt3d_multi_result *result);
 
+/* ---- wire formats and diagnostics (host only) ---------------------------------------- */
+
+/* pack_dense_points (crates/video-to-3d-wasm/src/lib.rs:227-242): n points as little-endian
+ * f32 (x, y, z, confidence) into xyzw_f32_le (16 n bytes) and (r, g, b) into rgb (3 n
+ * bytes). Returns n. */
+size_t vt3d_pack_dense_points(const vt3d_point3 *points, size_t n, uint8_t *xyzw_f32_le,
+                              uint8_t *rgb);
+/* pack_dense_grid_sites (lib.rs:244-254): little-endian u32 (x, y) pairs, 8 n bytes. */
+size_t vt3d_pack_grid_sites(const vt3d_grid_site *sites, size_t n, uint8_t *xy_u32_le);
+/* pack_dense_point_attributes (lib.rs:256-268): little-endian f32 (reciprocal_consistency,
+ * depth_margin) pairs, 8 n bytes. */
+size_t vt3d_pack_point_attributes(const vt3d_point_attributes *attributes, size_t n,
+                                  uint8_t *values_f32_le);
+/* unpack_dense_points (lib.rs:344-366): VT3D_E_INVALID unless n_bytes == 16 point_count;
+ * colours come back 0. */
+int32_t vt3d_unpack_dense_points(const uint8_t *xyzw_f32_le, size_t n_bytes, size_t point_count,
+                                 vt3d_point3 *points);
+/* dense_warning (crates/video-to-3d-core/src/reconstruction/warnings.rs:27-68): the exact
+ * text for these stats, written like snprintf into buf (cap bytes, NUL-terminated).
+ * Returns the text's length, or 0 where the reference returns None (not attempted). */
+size_t vt3d_dense_warning(const vt3d_dense_stats *stats, char *buf, size_t cap);
+
 /* ---- device-resident sessions (frames uploaded once, solved many times) -------------- */
 /* No counterpart in the reference: the reference re-reads host frames every call. A
  * session keeps luma textures in HBM so repeated solves (benchmarks, batching, serving)

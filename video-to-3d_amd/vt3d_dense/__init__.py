@@ -80,6 +80,17 @@ def _bind(lib: C.CDLL) -> None:
     lib.vt3d_dense_estimate.restype = C.c_int32
     lib.vt3d_multi_result_free.argtypes = [P(abi.MultiResult)]
     lib.vt3d_multi_result_free.restype = None
+    u8p = P(C.c_uint8)
+    lib.vt3d_pack_dense_points.argtypes = [P(abi.Point3), C.c_size_t, u8p, u8p]
+    lib.vt3d_pack_dense_points.restype = C.c_size_t
+    lib.vt3d_pack_grid_sites.argtypes = [P(abi.GridSite), C.c_size_t, u8p]
+    lib.vt3d_pack_grid_sites.restype = C.c_size_t
+    lib.vt3d_pack_point_attributes.argtypes = [P(abi.PointAttributes), C.c_size_t, u8p]
+    lib.vt3d_pack_point_attributes.restype = C.c_size_t
+    lib.vt3d_unpack_dense_points.argtypes = [u8p, C.c_size_t, C.c_size_t, P(abi.Point3)]
+    lib.vt3d_unpack_dense_points.restype = C.c_int32
+    lib.vt3d_dense_warning.argtypes = [P(abi.DenseStats), C.c_char_p, C.c_size_t]
+    lib.vt3d_dense_warning.restype = C.c_size_t
     lib.vt3d_session_create.argtypes = [C.c_int32, P(C.c_void_p)]
     lib.vt3d_session_create.restype = C.c_int32
     lib.vt3d_session_destroy.argtypes = [C.c_void_p]
@@ -126,7 +137,9 @@ EXPORTED_SYMBOLS = (
     "vt3d_session_set_tuning", "vt3d_batch_create", "vt3d_batch_run",
     "vt3d_batch_last_timing", "vt3d_batch_destroy", "vt3d_server_create",
     "vt3d_server_submit", "vt3d_server_poll", "vt3d_server_get_stats", "vt3d_server_destroy",
-    "vt3d_dense_estimate", "vt3d_multi_result_free",
+    "vt3d_dense_estimate", "vt3d_multi_result_free", "vt3d_pack_dense_points",
+    "vt3d_pack_grid_sites", "vt3d_pack_point_attributes", "vt3d_unpack_dense_points",
+    "vt3d_dense_warning",
 )
 
 
@@ -241,6 +254,48 @@ def estimate_depth_points(frames, cameras, sparse_points, focal: float,
     """dense.rs:222-244: all three reference views, the grid decided from the landmarks."""
     return estimate_depth_points_limited(frames, cameras, sparse_points, focal,
                                          abi.MAX_REFERENCE_VIEWS, None, control, sweep)[0]
+
+
+def pack_dense(analysis: DenseAnalysis) -> dict:
+    """The WASM adapter's packed buffers (crates/video-to-3d-wasm/src/lib.rs:227-268) of an
+    analysis: {"dense_points": {"values_f32_le", "rgb", "length"}, "dense_grid_sites":
+    {"xy_u32_le", "length"}, "dense_point_attributes": {"values_f32_le", "length"}}."""
+    lib = library()
+    pts = np.ascontiguousarray(analysis.points)
+    sites = np.ascontiguousarray(analysis.grid_sites)
+    attrs = np.ascontiguousarray(analysis.point_attributes)
+    n, ns, na = len(pts), len(sites), len(attrs)
+    xyzw = (C.c_uint8 * max(1, 16 * n))()
+    rgb = (C.c_uint8 * max(1, 3 * n))()
+    xy = (C.c_uint8 * max(1, 8 * ns))()
+    av = (C.c_uint8 * max(1, 8 * na))()
+    lib.vt3d_pack_dense_points(pts.ctypes.data_as(C.POINTER(abi.Point3)), n, xyzw, rgb)
+    lib.vt3d_pack_grid_sites(sites.ctypes.data_as(C.POINTER(abi.GridSite)), ns, xy)
+    lib.vt3d_pack_point_attributes(attrs.ctypes.data_as(C.POINTER(abi.PointAttributes)), na, av)
+    return {"dense_points": {"values_f32_le": bytes(xyzw)[:16 * n], "rgb": bytes(rgb)[:3 * n],
+                             "length": n},
+            "dense_grid_sites": {"xy_u32_le": bytes(xy)[:8 * ns], "length": ns},
+            "dense_point_attributes": {"values_f32_le": bytes(av)[:8 * na], "length": na}}
+
+
+def dense_warning(stats: dict) -> Optional[str]:
+    """dense_warning (reconstruction/warnings.rs:27-68) of an analysis' stats dict."""
+    st = abi.DenseStats()
+    for k in ("accepted_points", "source_views", "depth_hypotheses",
+              "reciprocal_consistent_points", "reciprocal_checked_points",
+              "fusion_rejected_observations"):
+        setattr(st, k, int(stats[k]))
+    st.attempted = int(bool(stats["attempted"]))
+    if stats.get("reference_frame") is not None:
+        st.has_reference_frame = 1
+        st.reference_frame = int(stats["reference_frame"])
+    lib = library()
+    n = lib.vt3d_dense_warning(C.byref(st), None, 0)
+    if n == 0:
+        return None
+    buf = C.create_string_buffer(n + 1)
+    lib.vt3d_dense_warning(C.byref(st), buf, n + 1)
+    return buf.value.decode()
 
 
 class Session:
