@@ -22,7 +22,8 @@ Four legs, all with frames resident in HBM; units = texture-gated samples x hypo
   grid rows (chosen hypothesis, photometric cost, fused points).
 
 Prints one JSON line. `roofline` prices the batched leg's primary sweep kernel against the
-FP32 vector peak with 24*S*P^2 FLOPs per unit; `cpu_baseline` times the FP64 oracle on whole
+FP32 vector peak at 24*P^2 FLOPs per patch cost, one patch cost per (sample, hypothesis,
+source kept by that keyframe's selection); `cpu_baseline` times the FP64 oracle on whole
 keyframes of the batched workload (about 10 s of work) on the host cores.
 """
 from __future__ import annotations
@@ -212,6 +213,10 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
     walls_ms = [w * 1e3 for w in walls]
     per_step_units = units // max(1, steps)
     ok = [r for r in res if r is not None]
+    # (sample, hypothesis, source) triples of one step: each is one P x P patch cost of the
+    # primary sweep, whatever the number of sources each keyframe kept.
+    sample_hyp_sources = sum(int(r.stats["sampled_pixels"]) * int(r.stats["depth_hypotheses"])
+                             * int(r.stats["source_views"]) for r in ok)
     return {
         "value": units / sum(walls),
         "device_value": units / (1e-3 * sum(dev_ms)),
@@ -231,7 +236,7 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
                                              "cached": tuned["cached"],
                                              "table": tuned.get("table")},
         "cpu_problems": problems[:4],
-        "S": 8, "P": 7,
+        "sample_hyp_sources": sample_hyp_sources, "P": 7,
     }
 
 
@@ -414,11 +419,12 @@ def main() -> int:
     prec = None
     if args.precision_rows > 0:
         prec = precision_leg(V, scenes, args.precision_rows, _host_threads(), log)
-    S, P = bat.pop("S"), bat.pop("P")
+    triples, P = bat.pop("sample_hyp_sources"), bat.pop("P")
     cpu_problems = bat.pop("cpu_problems")
-    flops_per_unit = 24 * S * P * P
+    flops_per_launch = 24 * P * P * triples
+    mean_sources = triples / max(1, bat["units_per_step"])
     sweep_s = bat["device_ms"]["sweep"] * 1e-3
-    achieved = bat["units_per_step"] * flops_per_unit / sweep_s / 1e12
+    achieved = flops_per_launch / sweep_s / 1e12
     kernel = ("k_sweep_tile<3, ZNCC, F16, 8>"
               if not bat["tuning"] or bat["tuning"]["choice"]["kernel"] == "tiled"
               else "k_sweep<3, ZNCC, F16, 8>")
@@ -439,7 +445,8 @@ def main() -> int:
                                "480x270 mix) each with its 8 nearest frames, 96 hyps 0.5-8 m, "
                                "7x7 ZNCC, stride 2, FP16 storage, reciprocal + fusion, one "
                                "hipGraph replay per step",
-                   "solves": bat["solves"], "sources": S, "hypotheses": 96, "patch": P,
+                   "solves": bat["solves"], "sources": 8,
+                   "mean_sources_kept": round(mean_sources, 3), "hypotheses": 96, "patch": P,
                    "stride": 2, "storage": "f16",
                    "sampled_pixels": bat["sampled_pixels"],
                    "accepted_points": bat["accepted_points"]},
@@ -456,7 +463,8 @@ def main() -> int:
         "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
                      "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": _traffic("k_sweep_tile_c3"),
-                     "kernel": kernel, "flops_per_unit": flops_per_unit,
+                     "kernel": kernel, "flops_per_patch_cost": 24 * P * P,
+                     "patch_costs_per_launch": triples,
                      "units_per_launch": bat["units_per_step"]},
     }
     if not args.no_cpu_baseline:

@@ -117,8 +117,9 @@ template <int ST> struct TexRef<ST, true> {
                                                  (int)kBufferFlags);
     }
     // Coefficients (a, b, c, d) of texel `i` (see Store).
-    __device__ __forceinline__ float4 coef(uint32_t i) const {
-        const uint32_t off = i * kTexelBytes;
+    __device__ __forceinline__ float4 coef(uint32_t i) const { return coef_at(i * kTexelBytes); }
+    // ... of the texel at byte offset `off`.
+    __device__ __forceinline__ float4 coef_at(uint32_t off) const {
         if constexpr (ST == VT3D_STORAGE_F32) {
             const auto v = __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)off, 0, 0);
             return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]),
@@ -151,6 +152,14 @@ template <int ST> struct TexRef<ST, false> {
 __device__ __forceinline__ int floor_to_int(float x) {
     int r;
     asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+// a * b + c on 24-bit signed operands in one instruction. Written out because the compiler
+// otherwise re-associates a following add into a three-instruction address sequence.
+__device__ __forceinline__ int mad_i24(int a, int b, int c) {
+    int r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
     return r;
 }
 
@@ -374,6 +383,7 @@ __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 *
     }
     if (!ok) return kInfF;
     const uint32_t base = (uint32_t)(w.iyc * (int)sw + w.ixc);
+    const uint32_t base_bytes = base * TexRef<ST, true>::kTexelBytes;
     const float pivot = st.mean;
     float s1 = 0.f, s2 = 0.f, srs = 0.f;
     float sv[COST == VT3D_COST_ZSAD ? T : 1];
@@ -396,7 +406,13 @@ __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 *
             const int ixo = floor_to_int(qx), iyo = floor_to_int(qy);
             tx[dx + R] = __builtin_amdgcn_fractf(qx);
             ty[dx + R] = __builtin_amdgcn_fractf(qy);
-            kk[dx + R] = tex.coef(base + (uint32_t)__mul24(iyo, (int)sw) + (uint32_t)ixo);
+            // texel offset from the centre's cell: one 24-bit multiply-add (and, for the
+            // buffer form, one shift-add to bytes)
+            const int rel = mad_i24(iyo, (int)sw, ixo);
+            if constexpr (BUF)
+                kk[dx + R] = tex.coef_at(base_bytes + (uint32_t)rel * TexRef<ST, true>::kTexelBytes);
+            else
+                kk[dx + R] = tex.coef(base + (uint32_t)rel);
         }
 #pragma unroll
         for (int dx = -R; dx <= R; ++dx) {
@@ -467,6 +483,102 @@ __device__ __forceinline__ void warp_region(float *__restrict__ L, const double 
             tx[j] = (float)((valid ? xs : 0.0) - fx);
             ty[j] = (float)((valid ? ys : 0.0) - fy);
             idx[j] = (uint32_t)fy * sw + (uint32_t)fx;
+            px += step_x;
+            py += step_y;
+            const bool wrap = px >= RW;
+            px = wrap ? px - RW : px;
+            py = wrap ? py + 1 : py;
+        }
+        float4 k[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) k[j] = tex.coef(idx[j]);
+        const float nan = __builtin_nanf("");
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const int q = q0 + j * kBlock;
+            if (q < npx) L[q] = ok[j] ? bicoef(k[j], tx[j], ty[j]) : nan;
+        }
+    }
+}
+
+// FP32 warp of a tile's region relative to one FP64 warp of the region's centre, as
+// make_warp does for one patch: x = xc + (ax du + bx dv) / (1 + g du + e dv) for offsets
+// (du, dv) from the centre, written in region pixel coordinates (u, v) so that a pixel costs
+// three FMAs per coordinate. A (source, hypothesis) step takes it when the denominator stays
+// within [0.75, 1.25] over the region and the centre's source depth exceeds 2e-4: then every
+// pixel's source depth exceeds 1e-4 (project's test, dense_legacy.rs:1052-1062, passes for all
+// of them) and the FP32 positions lie within about 1e-5 px of the FP64 ones. Otherwise the
+// step takes warp_region.
+struct RegionWarp {
+    float g0, gu, gv;     // denominator g0 + gu u + gv v
+    float x0, xu, xv;     // x numerator
+    float y0, yu, yv;     // y numerator
+    float fxc, fyc;       // fractional part of the centre's source position
+    int ixc, iyc;         // its integer part
+    bool ok;
+};
+
+__device__ __forceinline__ RegionWarp region_warp(const double (&h)[9], int X0, int Y0, int RW,
+                                                  int RH, uint32_t sw, uint32_t sh) {
+    const double hx = 0.5 * (double)(RW - 1), hy = 0.5 * (double)(RH - 1);
+    const double cx = (double)X0 + hx, cy = (double)Y0 + hy;
+    const double nc = fma(h[0], cx, fma(h[1], cy, h[2]));
+    const double mc = fma(h[3], cx, fma(h[4], cy, h[5]));
+    const double dc = fma(h[6], cx, fma(h[7], cy, h[8]));
+    const bool pos = dc > 2.0e-4;
+    const double rd = recip_f64(pos ? dc : 1.0);
+    const double xc = nc * rd, yc = mc * rd;
+    const double g = h[6] * rd, e = h[7] * rd;
+    const double ax = (h[0] - xc * h[6]) * rd, bx = (h[1] - xc * h[7]) * rd;
+    const double ay = (h[3] - yc * h[6]) * rd, by = (h[4] - yc * h[7]) * rd;
+    RegionWarp w;
+    // the offsets stay far inside int range: |x - xc| <= reach / 0.75
+    const double reach = (fabs(ax) + fabs(bx) + fabs(ay) + fabs(by)) * (hx + hy);
+    w.ok = pos && fabs(g) * hx + fabs(e) * hy <= 0.25 && reach < 1.0e5 && xc > -1.0e5 &&
+           xc < (double)sw + 1.0e5 && yc > -1.0e5 && yc < (double)sh + 1.0e5;
+    const double fx = w.ok ? floor(xc) : 0.0, fy = w.ok ? floor(yc) : 0.0;
+    w.ixc = (int)fx;
+    w.iyc = (int)fy;
+    w.fxc = w.ok ? (float)(xc - fx) : 0.f;
+    w.fyc = w.ok ? (float)(yc - fy) : 0.f;
+    w.g0 = (float)(1.0 - g * hx - e * hy);
+    w.gu = (float)g;
+    w.gv = (float)e;
+    w.x0 = (float)(-(ax * hx + bx * hy));
+    w.xu = (float)ax;
+    w.xv = (float)bx;
+    w.y0 = (float)(-(ay * hx + by * hy));
+    w.yu = (float)ay;
+    w.yv = (float)by;
+    return w;
+}
+
+// warp_region's contract with positions from a RegionWarp. A pixel is valid when its source
+// cell floor(x) lies in [0, w-2] and floor(y) in [0, h-2], which is x, y in [0, w-1) x [0, h-1).
+template <int NB, int ST, bool BUF>
+__device__ __forceinline__ void warp_region_rel(float *__restrict__ L, const RegionWarp &w,
+                                                int RW, int npx, int px0, int py0, int step_x,
+                                                int step_y, const TexRef<ST, BUF> &tex) {
+    const uint32_t sw = tex.w;
+    const uint32_t xmax = tex.w - 2u, ymax = tex.h - 2u;
+    int px = px0, py = py0;
+    for (int q0 = (int)threadIdx.x; q0 < npx; q0 += NB * kBlock) {
+        uint32_t idx[NB];
+        float tx[NB], ty[NB];
+        bool ok[NB];
+#pragma unroll
+        for (int j = 0; j < NB; ++j) {
+            const bool in = q0 + j * kBlock < npx;
+            const float u = (float)px, v = (float)py;
+            const float rinv = __builtin_amdgcn_rcpf(fmaf(w.gu, u, fmaf(w.gv, v, w.g0)));
+            const float qx = fmaf(fmaf(w.xu, u, fmaf(w.xv, v, w.x0)), rinv, w.fxc);
+            const float qy = fmaf(fmaf(w.yu, u, fmaf(w.yv, v, w.y0)), rinv, w.fyc);
+            const int ix = w.ixc + floor_to_int(qx), iy = w.iyc + floor_to_int(qy);
+            const bool valid = in && (uint32_t)ix <= xmax && (uint32_t)iy <= ymax;
+            ok[j] = valid;
+            tx[j] = __builtin_amdgcn_fractf(qx);
+            ty[j] = __builtin_amdgcn_fractf(qy);
+            idx[j] = valid ? (uint32_t)iy * sw + (uint32_t)ix : 0u;
             px += step_x;
             py += step_y;
             const bool wrap = px >= RW;
@@ -893,11 +1005,26 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
             for (int j = 0; j < HC; ++j) {
                 const int hyp = hc0 + j;
                 if (hyp < h1) {  // uniform across the workgroup
-                    double h[9];
-                    load_h(pd.Hp + ((size_t)s * D + hyp) * 9, h);
+                    const double *Hs = pd.Hp + ((size_t)s * D + hyp) * 9;
                     float *L = lds + buf * npx;
-                    warp_region<warp_batch<R>(), ST, false>(L, h, X0, Y0, RW, npx, px0, py0,
-                                                            step_x, step_y, tex);
+                    // The homography is reloaded for the rare FP64 step rather than kept
+                    // live, and that step gathers two pixels at a time: both keep the
+                    // kernel's registers at what the FP32 step needs.
+                    RegionWarp rw;
+                    {
+                        double h[9];
+                        load_h(Hs, h);
+                        rw = region_warp(h, X0, Y0, RW, RH, tex.w, tex.h);
+                    }
+                    if (rw.ok) {
+                        warp_region_rel<warp_batch<R>(), ST, false>(L, rw, RW, npx, px0, py0,
+                                                                    step_x, step_y, tex);
+                    } else {
+                        double h[9];
+                        load_h(Hs, h);
+                        warp_region<2, ST, false>(L, h, X0, Y0, RW, npx, px0, py0, step_x,
+                                                  step_y, tex);
+                    }
                     __syncthreads();
                     float c = kInfF;
                     if (active) c = lds_patch_cost<R, COST>(mine + buf * npx, RW, rp, st);

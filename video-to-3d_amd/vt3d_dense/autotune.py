@@ -1,7 +1,9 @@
 """Launch-shape autotuner for batches of dense plane-sweep solves.
 
-The knobs it searches are the ones that change how a batch maps onto the 256 CUs and their
-LDS, never what is computed (every candidate gives bit-identical results):
+The knobs it searches change how a batch maps onto the 256 CUs and their LDS, not what is
+computed. Candidates agree within FP32 rounding: the tiled sweep warps each tile's region
+about the region's centre, and the direct sweep about each patch's centre, so the kernel
+and tile width move the last bits of a cost, while hypothesis blocking changes none:
 
 * the primary-sweep kernel: the LDS-tiled sweep (neighbouring patches share warped source
   pixels) or the direct sweep (one patch per lane, gathers straight from the texture);
