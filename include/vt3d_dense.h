@@ -343,6 +343,75 @@ int32_t vt3d_dense_estimate(const vt3d_frame *frames, size_t n_frames,
                             vt3d_multi_result **out);
 void vt3d_multi_result_free(vt3d_multi_result *result);
 
+/* ---- mesh consumer: reference-grid triangulation of the dense points -------------------- */
+
+/* MeshTriangle (mesh_legacy.rs:17-23): indices into the dense points. */
+typedef struct vt3d_mesh_triangle {
+    uint64_t a, b, c;
+    float confidence;
+    uint32_t pad;
+} vt3d_mesh_triangle;
+
+/* MeshReferencePatchStats (mesh.rs:14-27); skip_reason "" when has_skip_reason is 0. */
+typedef struct vt3d_mesh_patch_stats {
+    uint64_t reference_frame;
+    uint64_t point_count;
+    int32_t attempted;
+    int32_t has_skip_reason;
+    char skip_reason[128];
+    uint64_t grid_vertices;
+    uint64_t rejected_grid_vertices;
+    uint64_t candidate_cells;
+    uint64_t candidate_triangles;
+    uint64_t accepted_triangles;
+    uint64_t rejected_discontinuities;
+    uint64_t rejected_degenerate;
+} vt3d_mesh_patch_stats;
+
+/* MeshStats (mesh.rs:29-43). */
+typedef struct vt3d_mesh_stats {
+    int32_t attempted;
+    int32_t has_skip_reason;
+    char skip_reason[128];
+    int32_t has_reference_frame;
+    int32_t pad;
+    uint64_t reference_frame;
+    uint64_t grid_vertices;
+    uint64_t rejected_grid_vertices;
+    uint64_t candidate_cells;
+    uint64_t candidate_triangles;
+    uint64_t accepted_triangles;
+    uint64_t rejected_discontinuities;
+    uint64_t rejected_degenerate;
+    size_t n_reference_frames;
+    uint64_t reference_frames[VT3D_MAX_REFERENCE_VIEWS];
+    size_t n_patches;
+    vt3d_mesh_patch_stats patches[VT3D_MAX_REFERENCE_VIEWS];
+} vt3d_mesh_stats;
+
+typedef struct vt3d_mesh_result {
+    vt3d_mesh_stats stats;
+    size_t n_triangles;
+    vt3d_mesh_triangle *triangles;
+} vt3d_mesh_result;
+
+/* Replaces reconstruct_dense_mesh_with_control (mesh.rs:77-215) with the reference-grid
+ * triangulation of mesh_legacy.rs:150-358 on the device. `dense` carries attempted,
+ * reference_frame, grid_stride and grid_border of the analysis; `patches` (n_patches 0..3) are
+ * its reference patches (vt3d_multi_result.patches), none for a single-reference analysis.
+ * The poll function is called as often, and at the same points of the work, as the
+ * reference polls; a cancel gives the reference's canceled analysis. width, height and
+ * focal are the reference camera's projection parameters. Free with vt3d_mesh_result_free. */
+int32_t vt3d_dense_mesh(const vt3d_point3 *points, size_t n_points, const vt3d_grid_site *sites,
+                        size_t n_sites, const vt3d_dense_stats *dense,
+                        const vt3d_reference_patch_stats *patches, size_t n_patches,
+                        const vt3d_camera *cameras, size_t n_cameras, uint32_t width,
+                        uint32_t height, double focal, vt3d_poll_fn poll, void *poll_user,
+                        vt3d_mesh_result **out);
+void vt3d_mesh_result_free(vt3d_mesh_result *result);
+/* mesh_warning (reconstruction/warnings.rs:5-25): like vt3d_dense_warning. */
+size_t vt3d_mesh_warning(const vt3d_mesh_stats *stats, char *buf, size_t cap);
+
 /* ---- wire formats and diagnostics (host only) ---------------------------------------- */
 
 /* pack_dense_points (crates/video-to-3d-wasm/src/lib.rs:227-242): n points as little-endian

@@ -9,6 +9,7 @@
 #include "vt3d_oracle.h"
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
@@ -1787,6 +1788,386 @@ size_t vt3d_oracle_merge_attempts(const vt3d_reference_attempt *attempts, size_t
     merge_attempts(v, fallback, n_fallback);
     std::copy(v.begin(), v.end(), out);
     return v.size();
+}
+
+}  // extern "C"
+
+// ---- mesh consumer (mesh_legacy.rs:150-556, mesh.rs:77-304) --------------------------------
+
+namespace {
+
+constexpr double kMaxGridOffsetStrides = 0.5;  // MAX_GRID_REPROJECTION_OFFSET_STRIDES, :10
+constexpr double kMaxMeshDepthJump = 0.15;     // MAX_MESH_RELATIVE_DEPTH_JUMP, :11
+constexpr double kMaxBridgeDepthJump = 0.08;   // MAX_BRIDGE_RELATIVE_DEPTH_JUMP, :12
+constexpr double kMaxEdgeFootprint = 3.0;      // MAX_MESH_EDGE_FOOTPRINT_MULTIPLIER, :13
+constexpr double kMinAreaRatio = 0.05;         // MIN_MESH_AREA_FOOTPRINT_RATIO, :14
+constexpr double kMinProjectedAreaRatio = 0.05;  // MIN_MESH_PROJECTED_AREA_FOOTPRINT_RATIO, :15
+
+struct GridVertex {  // mesh_legacy.rs:58-66
+    size_t index;
+    V3 pos;
+    double px, py, depth;
+    float conf;
+};
+
+struct LegacyMesh {  // legacy::MeshAnalysis
+    bool attempted = false;
+    std::string skip;
+    bool has_skip = false;
+    bool has_ref = false;
+    uint64_t ref = 0;
+    uint64_t grid_vertices = 0, rejected_grid_vertices = 0, candidate_cells = 0,
+             candidate_triangles = 0, rejected_disc = 0, rejected_degen = 0;
+    std::vector<vt3d_mesh_triangle> tris;
+};
+
+LegacyMesh mesh_skipped(bool has_ref, uint64_t ref, const char *reason) {
+    LegacyMesh m;
+    m.has_ref = has_ref;
+    m.ref = ref;
+    m.has_skip = true;
+    m.skip = reason;
+    return m;
+}
+
+inline V3 cross(V3 a, V3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+inline double norm2(V3 v) { return v.x * v.x + v.y * v.y + v.z * v.z; }
+
+// accepted_triangle, mesh_legacy.rs:473-537: 0 accepted, 1 discontinuity, 2 degenerate.
+int accept_triangle(const GridVertex &a, const GridVertex &b, const GridVertex &c, double stride,
+                    double focal, double max_jump, vt3d_mesh_triangle *t) {
+    const double d[3] = {a.depth, b.depth, c.depth};
+    for (double x : d)
+        if (!std::isfinite(x) || x <= 0.0) return 1;
+    const double mn = std::min(std::min(d[0], d[1]), d[2]);
+    const double mx = std::max(std::max(d[0], d[1]), d[2]);
+    if ((mx - mn) / mx > max_jump) return 1;
+    const double sa = (b.px - a.px) * (c.py - a.py) - (b.py - a.py) * (c.px - a.px);
+    if (!std::isfinite(sa) || sa <= stride * stride * kMinProjectedAreaRatio) return 2;
+    const double mean = (d[0] + d[1] + d[2]) / 3.0;
+    const double step = mean * stride / focal;
+    const double max_edge = step * 1.4142135623730951 * kMaxEdgeFootprint;  // SQRT_2
+    for (double e : {norm(a.pos - b.pos), norm(b.pos - c.pos), norm(c.pos - a.pos)})
+        if (!std::isfinite(e) || e > max_edge) return 1;
+    const double area = norm(cross(b.pos - a.pos, c.pos - a.pos));
+    if (!std::isfinite(area) || area <= step * step * kMinAreaRatio) return 2;
+    t->a = a.index;
+    t->b = b.index;
+    t->c = c.index;
+    // f32::min ignores a NaN operand; clamp(0, 1)
+    auto fmin_ = [](float x, float y) { return std::isnan(x) ? y : std::isnan(y) ? x : std::min(x, y); };
+    const float m = fmin_(fmin_(a.conf, b.conf), c.conf);
+    t->confidence = std::min(std::max(m, 0.0f), 1.0f);
+    t->pad = 0;
+    return 0;
+}
+
+// reconstruct_dense_mesh_samples, mesh_legacy.rs:150-358; sample i is (points[idx[i]],
+// sites[idx[i]]), and triangle indices are sample positions.
+LegacyMesh legacy_mesh(const vt3d_point3 *points, const vt3d_grid_site *sites,
+                       const std::vector<size_t> &idx, bool attempted, bool has_ref, uint64_t ref,
+                       uint64_t stride_u, uint32_t border, const vt3d_camera *cams, size_t n_cams,
+                       uint32_t w, uint32_t h, double f, Control &ctl) {
+    const size_t n = idx.size();
+    if (!attempted) return mesh_skipped(has_ref, ref, "dense reconstruction was not attempted");
+    if (n < 3)
+        return mesh_skipped(has_ref, ref,
+                            "fewer than three accepted fused dense points are available");
+    if (stride_u == 0)
+        return mesh_skipped(has_ref, ref,
+                            "dense reconstruction did not expose a valid reference-grid stride");
+    if (w == 0 || h == 0 || !std::isfinite(f) || f <= 0.0)
+        return mesh_skipped(
+            has_ref, ref,
+            "the reference camera does not provide finite positive projection parameters");
+    if (!has_ref)
+        return mesh_skipped(false, 0, "dense reconstruction did not select a reference frame");
+    const vt3d_camera *rc = nullptr;
+    for (size_t i = 0; i < n_cams && !rc; ++i)
+        if (cams[i].frame_index == ref) rc = &cams[i];
+    if (!rc)
+        return mesh_skipped(true, ref, "the dense reference frame has no accepted registered camera");
+    const Cam cam = to_cam(*rc);
+    const double stride = (double)stride_u;
+    auto grid_coord = [&](uint32_t value, uint32_t limit, int *g) {
+        // original_grid_coordinate, mesh_legacy.rs:442-451 (limit.saturating_sub(border))
+        const uint32_t hi = limit > border ? limit - border : 0;
+        if (value < border || value >= hi) return false;
+        const uint64_t off = value - border;
+        if (off % stride_u != 0) return false;
+        const uint64_t q = off / stride_u;
+        if (q > (uint64_t)INT32_MAX) return false;
+        *g = (int)q;
+        return true;
+    };
+    std::map<std::pair<int, int>, GridVertex> grid;
+    for (size_t i = 0; i < n; ++i) {
+        if (ctl.is_canceled()) return mesh_skipped(true, ref, "mesh reconstruction canceled");
+        const vt3d_point3 &pt = points[idx[i]];
+        const vt3d_grid_site &site = sites[idx[i]];
+        const V3 pos{(double)pt.x, (double)pt.y, (double)pt.z};
+        double px, py, pz;
+        if (!project(cam, pos, w, h, f, &px, &py, &pz)) continue;
+        // within_original_grid_footprint, mesh_legacy.rs:428-440
+        const double lim = stride * kMaxGridOffsetStrides;
+        if (!(std::fabs(px - (double)site.x) < lim && std::fabs(py - (double)site.y) < lim))
+            continue;
+        int gx, gy;
+        if (!grid_coord(site.x, w, &gx) || !grid_coord(site.y, h, &gy)) continue;
+        const GridVertex v{i, pos, px, py, pz, pt.confidence};
+        auto it = grid.find({gx, gy});
+        if (it == grid.end()) grid.emplace(std::make_pair(gx, gy), v);
+        else if (v.conf > it->second.conf ||
+                 (v.conf == it->second.conf && v.index < it->second.index))  // better_grid_vertex
+            it->second = v;
+    }
+    LegacyMesh m;
+    m.attempted = true;
+    m.has_ref = true;
+    m.ref = ref;
+    m.grid_vertices = grid.size();
+    m.rejected_grid_vertices = n > grid.size() ? n - grid.size() : 0;
+    if (grid.size() < 3) {
+        m.has_skip = true;
+        m.skip = "fewer than three fused dense points align with the reference sampling grid";
+        return m;
+    }
+    auto at = [&](int x, int y) -> const GridVertex * {
+        auto it = grid.find({x, y});
+        return it == grid.end() ? nullptr : &it->second;
+    };
+    // Within one call a point index sits at one grid site, and cells, horizontal and vertical
+    // bridges span different site sets, so no candidate repeats; triangle_keys is kept as the
+    // reference keeps it.
+    std::set<std::array<size_t, 3>> keys;
+    auto evaluate = [&](const GridVertex &a, const GridVertex &b, const GridVertex &c,
+                        double jump) {
+        std::array<size_t, 3> k{a.index, b.index, c.index};
+        std::sort(k.begin(), k.end());
+        if (!keys.insert(k).second) return;
+        ++m.candidate_triangles;
+        vt3d_mesh_triangle t;
+        const int r = accept_triangle(a, b, c, stride, f, jump, &t);
+        if (r == 0) m.tris.push_back(t);
+        else if (r == 1) ++m.rejected_disc;
+        else ++m.rejected_degen;
+    };
+    std::set<std::pair<int, int>> origins;
+    for (const auto &kv : grid)
+        for (int dx : {-1, 0})
+            for (int dy : {-1, 0}) origins.insert({kv.first.first + dx, kv.first.second + dy});
+    for (const auto &o : origins) {
+        if (ctl.is_canceled()) return mesh_skipped(true, ref, "mesh reconstruction canceled");
+        const int gx = o.first, gy = o.second;
+        const GridVertex *tl = at(gx, gy), *tr = at(gx + 1, gy), *bl = at(gx, gy + 1),
+                         *br = at(gx + 1, gy + 1);
+        if ((tl != nullptr) + (tr != nullptr) + (bl != nullptr) + (br != nullptr) < 3) continue;
+        ++m.candidate_cells;
+        // cell_triangles, mesh_legacy.rs:453-471
+        if (tl && tr && bl && br) {
+            if (norm2(tl->pos - br->pos) <= norm2(tr->pos - bl->pos)) {
+                evaluate(*tl, *tr, *br, kMaxMeshDepthJump);
+                evaluate(*tl, *br, *bl, kMaxMeshDepthJump);
+            } else {
+                evaluate(*tl, *tr, *bl, kMaxMeshDepthJump);
+                evaluate(*tr, *br, *bl, kMaxMeshDepthJump);
+            }
+        } else if (tl && tr && bl) {
+            evaluate(*tl, *tr, *bl, kMaxMeshDepthJump);
+        } else if (tl && tr && br) {
+            evaluate(*tl, *tr, *br, kMaxMeshDepthJump);
+        } else if (tl && bl && br) {
+            evaluate(*tl, *br, *bl, kMaxMeshDepthJump);
+        } else {
+            evaluate(*tr, *br, *bl, kMaxMeshDepthJump);
+        }
+    }
+    int mnx = INT32_MAX, mxx = INT32_MIN, mny = INT32_MAX, mxy = INT32_MIN;
+    for (const auto &kv : grid) {
+        mnx = std::min(mnx, kv.first.first);
+        mxx = std::max(mxx, kv.first.first);
+        mny = std::min(mny, kv.first.second);
+        mxy = std::max(mxy, kv.first.second);
+    }
+    if (mxx - mnx >= 2) {
+        for (int gx = mnx; gx <= mxx - 2; ++gx) {
+            if (ctl.is_canceled()) return mesh_skipped(true, ref, "mesh reconstruction canceled");
+            for (int gy = mny; gy <= mxy; ++gy) {
+                // horizontal_gap_bridge, mesh_legacy.rs:370-394
+                const GridVertex *tl = at(gx, gy), *tr = at(gx + 2, gy), *bl = at(gx, gy + 1),
+                                 *br = at(gx + 2, gy + 1), *tm = at(gx + 1, gy),
+                                 *bm = at(gx + 1, gy + 1);
+                if (tl && tr && !tm && bm) {
+                    ++m.candidate_cells;
+                    evaluate(*tl, *tr, *bm, kMaxBridgeDepthJump);
+                } else if (bl && br && tm && !bm) {
+                    ++m.candidate_cells;
+                    evaluate(*bl, *tm, *br, kMaxBridgeDepthJump);
+                }
+            }
+        }
+    }
+    if (mxy - mny >= 2) {
+        for (int gx = mnx; gx <= mxx; ++gx) {
+            if (ctl.is_canceled()) return mesh_skipped(true, ref, "mesh reconstruction canceled");
+            for (int gy = mny; gy <= mxy - 2; ++gy) {
+                // vertical_gap_bridge, mesh_legacy.rs:396-420
+                const GridVertex *tl = at(gx, gy), *tr = at(gx + 1, gy), *bl = at(gx, gy + 2),
+                                 *br = at(gx + 1, gy + 2), *lm = at(gx, gy + 1),
+                                 *rm = at(gx + 1, gy + 1);
+                if (tl && bl && !lm && rm) {
+                    ++m.candidate_cells;
+                    evaluate(*tl, *rm, *bl, kMaxBridgeDepthJump);
+                } else if (tr && br && lm && !rm) {
+                    ++m.candidate_cells;
+                    evaluate(*tr, *br, *lm, kMaxBridgeDepthJump);
+                }
+            }
+        }
+    }
+    return m;
+}
+
+void set_reason(char *dst, size_t cap, const std::string &s) {
+    std::strncpy(dst, s.c_str(), cap - 1);
+    dst[cap - 1] = '\0';
+}
+
+void patch_of_legacy(vt3d_mesh_patch_stats &p, uint64_t ref, uint64_t point_count,
+                     const LegacyMesh &m) {
+    p = vt3d_mesh_patch_stats{};
+    p.reference_frame = ref;
+    p.point_count = point_count;
+    p.attempted = m.attempted;
+    p.has_skip_reason = m.has_skip;
+    if (m.has_skip) set_reason(p.skip_reason, sizeof(p.skip_reason), m.skip);
+    p.grid_vertices = m.grid_vertices;
+    p.rejected_grid_vertices = m.rejected_grid_vertices;
+    p.candidate_cells = m.candidate_cells;
+    p.candidate_triangles = m.candidate_triangles;
+    p.accepted_triangles = m.tris.size();
+    p.rejected_discontinuities = m.rejected_disc;
+    p.rejected_degenerate = m.rejected_degen;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t vt3d_oracle_dense_mesh(const vt3d_point3 *points, size_t n_points,
+                               const vt3d_grid_site *sites, size_t n_sites,
+                               const vt3d_dense_stats *dense,
+                               const vt3d_reference_patch_stats *patches, size_t n_patches,
+                               const vt3d_camera *cameras, size_t n_cameras, uint32_t width,
+                               uint32_t height, double focal, vt3d_poll_fn poll, void *poll_user,
+                               vt3d_mesh_result **out) {
+    if (!out || !dense || n_patches > VT3D_MAX_REFERENCE_VIEWS) return VT3D_E_INVALID;
+    *out = nullptr;
+    Control ctl{poll, poll_user};
+    vt3d_mesh_result *r = (vt3d_mesh_result *)std::calloc(1, sizeof(vt3d_mesh_result));
+    vt3d_mesh_stats &st = r->stats;
+    std::vector<vt3d_mesh_triangle> tris;
+    if (n_patches == 0) {
+        // mesh.rs:87-101 and from_legacy, mesh.rs:236-276
+        LegacyMesh m;
+        const bool has_ref = dense->has_reference_frame != 0;
+        if (dense->attempted && n_points >= 3 && n_points != n_sites) {
+            m = mesh_skipped(has_ref, dense->reference_frame,
+                             "dense point and reference-grid evidence counts disagree");
+        } else {
+            std::vector<size_t> idx(std::min(n_points, n_sites));
+            for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+            m = legacy_mesh(points, sites, idx, dense->attempted != 0, has_ref,
+                            dense->reference_frame, dense->grid_stride, dense->grid_border,
+                            cameras, n_cameras, width, height, focal, ctl);
+        }
+        st.attempted = m.attempted;
+        st.has_skip_reason = m.has_skip;
+        if (m.has_skip) set_reason(st.skip_reason, sizeof(st.skip_reason), m.skip);
+        st.has_reference_frame = m.has_ref;
+        st.reference_frame = m.ref;
+        st.grid_vertices = m.grid_vertices;
+        st.rejected_grid_vertices = m.rejected_grid_vertices;
+        st.candidate_cells = m.candidate_cells;
+        st.candidate_triangles = m.candidate_triangles;
+        st.accepted_triangles = m.tris.size();
+        st.rejected_discontinuities = m.rejected_disc;
+        st.rejected_degenerate = m.rejected_degen;
+        if (m.has_ref) {
+            st.n_reference_frames = 1;
+            st.reference_frames[0] = m.ref;
+            st.n_patches = 1;
+            patch_of_legacy(st.patches[0], m.ref, m.grid_vertices + m.rejected_grid_vertices, m);
+        }
+        tris = m.tris;
+    } else {
+        // mesh.rs:103-215
+        for (size_t k = 0; k < n_patches; ++k) {
+            if (ctl.is_canceled()) {  // MeshAnalysis::default()
+                *r = vt3d_mesh_result{};
+                *out = r;
+                return VT3D_OK;
+            }
+            const vt3d_reference_patch_stats &p = patches[k];
+            vt3d_mesh_patch_stats &ps = st.patches[st.n_patches++];
+            // patch_point_indices, mesh.rs:218-234
+            const uint64_t pe = p.primary_start + p.primary_points;
+            const uint64_t ce = p.completion_start + p.completed_points;
+            const bool ok = n_points == n_sites && pe >= p.primary_start &&
+                            ce >= p.completion_start && pe <= n_points && ce <= n_points;
+            if (!ok) {
+                ps = vt3d_mesh_patch_stats{};
+                ps.reference_frame = p.reference_frame;
+                ps.point_count = p.accepted_points;
+                ps.has_skip_reason = 1;
+                set_reason(ps.skip_reason, sizeof(ps.skip_reason),
+                           "dense patch ranges exceed the accepted point evidence");
+                continue;
+            }
+            std::vector<size_t> idx;
+            for (uint64_t i = p.primary_start; i < pe; ++i) idx.push_back(i);
+            for (uint64_t i = p.completion_start; i < ce; ++i) idx.push_back(i);
+            const LegacyMesh m = legacy_mesh(points, sites, idx, true, true, p.reference_frame,
+                                             p.grid_stride, p.grid_border, cameras, n_cameras,
+                                             width, height, focal, ctl);
+            st.attempted = st.attempted || m.attempted;
+            st.grid_vertices += m.grid_vertices;
+            st.rejected_grid_vertices += m.rejected_grid_vertices;
+            st.candidate_cells += m.candidate_cells;
+            st.candidate_triangles += m.candidate_triangles;
+            st.rejected_discontinuities += m.rejected_disc;
+            st.rejected_degenerate += m.rejected_degen;
+            for (vt3d_mesh_triangle t : m.tris) {
+                t.a = idx[t.a];
+                t.b = idx[t.b];
+                t.c = idx[t.c];
+                tris.push_back(t);
+            }
+            patch_of_legacy(ps, p.reference_frame, idx.size(), m);
+        }
+        st.n_reference_frames = st.n_patches;
+        for (size_t k = 0; k < st.n_patches; ++k) st.reference_frames[k] = st.patches[k].reference_frame;
+        if (tris.empty()) {
+            st.has_skip_reason = 1;
+            set_reason(st.skip_reason, sizeof(st.skip_reason),
+                       "no reference patch produced an accepted surface triangle");
+        }
+        st.has_reference_frame = st.n_patches > 0;
+        st.reference_frame = st.n_patches ? st.reference_frames[0] : 0;
+        st.accepted_triangles = tris.size();
+    }
+    r->n_triangles = tris.size();
+    r->triangles = copy_out(tris);
+    *out = r;
+    return VT3D_OK;
+}
+
+void vt3d_oracle_mesh_result_free(vt3d_mesh_result *r) {
+    if (!r) return;
+    std::free(r->triangles);
+    std::free(r);
 }
 
 }  // extern "C"

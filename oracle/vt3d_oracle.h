@@ -61,6 +61,16 @@ int32_t vt3d_oracle_estimate(const vt3d_frame *frames, size_t n_frames,
                              int32_t threads, vt3d_multi_result **out);
 void vt3d_oracle_multi_result_free(vt3d_multi_result *result);
 
+/* Same contract as vt3d_dense_mesh. Free with vt3d_oracle_mesh_result_free. */
+int32_t vt3d_oracle_dense_mesh(const vt3d_point3 *points, size_t n_points,
+                               const vt3d_grid_site *sites, size_t n_sites,
+                               const vt3d_dense_stats *dense,
+                               const vt3d_reference_patch_stats *patches, size_t n_patches,
+                               const vt3d_camera *cameras, size_t n_cameras, uint32_t width,
+                               uint32_t height, double focal, vt3d_poll_fn poll, void *poll_user,
+                               vt3d_mesh_result **out);
+void vt3d_oracle_mesh_result_free(vt3d_mesh_result *result);
+
 /* Restricts plane_sweep to grid rows [first, first + count) (count 0: all rows), for
  * bounded timing samples. Process-wide; not thread-safe. */
 void vt3d_oracle_set_row_band(int32_t first, int32_t count);

@@ -71,6 +71,15 @@ def library() -> C.CDLL:
                                                P(abi.ReferenceAttempt), C.c_size_t,
                                                P(abi.ReferenceAttempt)]
     lib.vt3d_oracle_merge_attempts.restype = C.c_size_t
+    lib.vt3d_oracle_dense_mesh.argtypes = [P(abi.Point3), C.c_size_t, P(abi.GridSite),
+                                           C.c_size_t, P(abi.DenseStats),
+                                           P(abi.ReferencePatchStats), C.c_size_t,
+                                           P(abi.Camera), C.c_size_t, C.c_uint32, C.c_uint32,
+                                           C.c_double, abi.POLL_FN, C.c_void_p,
+                                           P(P(abi.MeshResult))]
+    lib.vt3d_oracle_dense_mesh.restype = C.c_int32
+    lib.vt3d_oracle_mesh_result_free.argtypes = [P(abi.MeshResult)]
+    lib.vt3d_oracle_mesh_result_free.restype = None
     lib.vt3d_oracle_set_row_band.argtypes = [C.c_int32, C.c_int32]
     lib.vt3d_oracle_set_row_band.restype = None
     lib.vt3d_oracle_result_free.argtypes = [P(abi.DenseResult)]
@@ -226,6 +235,30 @@ def reference_order(frames, cameras, sparse_points, focal: float):
                                               inp.n_cameras, inp.sparse_ptr, inp.n_sparse,
                                               float(focal), buf)
     return [int(buf[i]) for i in range(n)]
+
+
+def mesh(points, grid_sites, dense_stats, cameras, width: int, height: int, focal: float,
+         patches=None, control: Optional[Callable[[], bool]] = None) -> abi.MeshAnalysis:
+    """Oracle of reconstruct_dense_mesh_with_control (mesh.rs:77-215). `points`/`grid_sites`:
+    POINT_DTYPE / SITE_DTYPE arrays; `dense_stats`: an analysis' stats dict; `patches`: its
+    stats["reference_patches"] (None or [] for a single-reference analysis)."""
+    lib = library()
+    pts = np.ascontiguousarray(points, dtype=abi.POINT_DTYPE)
+    sites = np.ascontiguousarray(grid_sites, dtype=abi.SITE_DTYPE)
+    inp = abi.Inputs([], cameras, np.zeros((0, 3)))
+    st = abi.dense_stats_struct(dense_stats)
+    pa = abi.patch_structs(patches or [])
+    out = C.POINTER(abi.MeshResult)()
+    rc = lib.vt3d_oracle_dense_mesh(pts.ctypes.data_as(C.POINTER(abi.Point3)), len(pts),
+                                    sites.ctypes.data_as(C.POINTER(abi.GridSite)), len(sites),
+                                    C.byref(st), pa, len(patches or []), inp.cameras,
+                                    inp.n_cameras, int(width), int(height), float(focal),
+                                    _poll(control), None, C.byref(out))
+    if rc < 0:
+        raise ValueError(f"oracle rejected the input ({rc})")
+    m = abi.unpack_mesh(out.contents)
+    lib.vt3d_oracle_mesh_result_free(out)
+    return m
 
 
 def camera_struct(R, t, frame_index: int = 0) -> abi.Camera:

@@ -50,6 +50,8 @@ STRUCTS = {
     "vt3d_server_config": abi.ServerConfig, "vt3d_completion": abi.Completion,
     "vt3d_server_stats": abi.ServerStats, "vt3d_reference_attempt": abi.ReferenceAttempt,
     "vt3d_reference_patch_stats": abi.ReferencePatchStats, "vt3d_multi_result": abi.MultiResult,
+    "vt3d_mesh_triangle": abi.MeshTriangle, "vt3d_mesh_patch_stats": abi.MeshPatchStats,
+    "vt3d_mesh_stats": abi.MeshStats, "vt3d_mesh_result": abi.MeshResult,
 }
 
 

@@ -166,3 +166,21 @@ def test_python_mirrors(product, oracle):
     w = product.dense_warning(res.stats)
     assert w.startswith(f"Slice 4 dense point fusion accepted {n} fused scene points from "
                         "reference frame 1 using 2 registered source views")
+
+
+def test_mesh_warning_text(product):
+    """reconstruction/warnings.rs:5-25, both branches and None."""
+    base = {"attempted": True, "accepted_triangles": 12, "candidate_triangles": 15,
+            "candidate_cells": 9, "rejected_discontinuities": 2, "rejected_degenerate": 1}
+    assert product.mesh_warning(base) == (
+        "Slice 4 bounded mesh reconstruction accepted 12 triangles from 15 candidate triangles "
+        "across 9 reference-grid cells. It rejected 2 triangles at depth/spatial "
+        "discontinuities and 1 degenerate or orientation-flipped triangles. This is a bounded "
+        "multi-reference, non-watertight surface preview. Browser texture projection is "
+        "appearance-only over accepted per-reference topology; unsupported holes and metric "
+        "scale are not claimed.")
+    assert product.mesh_warning(dict(base, accepted_triangles=0)) == (
+        "Slice 4 bounded mesh reconstruction evaluated 15 candidate triangles across 9 "
+        "reference-grid cells, but no triangle survived the continuity and non-degeneracy "
+        "gates. No surface geometry was invented.")
+    assert product.mesh_warning(dict(base, attempted=False)) is None

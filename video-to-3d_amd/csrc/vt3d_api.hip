@@ -50,6 +50,13 @@ thread_local std::string g_error;
 
 void set_error(const std::string &msg) { g_error = msg; }
 
+}  // namespace
+
+// The message vt3d_last_error() returns on this thread (for the other translation units).
+void report_error(const std::string &msg) { set_error(msg); }
+
+namespace {
+
 #define VT3D_HIP(expr)                                                                     \
     do {                                                                                   \
         hipError_t e_ = (expr);                                                            \

@@ -3,8 +3,8 @@
 // The packed buffers are the ones the reference's WASM adapter hands to JavaScript
 // (crates/video-to-3d-wasm/src/lib.rs:227-270): little-endian f32 (x, y, z, confidence)
 // quadruples with a separate RGB byte array, little-endian u32 (x, y) grid-site pairs, and
-// little-endian f32 (reciprocal_consistency, depth_margin) pairs. The dense warning is the
-// reference's exact text (crates/video-to-3d-core/src/reconstruction/warnings.rs:27-68).
+// little-endian f32 (reciprocal_consistency, depth_margin) pairs. The dense and mesh warnings
+// are the reference's exact text (crates/video-to-3d-core/src/reconstruction/warnings.rs:5-68).
 
 #include <cstdint>
 #include <cstdio>
@@ -139,6 +139,37 @@ size_t vt3d_dense_warning(const vt3d_dense_stats *d, char *buf, size_t cap) {
             "source views, but no sampled pixel passed the texture, photometric-error, and "
             "ambiguity gates. No dense geometry was invented.",
             ref, views);
+    }
+    return n < 0 ? 0 : (size_t)n;
+}
+
+size_t vt3d_mesh_warning(const vt3d_mesh_stats *m, char *buf, size_t cap) {
+    // mesh_warning, reconstruction/warnings.rs:5-25
+    if (!m || !m->attempted) {
+        if (buf && cap) buf[0] = '\0';
+        return 0;
+    }
+    int n;
+    if (m->accepted_triangles > 0) {
+        n = std::snprintf(
+            buf, cap,
+            "Slice 4 bounded mesh reconstruction accepted %llu triangles from %llu candidate "
+            "triangles across %llu reference-grid cells. It rejected %llu triangles at "
+            "depth/spatial discontinuities and %llu degenerate or orientation-flipped "
+            "triangles. This is a bounded multi-reference, non-watertight surface preview. "
+            "Browser texture projection is appearance-only over accepted per-reference "
+            "topology; unsupported holes and metric scale are not claimed.",
+            (unsigned long long)m->accepted_triangles, (unsigned long long)m->candidate_triangles,
+            (unsigned long long)m->candidate_cells,
+            (unsigned long long)m->rejected_discontinuities,
+            (unsigned long long)m->rejected_degenerate);
+    } else {
+        n = std::snprintf(
+            buf, cap,
+            "Slice 4 bounded mesh reconstruction evaluated %llu candidate triangles across %llu "
+            "reference-grid cells, but no triangle survived the continuity and non-degeneracy "
+            "gates. No surface geometry was invented.",
+            (unsigned long long)m->candidate_triangles, (unsigned long long)m->candidate_cells);
     }
     return n < 0 ? 0 : (size_t)n;
 }

@@ -198,4 +198,7 @@ struct Outputs {
     ::vt3d_sample_diag *diag;  // null unless a problem emits samples
 };
 
+// Sets the message vt3d_last_error() returns on the calling thread.
+void report_error(const std::string &msg);
+
 }  // namespace vt3d

@@ -91,6 +91,15 @@ def _bind(lib: C.CDLL) -> None:
     lib.vt3d_unpack_dense_points.restype = C.c_int32
     lib.vt3d_dense_warning.argtypes = [P(abi.DenseStats), C.c_char_p, C.c_size_t]
     lib.vt3d_dense_warning.restype = C.c_size_t
+    lib.vt3d_dense_mesh.argtypes = [P(abi.Point3), C.c_size_t, P(abi.GridSite), C.c_size_t,
+                                    P(abi.DenseStats), P(abi.ReferencePatchStats), C.c_size_t,
+                                    P(abi.Camera), C.c_size_t, C.c_uint32, C.c_uint32,
+                                    C.c_double, abi.POLL_FN, C.c_void_p, P(P(abi.MeshResult))]
+    lib.vt3d_dense_mesh.restype = C.c_int32
+    lib.vt3d_mesh_result_free.argtypes = [P(abi.MeshResult)]
+    lib.vt3d_mesh_result_free.restype = None
+    lib.vt3d_mesh_warning.argtypes = [P(abi.MeshStats), C.c_char_p, C.c_size_t]
+    lib.vt3d_mesh_warning.restype = C.c_size_t
     lib.vt3d_session_create.argtypes = [C.c_int32, P(C.c_void_p)]
     lib.vt3d_session_create.restype = C.c_int32
     lib.vt3d_session_destroy.argtypes = [C.c_void_p]
@@ -139,7 +148,7 @@ EXPORTED_SYMBOLS = (
     "vt3d_server_submit", "vt3d_server_poll", "vt3d_server_get_stats", "vt3d_server_destroy",
     "vt3d_dense_estimate", "vt3d_multi_result_free", "vt3d_pack_dense_points",
     "vt3d_pack_grid_sites", "vt3d_pack_point_attributes", "vt3d_unpack_dense_points",
-    "vt3d_dense_warning",
+    "vt3d_dense_warning", "vt3d_dense_mesh", "vt3d_mesh_result_free", "vt3d_mesh_warning",
 )
 
 
@@ -295,6 +304,48 @@ def dense_warning(stats: dict) -> Optional[str]:
         return None
     buf = C.create_string_buffer(n + 1)
     lib.vt3d_dense_warning(C.byref(st), buf, n + 1)
+    return buf.value.decode()
+
+
+def reconstruct_dense_mesh(points, grid_sites, dense_stats: dict, cameras, width: int,
+                           height: int, focal: float,
+                           control: Optional[Callable[[], bool]] = None) -> "abi.MeshAnalysis":
+    """mesh.rs:77-215 on the GPU: reference-grid triangulation of an analysis' points.
+    `dense_stats` is the analysis' stats dict; its "reference_patches" (a multi-reference
+    estimate's) select the per-patch triangulation. `cameras` as for the sweeps."""
+    lib = library()
+    pts = np.ascontiguousarray(points, dtype=abi.POINT_DTYPE)
+    sites = np.ascontiguousarray(grid_sites, dtype=abi.SITE_DTYPE)
+    inp = abi.Inputs([], cameras, np.zeros((0, 3)))
+    st = abi.dense_stats_struct(dense_stats)
+    patches = dense_stats.get("reference_patches") or []
+    pa = abi.patch_structs(patches)
+    out = C.POINTER(abi.MeshResult)()
+    poll = _poll(control)
+    rc = lib.vt3d_dense_mesh(pts.ctypes.data_as(C.POINTER(abi.Point3)), len(pts),
+                             sites.ctypes.data_as(C.POINTER(abi.GridSite)), len(sites),
+                             C.byref(st), pa, len(patches), inp.cameras, inp.n_cameras,
+                             int(width), int(height), float(focal), poll, None, C.byref(out))
+    _check(rc)
+    try:
+        return abi.unpack_mesh(out.contents)
+    finally:
+        lib.vt3d_mesh_result_free(out)
+
+
+def mesh_warning(mesh_stats: dict) -> Optional[str]:
+    """mesh_warning (reconstruction/warnings.rs:5-25) of a mesh analysis' stats dict."""
+    st = abi.MeshStats()
+    st.attempted = int(bool(mesh_stats["attempted"]))
+    for k in ("accepted_triangles", "candidate_triangles", "candidate_cells",
+              "rejected_discontinuities", "rejected_degenerate"):
+        setattr(st, k, int(mesh_stats[k]))
+    lib = library()
+    n = lib.vt3d_mesh_warning(C.byref(st), None, 0)
+    if n == 0:
+        return None
+    buf = C.create_string_buffer(n + 1)
+    lib.vt3d_mesh_warning(C.byref(st), buf, n + 1)
     return buf.value.decode()
 
 

@@ -195,6 +195,40 @@ class MultiResult(C.Structure):
                 ("omitted_references", C.c_int32), ("fine_grid", C.c_int32)]
 
 
+class MeshTriangle(C.Structure):
+    _fields_ = [("a", C.c_uint64), ("b", C.c_uint64), ("c", C.c_uint64),
+                ("confidence", C.c_float), ("pad", C.c_uint32)]
+
+
+_MESH_COUNTS = ("grid_vertices", "rejected_grid_vertices", "candidate_cells",
+                "candidate_triangles", "accepted_triangles", "rejected_discontinuities",
+                "rejected_degenerate")
+
+
+class MeshPatchStats(C.Structure):
+    _fields_ = [("reference_frame", C.c_uint64), ("point_count", C.c_uint64),
+                ("attempted", C.c_int32), ("has_skip_reason", C.c_int32),
+                ("skip_reason", C.c_char * 128)] + [(n, C.c_uint64) for n in _MESH_COUNTS]
+
+
+class MeshStats(C.Structure):
+    _fields_ = [("attempted", C.c_int32), ("has_skip_reason", C.c_int32),
+                ("skip_reason", C.c_char * 128), ("has_reference_frame", C.c_int32),
+                ("pad", C.c_int32), ("reference_frame", C.c_uint64)] + [
+        (n, C.c_uint64) for n in _MESH_COUNTS] + [
+        ("n_reference_frames", C.c_size_t), ("reference_frames", C.c_uint64 * 3),
+        ("n_patches", C.c_size_t), ("patches", MeshPatchStats * 3)]
+
+
+class MeshResult(C.Structure):
+    _fields_ = [("stats", MeshStats), ("n_triangles", C.c_size_t),
+                ("triangles", C.POINTER(MeshTriangle))]
+
+
+TRIANGLE_DTYPE = np.dtype([("a", "<u8"), ("b", "<u8"), ("c", "<u8"), ("confidence", "<f4"),
+                           ("pad", "<u4")])
+
+
 POLL_FN = C.CFUNCTYPE(C.c_int32, C.c_void_p)
 
 POINT_DTYPE = np.dtype([("x", "<f4"), ("y", "<f4"), ("z", "<f4"), ("confidence", "<f4"),
@@ -306,6 +340,62 @@ def unpack_multi(m: MultiResult, owner=None) -> MultiAnalysis:
     st["reference_attempts"] = [_attempt(m.attempts[i]) for i in range(int(m.n_attempts))]
     st["reference_patches"] = [_patch(m.patches[i]) for i in range(int(m.n_patches))]
     return MultiAnalysis(combined, bool(m.omitted_references), bool(m.fine_grid))
+
+
+@dataclasses.dataclass
+class MeshAnalysis:
+    """MeshAnalysis (crates/video-to-3d-core/src/mesh.rs:45-49): stats dict and triangles."""
+    stats: Dict[str, object]
+    triangles: np.ndarray  # TRIANGLE_DTYPE
+
+
+def _mesh_patch(p: "MeshPatchStats") -> Dict[str, object]:
+    d = {n: int(getattr(p, n)) for n in ("reference_frame", "point_count") + _MESH_COUNTS}
+    d["attempted"] = bool(p.attempted)
+    d["skip_reason"] = p.skip_reason.decode() if p.has_skip_reason else None
+    return d
+
+
+def unpack_mesh(m: "MeshResult") -> MeshAnalysis:
+    st = m.stats
+    d = {n: int(getattr(st, n)) for n in _MESH_COUNTS}
+    d["attempted"] = bool(st.attempted)
+    d["skip_reason"] = st.skip_reason.decode() if st.has_skip_reason else None
+    d["reference_frame"] = int(st.reference_frame) if st.has_reference_frame else None
+    d["reference_frames"] = [int(st.reference_frames[i]) for i in range(int(st.n_reference_frames))]
+    d["reference_patches"] = [_mesh_patch(st.patches[i]) for i in range(int(st.n_patches))]
+    return MeshAnalysis(d, _array(m.triangles, int(m.n_triangles), TRIANGLE_DTYPE))
+
+
+def dense_stats_struct(stats: Dict[str, object]) -> "DenseStats":
+    """The DenseStats fields a mesh call reads, from an analysis' stats dict."""
+    st = DenseStats()
+    st.attempted = int(bool(stats["attempted"]))
+    if stats.get("reference_frame") is not None:
+        st.has_reference_frame = 1
+        st.reference_frame = int(stats["reference_frame"])
+    st.grid_stride = int(stats.get("grid_stride") or 0)
+    st.grid_border = int(stats.get("grid_border") or 0)
+    return st
+
+
+def patch_structs(patches) -> "C.Array":
+    """vt3d_reference_patch_stats[] from stats["reference_patches"] dicts."""
+    arr = (ReferencePatchStats * max(1, len(patches)))()
+    for i, p in enumerate(patches):
+        q = arr[i]
+        for n in ("reference_frame", "primary_start", "primary_points", "completion_start",
+                  "completed_points", "sampled_pixels", "accepted_points",
+                  "reciprocal_consistent_points", "grid_stride", "grid_border"):
+            setattr(q, n, int(p[n]))
+        q.n_source_frames = len(p["source_frames"])
+        for j, f in enumerate(p["source_frames"][:32]):
+            q.source_frames[j] = int(f)
+        if p.get("search_min_depth") is not None:
+            q.has_search_depth = 1
+            q.search_min_depth = float(p["search_min_depth"])
+            q.search_max_depth = float(p["search_max_depth"])
+    return arr
 
 
 class Inputs:
