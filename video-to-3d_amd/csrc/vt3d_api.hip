@@ -41,6 +41,7 @@ void launch_fuse(const Args &a, vt3d_sample_diag *diag, hipStream_t s);
 void launch_compact_points(const Args &a, const Outputs &o, hipStream_t s);
 void launch_occ_init(const Args &a, hipStream_t s);
 void launch_compact_props(const Args &a, hipStream_t s);
+void launch_fill_u32(uint32_t *p, uint32_t n, hipStream_t s);
 void launch_complete_fuse(const Args &a, int pass, hipStream_t s);
 void launch_compact_additions(const Args &a, const Outputs &o, int pass, hipStream_t s);
 
@@ -135,7 +136,7 @@ struct DevView {
 // Device buffers and stage events of one batch pipeline.
 struct Workspace {
     DevBuf prob, src, Hp, Hr, depth_p, depth_r, block_problem;
-    DevBuf site_flag, samples, site_map, partial, decision, recip, out;
+    DevBuf site_flag, samples, site_map, partial, decision, recip, recip_list, recip_count, out;
     DevBuf block_counts, block_offsets, meta;
     DevBuf points, sites, attrs, errors, supports, fobs, diag;
     DevBuf site_pt, prop_site, prop, prop_flag;                 // surface completion
@@ -155,7 +156,7 @@ struct Workspace {
     size_t device_bytes() const {
         const DevBuf *all[] = {&prob, &src, &Hp, &Hr, &depth_p, &depth_r, &block_problem,
                                &site_flag, &samples, &site_map, &partial, &decision, &recip,
-                               &out, &block_counts, &block_offsets, &meta, &points, &sites,
+                               &recip_list, &recip_count, &out, &block_counts, &block_offsets, &meta, &points, &sites,
                                &attrs, &errors, &supports, &fobs, &diag, &site_pt, &prop_site,
                                &prop, &prop_flag, &c_points, &c_sites, &c_attrs, &c_errors,
                                &c_supports};
@@ -910,6 +911,8 @@ int upload_plan(vt3d_batch &b) {
         (rc = ws.partial.ensure(sizeof(Top2) * pl.partial_slots)) ||
         (rc = ws.decision.ensure(sizeof(Decision) * pl.slots)) ||
         (rc = ws.recip.ensure(sizeof(Recip) * pl.recip_slots)) ||
+        (rc = ws.recip_list.ensure(4ull * pl.recip_slots)) ||
+        (rc = ws.recip_count.ensure(4ull * kMaxRecipSources * n)) ||
         (rc = ws.out.ensure(sizeof(SampleOut) * pl.slots)) ||
         (rc = ws.block_counts.ensure(4ull * kCounters * pl.n_blocks)) ||
         (rc = ws.block_offsets.ensure(4ull * kCounters * pl.n_blocks)) ||
@@ -967,6 +970,8 @@ int upload_plan(vt3d_batch &b) {
     a.partial = ws.partial.as<Top2>();
     a.decision = ws.decision.as<Decision>();
     a.recip = ws.recip.as<Recip>();
+    a.recip_list = ws.recip_list.as<uint32_t>();
+    a.recip_count = ws.recip_count.as<uint32_t>();
     a.out = ws.out.as<SampleOut>();
     a.block_counts = ws.block_counts.as<uint32_t>();
     a.block_offsets = ws.block_offsets.as<uint32_t>();
@@ -1015,6 +1020,20 @@ Args batch_args(const vt3d_batch &b) {
     return a;
 }
 
+// The reciprocal check of every problem: list lengths cleared (a fill kernel, so a captured
+// graph holds no memset), the gates and pair lists of every group, then the reverse sweeps.
+void enqueue_recip(const Plan &pl, const Args &a, hipStream_t st) {
+    launch_fill_u32(a.recip_count, a.n_prob * (uint32_t)kMaxRecipSources, st);
+    for (int kind : {3, 1})
+        for (const Group &g : pl.recip_groups) {
+            Args ga = a;
+            ga.group_p0 = g.p0;
+            ga.group_p1 = g.p1;
+            ga.group_items = g.items;
+            launch_group(g.storage, g.cost, g.radius, 0, kind, false, 0, 0, ga, st);
+        }
+}
+
 // Surface-completion pass k of every problem: proposals, their compaction, the reciprocal
 // check of the proposals, fusion and footprint, and the compaction of the completed points.
 int enqueue_pass(vt3d_batch &b, int k) {
@@ -1036,13 +1055,7 @@ int enqueue_pass(vt3d_batch &b, int k) {
     launch_compact_props(a, st);
     Args ra = a;
     ra.n_samples = a.n_props + (size_t)k * n;
-    for (const Group &g : pl.recip_groups) {
-        Args ga = ra;
-        ga.group_p0 = g.p0;
-        ga.group_p1 = g.p1;
-        ga.group_items = g.items;
-        launch_group(g.storage, g.cost, g.radius, 0, 1, false, 0, k, ga, st);
-    }
+    enqueue_recip(pl, ra, st);
     launch_complete_fuse(a, k, st);
     launch_scan(a, stage_complete(k), nullptr, st);
     launch_problem_offsets(a, 2 + k, st);
@@ -1097,13 +1110,7 @@ int enqueue(vt3d_batch &b, hipEvent_t *ev) {
     VT3D_HIP(hipGetLastError());
     if ((rc = mark(3))) return rc;
     launch_decide(a, st);
-    for (const Group &g : pl.recip_groups) {
-        Args ga = a;
-        ga.group_p0 = g.p0;
-        ga.group_p1 = g.p1;
-        ga.group_items = g.items;
-        launch_group(g.storage, g.cost, g.radius, 0, 1, false, 0, 0, ga, st);
-    }
+    enqueue_recip(pl, a, st);
     VT3D_HIP(hipGetLastError());
     if ((rc = mark(4))) return rc;
     launch_fuse(a, b.outs.diag, st);
@@ -1570,7 +1577,8 @@ uint64_t request_bytes(const Prepared &pr, const vt3d_tuning &tuning) {
                               (pr.p.emit_samples ? sizeof(vt3d_sample_diag) : 0);
     const uint64_t tables = sizeof(ProblemDev) + (uint64_t)pr.S * sizeof(SourceDev) +
                             2ull * pr.S * pr.D * 9 * 8 + 8ull * pr.D * (1 + pr.S);
-    return cap * per_slot + cap * n_hb * sizeof(Top2) + cap * (uint64_t)pr.S * sizeof(Recip) +
+    return cap * per_slot + cap * n_hb * sizeof(Top2) +
+           cap * (uint64_t)pr.S * (sizeof(Recip) + 4) + 4ull * kMaxRecipSources +
            blocks * kCounters * 8 + tables + 64 * 4;
 }
 

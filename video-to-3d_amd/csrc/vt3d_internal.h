@@ -148,6 +148,7 @@ struct Proposal {
 
 // Counter stages of a batch's compaction scans: totals[stage][problem][kCounters].
 constexpr int kStageGate = 0;      // [0] texture-gated samples
+constexpr int kMaxRecipSources = kMaxSources;  // stride of the pair-list lengths
 constexpr int kStageFuse = 1;      // [0] accepted, [1..6] reciprocal / fusion counters
 __host__ __device__ constexpr int stage_propose(int pass) { return 2 + 2 * pass; }
 // [0] sent to the reciprocal check, [1] proposals, [2] rejected texture, [3] rejected cross view
@@ -170,6 +171,8 @@ struct Args {
     Top2 *partial;
     Decision *decision;
     Recip *recip;
+    uint32_t *recip_list;           // per Recip slot: the pairs that passed the gates, listed
+    uint32_t *recip_count;          // [n_prob][kMaxRecipSources] list lengths
     SampleOut *out;
     // compaction scans
     uint32_t *block_counts;         // [n_blocks][kCounters]

@@ -1121,15 +1121,77 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
     a.decision[pd.base + i] = d;
 }
 
-// ---- k_recip -------------------------------------------------------------------------------
+// ---- k_recip_pre / k_recip -----------------------------------------------------------------
 
-// One lane per (sample, source); a work item is (problem, block of 256 samples, source),
-// source fastest, so every wave shares the source and its reverse homographies.
-// reciprocal_depth_observations (dense_legacy.rs:795-856): direct cost at the re-projected
-// reference point, projection into the source, the texture gate there, a reverse sweep over
-// the source's landmark envelope against the reference image (estimate_single_view_depth,
-// dense_legacy.rs:887-949), the 8% agreement test and the unprojection of the source pixel
-// at the recovered depth.
+// reciprocal_depth_observations (dense_legacy.rs:795-856) in two launches over the same work
+// items, a work item being (problem, source, block of 256 samples), so every wave shares the
+// source and its reverse homographies. Sources are outer within a problem: each source's list
+// ends in blocks with nothing to do, and with the lists contiguous those idle tails fall into
+// every XCD's contiguous share of the items, not only into the last one's.
+//
+// k_recip_pre runs the per-pair gates: the direct cost at the re-projected reference point
+// (dense_legacy.rs:803-811), the projection into the source and the texture gate there
+// (:812-832). A pair that fails gets its (empty) observation written at once; a pair that
+// passes claims the next entry of its (problem, source) list, one atomic per wave. About 7%
+// of pairs fail, and without the lists their lanes would idle through the reverse sweep.
+//
+// k_recip runs the reverse sweep (estimate_single_view_depth, dense_legacy.rs:887-949) for
+// the listed pairs only, so its waves are dense and the tail blocks of each list leave at
+// once. Each pair's result goes to its own slot: the order of the list changes which pairs
+// share a wave, never a result.
+template <int R, int COST, int ST>
+__global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip_pre(Args a) {
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    const uint32_t item = xcd_chunk(blockIdx.x, gridDim.x);
+    if (item >= a.group_items) return;  // whole block
+    const uint32_t p = problem_of_item<1>(a, item);
+    const ProblemDev &pd = a.prob[p];
+    const uint32_t local = item - pd.recip_item0, nb = pd.cap / kBlock;
+    const int s = (int)(local / nb);
+    const uint32_t i = (local % nb) * kBlock + threadIdx.x;
+    const bool in = i < a.n_samples[p];
+    const SourceDev &src = pd.src[s];
+    bool ok = false;
+    if (in) {
+        const Decision &d = a.decision[pd.base + i];
+        const TexRef<ST, true> src_tex(src.tex.quad, src.tex.w, src.tex.h);
+        ok = d.status == 0 && d.proj_ok;
+        float rp[T];
+        if (ok) ok = sample_patch<R>(pd.ref.luma, pd.ref.w, pd.ref.h, d.xr, d.yr, rp);
+        if (ok) {
+            const RefStats st = ref_stats<COST, T>(rp);
+            double H[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) H[k] = fma(d.dr, src.A[k], src.B[k]);
+            ok = patch_cost<R, COST, ST, true>(rp, st, H, d.xr, d.yr, src_tex) <= pd.max_cost;
+        }
+        double xs = 0.0, ys = 0.0, zs = 0.0;
+        if (ok) ok = project_d(src.R, src.t, d.pos, src.tex.w, src.tex.h, pd.focal, &xs, &ys, &zs);
+        if (ok) ok = sample_patch<R>(src.tex.luma, src.tex.w, src.tex.h, xs, ys, rp);
+        if (ok) {
+            float mn = rp[0], mx = rp[0];
+#pragma unroll
+            for (int k = 1; k < T; ++k) {
+                mn = fminf(mn, rp[k]);
+                mx = fmaxf(mx, rp[k]);
+            }
+            ok = (mx - mn) >= pd.min_contrast;
+        }
+        if (!ok) a.recip[pd.recip_base + (size_t)s * pd.cap + i] = Recip{{0.0, 0.0, 0.0}, 0, -1};
+    }
+    const uint64_t m = __ballot(ok);
+    if (m == 0ull) return;
+    const int leader = __ffsll((unsigned long long)m) - 1;
+    uint32_t base = 0;
+    if (__lane_id() == (unsigned)leader)
+        base = atomicAdd(&a.recip_count[p * kMaxRecipSources + s], (uint32_t)__popcll(m));
+    base = __shfl(base, leader, 64);
+    if (ok)
+        a.recip_list[pd.recip_base + (size_t)s * pd.cap + base +
+                     (uint32_t)__popcll(m & lanemask_lt())] = i;
+}
+
 template <int R, int COST, int ST>
 __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     constexpr int P = 2 * R + 1;
@@ -1138,70 +1200,56 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     if (item >= a.group_items) return;
     const uint32_t p = problem_of_item<1>(a, item);
     const ProblemDev &pd = a.prob[p];
-    const uint32_t local = item - pd.recip_item0;
-    const int s = (int)(local % (uint32_t)pd.S);
-    const uint32_t i = (local / (uint32_t)pd.S) * kBlock + threadIdx.x;
-    if (i >= a.n_samples[p]) return;
-    Recip out{{0.0, 0.0, 0.0}, 0, -1};
+    const uint32_t local = item - pd.recip_item0, nb = pd.cap / kBlock;
+    const int s = (int)(local / nb);
+    const uint32_t e = (local % nb) * kBlock + threadIdx.x;
+    if (e >= a.recip_count[p * kMaxRecipSources + s]) return;
+    const uint32_t i = a.recip_list[pd.recip_base + (size_t)s * pd.cap + e];
     const Decision &d = a.decision[pd.base + i];
     const SourceDev &src = pd.src[s];
-    const TexRef<ST, true> src_tex(src.tex.quad, src.tex.w, src.tex.h);
     const TexRef<ST, true> ref_tex(pd.ref.quad, pd.ref.w, pd.ref.h);
     const int D = pd.D;
     const float maxc = pd.max_cost;
-    bool ok = d.status == 0 && d.proj_ok;
-    float rp[T];
-    if (ok) ok = sample_patch<R>(pd.ref.luma, pd.ref.w, pd.ref.h, d.xr, d.yr, rp);
-    if (ok) {
-        const RefStats st = ref_stats<COST, T>(rp);
-        double H[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) H[k] = fma(d.dr, src.A[k], src.B[k]);
-        const float c = patch_cost<R, COST, ST, true>(rp, st, H, d.xr, d.yr, src_tex);
-        ok = c <= maxc;
-    }
+    // the pair passed k_recip_pre's gates: the projection and the source patch repeat them
+    // with the same arithmetic, so they succeed again
     double xs = 0.0, ys = 0.0, zs = 0.0;
-    if (ok) ok = project_d(src.R, src.t, d.pos, src.tex.w, src.tex.h, pd.focal, &xs, &ys, &zs);
-    if (ok) ok = sample_patch<R>(src.tex.luma, src.tex.w, src.tex.h, xs, ys, rp);
-    if (ok) {
-        float mn = rp[0], mx = rp[0];
-#pragma unroll
-        for (int k = 1; k < T; ++k) {
-            mn = fminf(mn, rp[k]);
-            mx = fmaxf(mx, rp[k]);
+    project_d(src.R, src.t, d.pos, src.tex.w, src.tex.h, pd.focal, &xs, &ys, &zs);
+    float rp[T];
+    sample_patch<R>(src.tex.luma, src.tex.w, src.tex.h, xs, ys, rp);
+    const RefStats st = ref_stats<COST, T>(rp);
+    Recip out{{0.0, 0.0, 0.0}, 0, -1};
+    float best_c = kInfF;
+    int best_h = -1;
+    const double *Hs = pd.Hr + (size_t)s * D * 9;
+    for (int hyp = 0; hyp < D; ++hyp) {
+        double h[9];
+        load_h(Hs + (size_t)hyp * 9, h);
+        const float c = patch_cost<R, COST, ST, true>(rp, st, h, xs, ys, ref_tex);
+        // cost ascending, then depth (hypothesis) ascending: strict < keeps the first
+        if (c <= maxc && c < best_c) {
+            best_c = c;
+            best_h = hyp;
         }
-        ok = (mx - mn) >= pd.min_contrast;
     }
+    bool ok = best_h >= 0;
+    out.rh = best_h;
     if (ok) {
-        const RefStats st = ref_stats<COST, T>(rp);
-        float best_c = kInfF;
-        int best_h = -1;
-        const double *Hs = pd.Hr + (size_t)s * D * 9;
-        for (int hyp = 0; hyp < D; ++hyp) {
-            double h[9];
-            load_h(Hs + (size_t)hyp * 9, h);
-            const float c = patch_cost<R, COST, ST, true>(rp, st, h, xs, ys, ref_tex);
-            // cost ascending, then depth (hypothesis) ascending: strict < keeps the first
-            if (c <= maxc && c < best_c) {
-                best_c = c;
-                best_h = hyp;
-            }
-        }
-        ok = best_h >= 0;
-        out.rh = best_h;
+        const double dh = pd.depth_r[(size_t)s * D + best_h];
+        // reciprocal_depth_agrees, dense_legacy.rs:951-957.
+        ok = isfinite(zs) && isfinite(dh) && zs > 0.0 && dh > 0.0 &&
+             fabs(zs - dh) / fmax(fmax(zs, dh), 1.0e-9) <= pd.recip_rel;
         if (ok) {
-            const double dh = pd.depth_r[(size_t)s * D + best_h];
-            // reciprocal_depth_agrees, dense_legacy.rs:951-957.
-            ok = isfinite(zs) && isfinite(dh) && zs > 0.0 && dh > 0.0 &&
-                 fabs(zs - dh) / fmax(fmax(zs, dh), 1.0e-9) <= pd.recip_rel;
-            if (ok) {
-                unproject_d(src.R, src.t, xs, ys, dh, src.tex.w, src.tex.h, pd.focal, out.pos);
-                ok = isfinite(out.pos[0]) && isfinite(out.pos[1]) && isfinite(out.pos[2]);
-            }
+            unproject_d(src.R, src.t, xs, ys, dh, src.tex.w, src.tex.h, pd.focal, out.pos);
+            ok = isfinite(out.pos[0]) && isfinite(out.pos[1]) && isfinite(out.pos[2]);
         }
     }
     out.ok = ok ? 1 : 0;
     a.recip[pd.recip_base + (size_t)s * pd.cap + i] = out;
+}
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint32_t n) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < n) p[i] = 0u;
 }
 
 // ---- k_fuse --------------------------------------------------------------------------------
@@ -1672,9 +1720,13 @@ void launch_sweep_t(const Args &a, int smax, bool tiled, size_t lds_bytes, hipSt
 }
 
 template <int R, int COST, int ST>
-void launch_recip_t(const Args &a, hipStream_t s) {
-    hipLaunchKernelGGL((k_recip<R, COST, ST>), dim3(xcd_grid(a.group_items)), dim3(kBlock), 0,
-                       s, a);
+void launch_recip_t(const Args &a, bool pre, hipStream_t s) {
+    if (pre)
+        hipLaunchKernelGGL((k_recip_pre<R, COST, ST>), dim3(xcd_grid(a.group_items)),
+                           dim3(kBlock), 0, s, a);
+    else
+        hipLaunchKernelGGL((k_recip<R, COST, ST>), dim3(xcd_grid(a.group_items)), dim3(kBlock),
+                           0, s, a);
 }
 
 template <int R, int COST, int ST>
@@ -1686,13 +1738,14 @@ void launch_propose_t(const Args &a, int smax, int pass, hipStream_t s) {
     else hipLaunchKernelGGL((k_propose<R, COST, ST, 32>), grid, dim3(kBlock), 0, s, a, pass);
 }
 
-// kind: 0 primary sweep, 1 reciprocal kernel, 2 surface-completion proposals.
+// kind: 0 primary sweep, 1 reverse sweep of the listed pairs, 2 surface-completion
+// proposals, 3 reciprocal gates and pair lists (before kind 1).
 template <int COST, int ST>
 bool dispatch_r(int radius, const Args &a, int smax, int kind, bool tiled, size_t lds, int pass,
                 hipStream_t s) {
 #define VT3D_CASE(RR)                                                           \
     case RR:                                                                    \
-        if (kind == 1) launch_recip_t<RR, COST, ST>(a, s);                      \
+        if (kind == 1 || kind == 3) launch_recip_t<RR, COST, ST>(a, kind == 3, s); \
         else if (kind == 2) launch_propose_t<RR, COST, ST>(a, smax, pass, s);   \
         else launch_sweep_t<RR, COST, ST>(a, smax, tiled, lds, s);              \
         return true;
@@ -1737,6 +1790,10 @@ bool launch_group(int storage, int cost, int radius, int smax, int kind, bool ti
         return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, kind, tiled, lds, pass, s);
     default: return false;
     }
+}
+
+void launch_fill_u32(uint32_t *p, uint32_t n, hipStream_t s) {
+    if (n) hipLaunchKernelGGL(k_fill_u32, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, p, n);
 }
 
 bool launch_build_view(int storage, const uint8_t *rgba, uint32_t w, uint32_t h, uint8_t *luma,
