@@ -113,8 +113,9 @@ template <int ST> struct TexRef<ST, true> {
         const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)a);
         const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(a >> 32));
         void *base = reinterpret_cast<void *>(((uint64_t)hi << 32) | lo);
-        rsrc = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)(w_ * h_ * kTexelBytes),
-                                                 (int)kBufferFlags);
+        // every field wave-uniform, so the loads need no loop over distinct descriptors
+        const int records = (int)__builtin_amdgcn_readfirstlane(w_ * h_ * kTexelBytes);
+        rsrc = __builtin_amdgcn_make_buffer_rsrc(base, (short)0, records, (int)kBufferFlags);
     }
     // Coefficients (a, b, c, d) of texel `i` (see Store).
     __device__ __forceinline__ float4 coef(uint32_t i) const { return coef_at(i * kTexelBytes); }
@@ -555,13 +556,24 @@ __device__ __forceinline__ RegionWarp region_warp(const double (&h)[9], int X0, 
 
 // warp_region's contract with positions from a RegionWarp. A pixel is valid when its source
 // cell floor(x) lies in [0, w-2] and floor(y) in [0, h-2], which is x, y in [0, w-1) x [0, h-1).
+typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+
+// Region pixel q's column and row (exact small integers in FP16), written once per workgroup:
+// they are the same for every (source, hypothesis) step, and the warp's mixed-precision FMAs
+// read the halves directly.
+__device__ __forceinline__ void region_coords(half2_t *__restrict__ uv, int RW, int npx) {
+    for (int q = (int)threadIdx.x; q < npx; q += kBlock) {
+        const int y = q / RW;
+        uv[q] = half2_t{(_Float16)(q - y * RW), (_Float16)y};
+    }
+}
+
 template <int NB, int ST, bool BUF>
 __device__ __forceinline__ void warp_region_rel(float *__restrict__ L, const RegionWarp &w,
-                                                int RW, int npx, int px0, int py0, int step_x,
-                                                int step_y, const TexRef<ST, BUF> &tex) {
+                                                const half2_t *__restrict__ uv, int npx,
+                                                const TexRef<ST, BUF> &tex) {
     const uint32_t sw = tex.w;
     const uint32_t xmax = tex.w - 2u, ymax = tex.h - 2u;
-    int px = px0, py = py0;
     for (int q0 = (int)threadIdx.x; q0 < npx; q0 += NB * kBlock) {
         uint32_t idx[NB];
         float tx[NB], ty[NB];
@@ -569,7 +581,8 @@ __device__ __forceinline__ void warp_region_rel(float *__restrict__ L, const Reg
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             const bool in = q0 + j * kBlock < npx;
-            const float u = (float)px, v = (float)py;
+            const half2_t c = uv[in ? q0 + j * kBlock : 0];
+            const float u = (float)c.x, v = (float)c.y;
             const float rinv = __builtin_amdgcn_rcpf(fmaf(w.gu, u, fmaf(w.gv, v, w.g0)));
             const float qx = fmaf(fmaf(w.xu, u, fmaf(w.xv, v, w.x0)), rinv, w.fxc);
             const float qy = fmaf(fmaf(w.yu, u, fmaf(w.yv, v, w.y0)), rinv, w.fyc);
@@ -579,11 +592,6 @@ __device__ __forceinline__ void warp_region_rel(float *__restrict__ L, const Reg
             tx[j] = __builtin_amdgcn_fractf(qx);
             ty[j] = __builtin_amdgcn_fractf(qy);
             idx[j] = valid ? (uint32_t)iy * sw + (uint32_t)ix : 0u;
-            px += step_x;
-            py += step_y;
-            const bool wrap = px >= RW;
-            px = wrap ? px - RW : px;
-            py = wrap ? py + 1 : py;
         }
         float4 k[NB];
 #pragma unroll
@@ -975,6 +983,9 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
                     active ? (float)l[(size_t)(y + dy) * w + (x + dx)] : 0.f;
     }
     const RefStats st = ref_stats<COST, T>(rp);
+    half2_t *uv = reinterpret_cast<half2_t *>(lds + 2 * npx);
+    region_coords(uv, RW, npx);
+    __syncthreads();
     // Region origin (reference pixels) and this thread's walk over the region.
     const int X0 = (int)pd.border + bx * TX * stride - R;
     const int Y0 = (int)pd.border + by * TY * stride - R;
@@ -1000,7 +1011,7 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
             for (int k = 0; k < SMAX; ++k) l[j][k] = kInfF;
         for (int s = 0; s < S; ++s) {
             const SourceDev &src = pd.src[s];
-            const TexRef<ST, false> tex(src.tex.quad, src.tex.w, src.tex.h);
+            const TexRef<ST, true> tex(src.tex.quad, src.tex.w, src.tex.h);
 #pragma unroll
             for (int j = 0; j < HC; ++j) {
                 const int hyp = hc0 + j;
@@ -1017,13 +1028,12 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
                         rw = region_warp(h, X0, Y0, RW, RH, tex.w, tex.h);
                     }
                     if (rw.ok) {
-                        warp_region_rel<warp_batch<R>(), ST, false>(L, rw, RW, npx, px0, py0,
-                                                                    step_x, step_y, tex);
+                        warp_region_rel<warp_batch<R>(), ST, true>(L, rw, uv, npx, tex);
                     } else {
                         double h[9];
                         load_h(Hs, h);
-                        warp_region<2, ST, false>(L, h, X0, Y0, RW, npx, px0, py0, step_x,
-                                                  step_y, tex);
+                        warp_region<2, ST, true>(L, h, X0, Y0, RW, npx, px0, py0, step_x,
+                                                 step_y, tex);
                     }
                     __syncthreads();
                     float c = kInfF;
@@ -1810,11 +1820,13 @@ size_t texel_bytes(int storage) {
     return storage == VT3D_STORAGE_F32 ? 16 : 8;
 }
 
-// LDS bytes of the tiled sweep for a tile width (sites per tile row) and grid stride.
+// LDS bytes of the tiled sweep for a tile width (sites per tile row) and grid stride: two
+// buffers of warped values (FP32) and the region's pixel coordinates (two FP16) per pixel.
 size_t tile_lds_bytes(int tile_w, int stride, int radius) {
     const int P = 2 * radius + 1;
     const int TY = kBlock / tile_w;
-    return 2ull * 4ull * (size_t)((tile_w - 1) * stride + P) * (size_t)((TY - 1) * stride + P);
+    return (2ull * 4ull + 4ull) * (size_t)((tile_w - 1) * stride + P) *
+           (size_t)((TY - 1) * stride + P);
 }
 
 void launch_gate(const Args &a, hipStream_t s) {
