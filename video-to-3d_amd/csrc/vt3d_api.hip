@@ -31,7 +31,7 @@ bool launch_group(int storage, int cost, int radius, int smax, int kind, bool ti
 bool launch_build_view(int storage, const uint8_t *rgba, uint32_t w, uint32_t h, uint8_t *luma,
                        void *quad, hipStream_t s);
 size_t texel_bytes(int storage);
-size_t tile_lds_bytes(int tile_w, int stride, int radius);
+size_t tile_lds_bytes(int tile_w, int stride, int radius, int smax);
 void launch_gate(const Args &a, hipStream_t s);
 void launch_scan(const Args &a, int stage, uint32_t *count_out, hipStream_t s);
 void launch_problem_offsets(const Args &a, int which, hipStream_t s);
@@ -675,7 +675,7 @@ bool wants_tiled(const Prepared &pr, int tile_w) {
     const int P = 2 * pr.R + 1;
     const bool want = pr.p.sweep_kernel == VT3D_KERNEL_TILED ||
                       (pr.p.sweep_kernel == VT3D_KERNEL_AUTO && (int)pr.stride < P);
-    return want && tile_lds_bytes(tile_w, (int)pr.stride, pr.R) <= kMaxTileLds;
+    return want && tile_lds_bytes(tile_w, (int)pr.stride, pr.R, smax_of(pr.S)) <= kMaxTileLds;
 }
 
 // Orders the device problems by kernel instantiation and lays out slots, tables and items.
@@ -772,7 +772,7 @@ int build_plan(const std::vector<Prepared *> &reqs, const vt3d_tuning &tuning, P
                 smax_of(q.S) != g.smax || pl.prob[e].tiled != g.tiled)
                 break;
             ProblemDev &pd = pl.prob[e];
-            if (g.tiled) g.lds = std::max(g.lds, tile_lds_bytes(tile_w, (int)q.stride, q.R));
+            if (g.tiled) g.lds = std::max(g.lds, tile_lds_bytes(tile_w, (int)q.stride, q.R, g.smax));
             pd.sweep_item0 = g.items;
             g.items += (g.tiled ? pd.tiles : pd.cap / kBlock) * (uint32_t)pd.n_hb;
             pd.prop_item0 = g.prop_items;

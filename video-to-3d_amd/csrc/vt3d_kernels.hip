@@ -164,6 +164,14 @@ __device__ __forceinline__ int mad_i24(int a, int b, int c) {
     return r;
 }
 
+// a * b + c with a and b below 2^24 in one full-rate instruction (a 32-bit multiply issues at
+// a quarter of the rate).
+__device__ __forceinline__ uint32_t mad_u24(uint32_t a, uint32_t b, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_u32_u24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(b), "v"(c));
+    return r;
+}
+
 // Bilinear value from a coefficient texel.
 __device__ __forceinline__ float bicoef(float4 k, float tx, float ty) {
     // (a + tx b) + ty (c + tx d): three FMAs (with FP16 texels, three mixed-precision FMAs
@@ -334,14 +342,17 @@ __device__ __forceinline__ Warp make_warp(const double *__restrict__ H, double c
 template <int COST, int T>
 __device__ __forceinline__ float finish_cost(const float (&rp)[T], const RefStats &st, float s1,
                                              float s2, float srs, const float *sv) {
+    // means as products with the rounded 1 / T (within an ulp of the quotient; an IEEE
+    // division costs about ten instructions per patch)
+    constexpr float kInvT = 1.0f / (float)T;
     if constexpr (COST == VT3D_COST_ZSAD) {
-        const float sm = s1 / (float)T;
+        const float sm = s1 * kInvT;
         float acc = 0.f;
 #pragma unroll
         for (int k = 0; k < T; ++k) acc += fabsf(rp[k] - (sv[k] - sm));
-        return acc / (float)T;
+        return acc * kInvT;
     } else {
-        const float m = s1 / (float)T;
+        const float m = s1 * kInvT;
         const float var = fmaf(-s1, m, s2);
         if (st.flat || !(var > kZnccMinVariance * (float)T)) return 1.f;
         const float cov = fmaf(-m, st.sum_hat, srs);
@@ -451,11 +462,21 @@ __device__ __forceinline__ void load_h(const double *__restrict__ H, double (&h)
 // All of a batch's gathers are issued before any is consumed, and an invalid pixel gathers
 // texel 0 (always in bounds) and discards it, so the loop has no branches. The batch is
 // smaller for large patches, whose reference values already fill most registers.
-template <int R> constexpr int warp_batch() { return R >= 5 ? 4 : 6; }
+__host__ __device__ constexpr int warp_batch_of(int R) { return R >= 5 ? 4 : 6; }
+template <int R> constexpr int warp_batch() { return warp_batch_of(R); }
+// The region's pixel count rounded up to whole warp batches of the workgroup.
+__host__ __device__ constexpr int region_pad(int npx, int R) {
+    const int b = warp_batch_of(R) * kBlock;
+    return (npx + b - 1) / b * b;
+}
 
 // Hypotheses per chunk of the tiled sweep: the chunk's sorted source lists (HC * SMAX
 // values) stay within 16 registers (measured: 32 is no faster, 64 spills).
-template <int SMAX> constexpr int hyp_chunk() { return 16 / SMAX > 1 ? 16 / SMAX : 1; }
+__host__ __device__ constexpr int hyp_chunk_of(int smax) { return 16 / smax > 1 ? 16 / smax : 1; }
+template <int SMAX> constexpr int hyp_chunk() { return hyp_chunk_of(SMAX); }
+// Region warps each wave of the tiled sweep keeps in LDS: one per (source, hypothesis) step of
+// a chunk (at most 32, one lane each).
+__host__ __device__ constexpr int warp_table_of(int smax) { return smax * hyp_chunk_of(smax); }
 
 template <int NB, int ST, bool BUF>
 __device__ __forceinline__ void warp_region(float *__restrict__ L, const double (&h)[9], int X0,
@@ -510,13 +531,16 @@ __device__ __forceinline__ void warp_region(float *__restrict__ L, const double 
 // pixel's source depth exceeds 1e-4 (project's test, dense_legacy.rs:1052-1062, passes for all
 // of them) and the FP32 positions lie within about 1e-5 px of the FP64 ones. Otherwise the
 // step takes warp_region.
-struct RegionWarp {
+//
+// 64 bytes: the tiled sweep keeps one per (source, hypothesis) step of a chunk in LDS.
+struct alignas(16) RegionWarp {
     float g0, gu, gv;     // denominator g0 + gu u + gv v
     float x0, xu, xv;     // x numerator
     float y0, yu, yv;     // y numerator
     float fxc, fyc;       // fractional part of the centre's source position
     int ixc, iyc;         // its integer part
-    bool ok;
+    int ok;
+    int pad[2];
 };
 
 __device__ __forceinline__ RegionWarp region_warp(const double (&h)[9], int X0, int Y0, int RW,
@@ -537,6 +561,7 @@ __device__ __forceinline__ RegionWarp region_warp(const double (&h)[9], int X0, 
     const double reach = (fabs(ax) + fabs(bx) + fabs(ay) + fabs(by)) * (hx + hy);
     w.ok = pos && fabs(g) * hx + fabs(e) * hy <= 0.25 && reach < 1.0e5 && xc > -1.0e5 &&
            xc < (double)sw + 1.0e5 && yc > -1.0e5 && yc < (double)sh + 1.0e5;
+    w.pad[0] = w.pad[1] = 0;
     const double fx = w.ok ? floor(xc) : 0.0, fy = w.ok ? floor(yc) : 0.0;
     w.ixc = (int)fx;
     w.iyc = (int)fy;
@@ -560,48 +585,58 @@ typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
 
 // Region pixel q's column and row (exact small integers in FP16), written once per workgroup:
 // they are the same for every (source, hypothesis) step, and the warp's mixed-precision FMAs
-// read the halves directly.
-__device__ __forceinline__ void region_coords(half2_t *__restrict__ uv, int RW, int npx) {
-    for (int q = (int)threadIdx.x; q < npx; q += kBlock) {
-        const int y = q / RW;
-        uv[q] = half2_t{(_Float16)(q - y * RW), (_Float16)y};
+// read the halves directly. The table runs to `npad` (a whole number of warp batches); the
+// pixels past the region repeat pixel 0, so a batch needs no bounds tests, and their values
+// land past the region, where no patch reads.
+__device__ __forceinline__ void region_coords(half2_t *__restrict__ uv, int RW, int npx,
+                                              int npad) {
+    for (int q = (int)threadIdx.x; q < npad; q += kBlock) {
+        const int y = q < npx ? q / RW : 0;
+        const int x = q < npx ? q - y * RW : 0;
+        uv[q] = half2_t{(_Float16)x, (_Float16)y};
     }
 }
 
-template <int NB, int ST, bool BUF>
+// `npad` is a multiple of NB * kBlock (region_pad). The texel offset is formed in bytes: the
+// column times the texel size plus the row times the row pitch in one 24-bit multiply-add
+// (frames are at most 65535 px wide, so the pitch stays below 2^24); an x outside [0, w-2]
+// wraps to a large unsigned value and fails the same test as before scaling.
+template <int NB, int ST>
 __device__ __forceinline__ void warp_region_rel(float *__restrict__ L, const RegionWarp &w,
-                                                const half2_t *__restrict__ uv, int npx,
-                                                const TexRef<ST, BUF> &tex) {
-    const uint32_t sw = tex.w;
-    const uint32_t xmax = tex.w - 2u, ymax = tex.h - 2u;
-    for (int q0 = (int)threadIdx.x; q0 < npx; q0 += NB * kBlock) {
-        uint32_t idx[NB];
+                                                const half2_t *__restrict__ uv, int npad,
+                                                const TexRef<ST, true> &tex) {
+    constexpr uint32_t kTB = TexRef<ST, true>::kTexelBytes;
+    constexpr int kShift = kTB == 16u ? 4 : 3;
+    const uint32_t pitch = tex.w * kTB;
+    const uint32_t xmax = (tex.w - 2u) * kTB, ymax = tex.h - 2u;
+    const uint32_t xc = (uint32_t)w.ixc * kTB;
+    for (int q0 = (int)threadIdx.x; q0 < npad; q0 += NB * kBlock) {
+        uint32_t off[NB];
         float tx[NB], ty[NB];
         bool ok[NB];
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
-            const bool in = q0 + j * kBlock < npx;
-            const half2_t c = uv[in ? q0 + j * kBlock : 0];
+            const half2_t c = uv[q0 + j * kBlock];
             const float u = (float)c.x, v = (float)c.y;
             const float rinv = __builtin_amdgcn_rcpf(fmaf(w.gu, u, fmaf(w.gv, v, w.g0)));
             const float qx = fmaf(fmaf(w.xu, u, fmaf(w.xv, v, w.x0)), rinv, w.fxc);
             const float qy = fmaf(fmaf(w.yu, u, fmaf(w.yv, v, w.y0)), rinv, w.fyc);
-            const int ix = w.ixc + floor_to_int(qx), iy = w.iyc + floor_to_int(qy);
-            const bool valid = in && (uint32_t)ix <= xmax && (uint32_t)iy <= ymax;
+            const uint32_t xb = ((uint32_t)floor_to_int(qx) << kShift) + xc;
+            const uint32_t iy = (uint32_t)(w.iyc + floor_to_int(qy));
+            const bool valid = xb <= xmax && iy <= ymax;
             ok[j] = valid;
             tx[j] = __builtin_amdgcn_fractf(qx);
             ty[j] = __builtin_amdgcn_fractf(qy);
-            idx[j] = valid ? (uint32_t)iy * sw + (uint32_t)ix : 0u;
+            // formed for every pixel and then selected: a conditional asm becomes a branch
+            const uint32_t o = mad_u24(iy, pitch, xb);
+            off[j] = valid ? o : 0u;
         }
         float4 k[NB];
 #pragma unroll
-        for (int j = 0; j < NB; ++j) k[j] = tex.coef(idx[j]);
+        for (int j = 0; j < NB; ++j) k[j] = tex.coef_at(off[j]);
         const float nan = __builtin_nanf("");
 #pragma unroll
-        for (int j = 0; j < NB; ++j) {
-            const int q = q0 + j * kBlock;
-            if (q < npx) L[q] = ok[j] ? bicoef(k[j], tx[j], ty[j]) : nan;
-        }
+        for (int j = 0; j < NB; ++j) L[q0 + j * kBlock] = ok[j] ? bicoef(k[j], tx[j], ty[j]) : nan;
     }
 }
 
@@ -983,8 +1018,14 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
                     active ? (float)l[(size_t)(y + dy) * w + (x + dx)] : 0.f;
     }
     const RefStats st = ref_stats<COST, T>(rp);
-    half2_t *uv = reinterpret_cast<half2_t *>(lds + 2 * npx);
-    region_coords(uv, RW, npx);
+    // LDS: two buffers of warped values and the pixel coordinates, each `npad` words, then
+    // each wave's table of region warps (tile_lds_bytes on the host is the same layout)
+    const int npad = region_pad(npx, R);
+    half2_t *uv = reinterpret_cast<half2_t *>(lds + 2 * npad);
+    constexpr int kTable = warp_table_of(SMAX);
+    RegionWarp *table = reinterpret_cast<RegionWarp *>(lds + 3 * npad) + (threadIdx.x >> 6) * kTable;
+    const int lane = (int)__lane_id();
+    region_coords(uv, RW, npx, npad);
     __syncthreads();
     // Region origin (reference pixels) and this thread's walk over the region.
     const int X0 = (int)pd.border + bx * TX * stride - R;
@@ -1004,6 +1045,22 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
     constexpr int HC = hyp_chunk<SMAX>();
     int buf = 0;
     for (int hc0 = h0; hc0 < h1; hc0 += HC) {
+        // The chunk's region warps (FP64 centre, FP32 coefficients), one per lane of each wave
+        // for step k = s * HC + j, into the wave's own table: the steps below read them from
+        // LDS instead of every lane repeating the FP64 work. Only this wave reads its table,
+        // and a wave's LDS accesses complete in order, so a wave-level fence orders the writes
+        // before the reads (and the reads of the previous chunk before these writes).
+        if (lane < S * HC) {
+            const int ks = lane / HC, hyp = hc0 + lane % HC;
+            if (hyp < h1) {
+                double h[9];
+                load_h(pd.Hp + ((size_t)ks * D + hyp) * 9, h);
+                table[lane] = region_warp(h, X0, Y0, RW, RH, pd.src[ks].tex.w, pd.src[ks].tex.h);
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
         float l[HC][SMAX];
 #pragma unroll
         for (int j = 0; j < HC; ++j)
@@ -1016,28 +1073,21 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
             for (int j = 0; j < HC; ++j) {
                 const int hyp = hc0 + j;
                 if (hyp < h1) {  // uniform across the workgroup
-                    const double *Hs = pd.Hp + ((size_t)s * D + hyp) * 9;
-                    float *L = lds + buf * npx;
-                    // The homography is reloaded for the rare FP64 step rather than kept
-                    // live, and that step gathers two pixels at a time: both keep the
-                    // kernel's registers at what the FP32 step needs.
-                    RegionWarp rw;
-                    {
-                        double h[9];
-                        load_h(Hs, h);
-                        rw = region_warp(h, X0, Y0, RW, RH, tex.w, tex.h);
-                    }
-                    if (rw.ok) {
-                        warp_region_rel<warp_batch<R>(), ST, true>(L, rw, uv, npx, tex);
+                    float *L = lds + buf * npad;
+                    const RegionWarp rw = table[s * HC + j];
+                    if (__builtin_amdgcn_readfirstlane(rw.ok)) {
+                        warp_region_rel<warp_batch<R>(), ST>(L, rw, uv, npad, tex);
                     } else {
+                        // the rare FP64 step gathers two pixels at a time, which keeps the
+                        // kernel's registers at what the FP32 step needs
                         double h[9];
-                        load_h(Hs, h);
+                        load_h(pd.Hp + ((size_t)s * D + hyp) * 9, h);
                         warp_region<2, ST, true>(L, h, X0, Y0, RW, npx, px0, py0, step_x,
                                                  step_y, tex);
                     }
                     __syncthreads();
                     float c = kInfF;
-                    if (active) c = lds_patch_cost<R, COST>(mine + buf * npx, RW, rp, st);
+                    if (active) c = lds_patch_cost<R, COST>(mine + buf * npad, RW, rp, st);
                     c = c <= maxc ? c : kInfF;
                     sorted_insert<SMAX>(l[j], c);
                     buf ^= 1;
@@ -1820,13 +1870,15 @@ size_t texel_bytes(int storage) {
     return storage == VT3D_STORAGE_F32 ? 16 : 8;
 }
 
-// LDS bytes of the tiled sweep for a tile width (sites per tile row) and grid stride: two
-// buffers of warped values (FP32) and the region's pixel coordinates (two FP16) per pixel.
-size_t tile_lds_bytes(int tile_w, int stride, int radius) {
+// LDS bytes of the tiled sweep for a tile width (sites per tile row), grid stride and source
+// bucket: two buffers of warped values (FP32) and the region's pixel coordinates (two FP16)
+// per pixel of the padded region, then each wave's table of region warps.
+size_t tile_lds_bytes(int tile_w, int stride, int radius, int smax) {
     const int P = 2 * radius + 1;
     const int TY = kBlock / tile_w;
-    return (2ull * 4ull + 4ull) * (size_t)((tile_w - 1) * stride + P) *
-           (size_t)((TY - 1) * stride + P);
+    const int npad = region_pad(((tile_w - 1) * stride + P) * ((TY - 1) * stride + P), radius);
+    return (2ull * 4ull + 4ull) * (size_t)npad +
+           (size_t)(kBlock / 64) * (size_t)warp_table_of(smax) * sizeof(RegionWarp);
 }
 
 void launch_gate(const Args &a, hipStream_t s) {
