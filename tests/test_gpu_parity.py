@@ -112,6 +112,44 @@ def test_c2_full_size_band_matches_oracle(product, oracle, kernel):
     _check_counters(g.stats)
 
 
+def _band(g, o):
+    """The GPU result restricted to the grid rows the oracle evaluated."""
+    ys = np.unique(o.samples["y"])
+    gb = copy.copy(g)
+    keep = np.isin(g.samples["y"], ys)
+    gb.samples = g.samples[keep]
+    pk = np.isin(g.grid_sites["y"], ys)
+    gb.points, gb.grid_sites = g.points[pk], g.grid_sites[pk]
+    gb.stats = dict(g.stats, sampled_pixels=int(keep.sum()))
+    return gb, ys
+
+
+@pytest.mark.parametrize("radius", [2, 3, 5])
+def test_stride1_box_sums_match_oracle(product, oracle, radius):
+    """At stride 1 with ZNCC patches of 5 x 5 and up, the tiled sweep takes each patch's source
+    sums from FP64 box sums shared by the tile's overlapping patches. Against the oracle on a
+    band of grid rows, and against the direct kernel (per-patch FP32 sums) on every sample."""
+    c1 = scenes.config_c1()
+    st = dict(c1.settings, patch_radius=radius, depth_hypotheses=48, grid_stride=1,
+              surface_completion_passes=0, emit_samples=1)
+    g = _gpu(product, c1, st, kernel="tiled")
+    oracle.set_row_band(90, 3)
+    try:
+        o, sl = _oracle(oracle, c1, st)
+    finally:
+        oracle.set_row_band(0, 0)
+    gb, ys = _band(g, o)
+    assert len(ys) == 3
+    rep = parity.compare(gb, o, sl, abi.COST_ZNCC)
+    print("stride 1 box sums", radius, rep)
+    assert rep["samples"] > 500 and rep["agree"] >= 0.999
+    _check_counters(g.stats)
+    d = _gpu(product, c1, st, kernel="direct")
+    same = ((d.samples["status"] == g.samples["status"])
+            & (d.samples["best_hypothesis"] == g.samples["best_hypothesis"]))
+    assert len(g.samples) > 20000 and same.mean() >= 0.999
+
+
 def test_c2_is_deterministic_and_storage_independent(product):
     """Luma is an integer 0..255, exact in FP16 and BF16: every storage gives the same bits,
     and repeated solves give the same bits."""

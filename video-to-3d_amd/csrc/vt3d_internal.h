@@ -18,6 +18,9 @@ constexpr int kMaxSources = 32;     // vt3d_sweep_params.max_source_views limit
 constexpr int kMaxRadiusZsad = 3;   // zero-mean SAD keeps P*P warped samples in registers
 constexpr int kMaxRadiusZncc = 5;   // 11x11 ZNCC
 constexpr int kBlock = 256;         // threads per block of the per-sample kernels
+// Primary sweep kernels: one lane per sample; one workgroup per tile of sites warping their
+// shared region into LDS; the same with the ZNCC source sums taken from shared box sums.
+constexpr int kSweepDirect = 0, kSweepTiled = 1, kSweepTiledBox = 2;
 constexpr int kCounters = 8;        // per-block category counters of a compaction scan
 // ZNCC: a patch whose mean squared deviation is below this (intensity^2) has no defined
 // correlation and costs 1 (uncorrelated).

@@ -455,7 +455,7 @@ __device__ __forceinline__ void load_h(const double *__restrict__ H, double (&h)
 }
 
 // Warps NB pixels of a tile's region per thread and stores their bilinear source
-// values into LDS, exactly as patch_error evaluates one tap (FP64 homography, FP64 division
+// values, less the workgroup's `shift`, into LDS, exactly as patch_error evaluates one tap (FP64 homography, FP64 division
 // of the homogeneous coordinates; dense_legacy.rs:1052-1062); NaN marks an invalid tap
 // (source depth <= 1e-4 or outside [0, w-1) x [0, h-1)). Region pixel q (row-major, pitch
 // RW) sits at reference pixel (X0 + q % RW, Y0 + q / RW); thread t handles q = t + j * kBlock.
@@ -481,7 +481,8 @@ __host__ __device__ constexpr int warp_table_of(int smax) { return smax * hyp_ch
 template <int NB, int ST, bool BUF>
 __device__ __forceinline__ void warp_region(float *__restrict__ L, const double (&h)[9], int X0,
                                             int Y0, int RW, int npx, int px0, int py0,
-                                            int step_x, int step_y, const TexRef<ST, BUF> &tex) {
+                                            int step_x, int step_y, const TexRef<ST, BUF> &tex,
+                                            float shift) {
     const uint32_t sw = tex.w;
     const double wlim = (double)(tex.w - 1), hlim = (double)(tex.h - 1);
     int px = px0, py = py0;
@@ -518,7 +519,7 @@ __device__ __forceinline__ void warp_region(float *__restrict__ L, const double 
 #pragma unroll
         for (int j = 0; j < NB; ++j) {
             const int q = q0 + j * kBlock;
-            if (q < npx) L[q] = ok[j] ? bicoef(k[j], tx[j], ty[j]) : nan;
+            if (q < npx) L[q] = ok[j] ? bicoef(k[j], tx[j], ty[j]) - shift : nan;
         }
     }
 }
@@ -604,7 +605,7 @@ __device__ __forceinline__ void region_coords(half2_t *__restrict__ uv, int RW, 
 template <int NB, int ST>
 __device__ __forceinline__ void warp_region_rel(float *__restrict__ L, const RegionWarp &w,
                                                 const half2_t *__restrict__ uv, int npad,
-                                                const TexRef<ST, true> &tex) {
+                                                const TexRef<ST, true> &tex, float shift) {
     constexpr uint32_t kTB = TexRef<ST, true>::kTexelBytes;
     constexpr int kShift = kTB == 16u ? 4 : 3;
     const uint32_t pitch = tex.w * kTB;
@@ -636,7 +637,8 @@ __device__ __forceinline__ void warp_region_rel(float *__restrict__ L, const Reg
         for (int j = 0; j < NB; ++j) k[j] = tex.coef_at(off[j]);
         const float nan = __builtin_nanf("");
 #pragma unroll
-        for (int j = 0; j < NB; ++j) L[q0 + j * kBlock] = ok[j] ? bicoef(k[j], tx[j], ty[j]) : nan;
+        for (int j = 0; j < NB; ++j)
+            L[q0 + j * kBlock] = ok[j] ? bicoef(k[j], tx[j], ty[j]) - shift : nan;
     }
 }
 
@@ -671,6 +673,79 @@ __device__ __forceinline__ float lds_patch_cost(const float *__restrict__ L, int
     }
     if (__builtin_isnan(s1)) return kInfF;
     return finish_cost<COST, T>(rp, st, s1, s2, srs, sv);
+}
+
+// Source-side sums of the tile's ZNCC patches, shared between overlapping patches: for each
+// region row r and site column c, the sum and the sum of squares (FP64) of the P values that
+// the patch of column c reads in row r. A patch then adds P of these per sum instead of
+// reading its P * P taps twice more. In FP64 the sums of the shifted FP32 values are exact up
+// to the squares' last bits, so the variance is at least as accurate as a per-patch FP32 sum
+// around the patch's own reference mean. NaN (an invalid tap) carries through.
+struct BoxWalk {
+    int off0;  // this thread's first entry: its first tap in the region (row-major, pitch RW)
+    int step;  // region words between consecutive entries of one thread
+};
+
+// Entry e = thread + k * kBlock is region row e / TX, site column e % TX; TX divides kBlock
+// (tile widths are 16, 32 or 64), so a thread's column is fixed and its row advances by
+// kBlock / TX per entry.
+__device__ __forceinline__ BoxWalk box_walk(int RW, int TX, int stride) {
+    const int r = (int)threadIdx.x / TX, c = (int)threadIdx.x - r * TX;
+    return BoxWalk{r * RW + c * stride, (kBlock / TX) * RW};
+}
+
+template <int R>
+__device__ __forceinline__ void box_rows(const float *__restrict__ L, int n, const BoxWalk &bw,
+                                         double2 *__restrict__ H) {
+    constexpr int P = 2 * R + 1;
+    int off = bw.off0;
+    for (int e = (int)threadIdx.x; e < n; e += kBlock, off += bw.step) {
+        const float *row = L + off;
+        double a = (double)row[0];
+        double b = a * a;
+#pragma unroll
+        for (int dx = 1; dx < P; ++dx) {
+            const double v = (double)row[dx];
+            a += v;
+            b = fma(v, v, b);
+        }
+        H[e] = make_double2(a, b);
+    }
+}
+
+// ZNCC cost of one patch: its source sums from the box sums (`Hp`: the entry of the patch's
+// first row and column, rows TX entries apart) and its own taps for the cross term. The LDS
+// values are the source values less a shift that is the same for every tap of the patch,
+// which leaves the mean-free sums unchanged.
+template <int R>
+__device__ __forceinline__ float lds_zncc_cost(const float *__restrict__ L, int pitch,
+                                               const double2 *__restrict__ Hp, int TX,
+                                               const float (&rp)[(2 * R + 1) * (2 * R + 1)],
+                                               const RefStats &st) {
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    // entries are 16-byte aligned (one 128-bit LDS read each)
+    const double2 *H = static_cast<const double2 *>(__builtin_assume_aligned(Hp, 16));
+    double s1 = H[0].x, s2 = H[0].y;
+#pragma unroll
+    for (int dy = 1; dy < P; ++dy) {
+        const double2 h = H[dy * TX];
+        s1 += h.x;
+        s2 += h.y;
+    }
+    float srs = 0.f;
+#pragma unroll
+    for (int dy = 0; dy < P; ++dy) {
+        const float *row = L + dy * pitch;
+#pragma unroll
+        for (int dx = 0; dx < P; ++dx) srs = fmaf(rp[dy * P + dx], row[dx], srs);
+    }
+    if (__builtin_isnan(s1)) return kInfF;
+    const double m = s1 * (1.0 / (double)T);
+    const double var = fma(-s1, m, s2);
+    if (st.flat || !(var > (double)(kZnccMinVariance * (float)T))) return 1.f;
+    const float cov = fmaf(-(float)m, st.sum_hat, srs);
+    return 1.f - cov * rsqrtf((float)var);
 }
 
 // ---- XCD-aware block order -----------------------------------------------------------------
@@ -983,8 +1058,9 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_sweep(Args a) {
 // A work item is (problem, tile, hypothesis block), hypothesis block fastest, taken in XCD
 // chunks: each XCD sweeps a contiguous strip of tiles over all hypotheses, so its L2 holds
 // one strip's source footprints instead of the whole image's.
-template <int R, int COST, int ST, int SMAX>
+template <int R, int COST, int ST, int SMAX, bool BOX>
 __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
+    static_assert(!BOX || COST == VT3D_COST_ZNCC, "box sums serve the ZNCC cost");
     extern __shared__ float lds[];
     constexpr int P = 2 * R + 1;
     constexpr int T = P * P;
@@ -1018,15 +1094,41 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
                     active ? (float)l[(size_t)(y + dy) * w + (x + dx)] : 0.f;
     }
     const RefStats st = ref_stats<COST, T>(rp);
-    // LDS: two buffers of warped values and the pixel coordinates, each `npad` words, then
-    // each wave's table of region warps (tile_lds_bytes on the host is the same layout)
+    // LDS: two buffers of warped values and the pixel coordinates, each `npad` words, each
+    // wave's table of region warps, then (with box sums) the row sums, RH x TX; tile_lds_bytes
+    // on the host is the same layout.
     const int npad = region_pad(npx, R);
     half2_t *uv = reinterpret_cast<half2_t *>(lds + 2 * npad);
     constexpr int kTable = warp_table_of(SMAX);
     RegionWarp *table = reinterpret_cast<RegionWarp *>(lds + 3 * npad) + (threadIdx.x >> 6) * kTable;
+    double2 *rows = reinterpret_cast<double2 *>(lds + 3 * npad +
+                                                (kBlock / 64) * kTable * (int)(sizeof(RegionWarp) / 4));
     const int lane = (int)__lane_id();
+    // With box sums the values in LDS are the source values less the mean of the workgroup's
+    // active reference means, so the cross term sums values near zero.
+    __shared__ float2 wg_mean[kBlock / 64];
+    if constexpr (BOX) {
+        float sm = active ? st.mean : 0.f, cnt = active ? 1.f : 0.f;
+#pragma unroll
+        for (int off = 32; off >= 1; off >>= 1) {
+            sm += __shfl_xor(sm, off, 64);
+            cnt += __shfl_xor(cnt, off, 64);
+        }
+        if (lane == 0) wg_mean[threadIdx.x >> 6] = make_float2(sm, cnt);
+    }
     region_coords(uv, RW, npx, npad);
     __syncthreads();
+    const BoxWalk bw = box_walk(RW, TX, stride);
+    float shift = 0.f;
+    if constexpr (BOX) {
+        float sm = 0.f, cnt = 0.f;
+#pragma unroll
+        for (int w = 0; w < kBlock / 64; ++w) {
+            sm += wg_mean[w].x;
+            cnt += wg_mean[w].y;
+        }
+        shift = cnt > 0.f ? sm / cnt : 0.f;
+    }
     // Region origin (reference pixels) and this thread's walk over the region.
     const int X0 = (int)pd.border + bx * TX * stride - R;
     const int Y0 = (int)pd.border + by * TY * stride - R;
@@ -1076,18 +1178,28 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
                     float *L = lds + buf * npad;
                     const RegionWarp rw = table[s * HC + j];
                     if (__builtin_amdgcn_readfirstlane(rw.ok)) {
-                        warp_region_rel<warp_batch<R>(), ST>(L, rw, uv, npad, tex);
+                        warp_region_rel<warp_batch<R>(), ST>(L, rw, uv, npad, tex, shift);
                     } else {
                         // the rare FP64 step gathers two pixels at a time, which keeps the
                         // kernel's registers at what the FP32 step needs
                         double h[9];
                         load_h(pd.Hp + ((size_t)s * D + hyp) * 9, h);
                         warp_region<2, ST, true>(L, h, X0, Y0, RW, npx, px0, py0, step_x,
-                                                 step_y, tex);
+                                                 step_y, tex, shift);
                     }
                     __syncthreads();
                     float c = kInfF;
-                    if (active) c = lds_patch_cost<R, COST>(mine + buf * npad, RW, rp, st);
+                    if constexpr (BOX) {
+                        // the row sums of this step; the rows of the step before were read
+                        // before the barrier above
+                        box_rows<R>(L, RH * TX, bw, rows);
+                        __syncthreads();
+                        if (active)
+                            c = lds_zncc_cost<R>(mine + buf * npad, RW,
+                                                 rows + (ly * stride) * TX + lx, TX, rp, st);
+                    } else {
+                        if (active) c = lds_patch_cost<R, COST>(mine + buf * npad, RW, rp, st);
+                    }
                     c = c <= maxc ? c : kInfF;
                     sorted_insert<SMAX>(l[j], c);
                     buf ^= 1;
@@ -1761,22 +1873,28 @@ void launch_build_view_t(const uint8_t *rgba, uint32_t w, uint32_t h, uint8_t *l
 
 static uint32_t xcd_grid(uint32_t items) { return (items + kXcds - 1) / kXcds * kXcds; }
 
+// mode: VT3D_SWEEP_DIRECT, VT3D_SWEEP_TILED or VT3D_SWEEP_TILED_BOX (ZNCC only).
 template <int R, int COST, int ST, int SMAX>
-void launch_sweep_s(const Args &a, bool tiled, size_t lds_bytes, hipStream_t s) {
+void launch_sweep_s(const Args &a, int mode, size_t lds_bytes, hipStream_t s) {
     const uint32_t grid = xcd_grid(a.group_items);
-    if (tiled)
-        hipLaunchKernelGGL((k_sweep_tile<R, COST, ST, SMAX>), dim3(grid), dim3(kBlock), lds_bytes,
-                           s, a);
-    else
+    if (mode == kSweepTiledBox) {
+        if constexpr (COST == VT3D_COST_ZNCC)
+            hipLaunchKernelGGL((k_sweep_tile<R, COST, ST, SMAX, true>), dim3(grid), dim3(kBlock),
+                               lds_bytes, s, a);
+    } else if (mode == kSweepTiled) {
+        hipLaunchKernelGGL((k_sweep_tile<R, COST, ST, SMAX, false>), dim3(grid), dim3(kBlock),
+                           lds_bytes, s, a);
+    } else {
         hipLaunchKernelGGL((k_sweep<R, COST, ST, SMAX>), dim3(grid), dim3(kBlock), 0, s, a);
+    }
 }
 
 template <int R, int COST, int ST>
-void launch_sweep_t(const Args &a, int smax, bool tiled, size_t lds_bytes, hipStream_t s) {
-    if (smax <= 4) launch_sweep_s<R, COST, ST, 4>(a, tiled, lds_bytes, s);
-    else if (smax <= 8) launch_sweep_s<R, COST, ST, 8>(a, tiled, lds_bytes, s);
-    else if (smax <= 16) launch_sweep_s<R, COST, ST, 16>(a, tiled, lds_bytes, s);
-    else launch_sweep_s<R, COST, ST, 32>(a, tiled, lds_bytes, s);
+void launch_sweep_t(const Args &a, int smax, int mode, size_t lds_bytes, hipStream_t s) {
+    if (smax <= 4) launch_sweep_s<R, COST, ST, 4>(a, mode, lds_bytes, s);
+    else if (smax <= 8) launch_sweep_s<R, COST, ST, 8>(a, mode, lds_bytes, s);
+    else if (smax <= 16) launch_sweep_s<R, COST, ST, 16>(a, mode, lds_bytes, s);
+    else launch_sweep_s<R, COST, ST, 32>(a, mode, lds_bytes, s);
 }
 
 template <int R, int COST, int ST>
@@ -1801,13 +1919,13 @@ void launch_propose_t(const Args &a, int smax, int pass, hipStream_t s) {
 // kind: 0 primary sweep, 1 reverse sweep of the listed pairs, 2 surface-completion
 // proposals, 3 reciprocal gates and pair lists (before kind 1).
 template <int COST, int ST>
-bool dispatch_r(int radius, const Args &a, int smax, int kind, bool tiled, size_t lds, int pass,
+bool dispatch_r(int radius, const Args &a, int smax, int kind, int mode, size_t lds, int pass,
                 hipStream_t s) {
 #define VT3D_CASE(RR)                                                           \
     case RR:                                                                    \
         if (kind == 1 || kind == 3) launch_recip_t<RR, COST, ST>(a, kind == 3, s); \
         else if (kind == 2) launch_propose_t<RR, COST, ST>(a, smax, pass, s);   \
-        else launch_sweep_t<RR, COST, ST>(a, smax, tiled, lds, s);              \
+        else launch_sweep_t<RR, COST, ST>(a, smax, mode, lds, s);               \
         return true;
     if constexpr (COST == VT3D_COST_ZSAD) {
         switch (radius) {
@@ -1830,24 +1948,26 @@ bool dispatch_r(int radius, const Args &a, int smax, int kind, bool tiled, size_
 }
 
 template <int ST>
-bool dispatch_cost(int cost, int radius, const Args &a, int smax, int kind, bool tiled,
+bool dispatch_cost(int cost, int radius, const Args &a, int smax, int kind, int mode,
                    size_t lds, int pass, hipStream_t s) {
-    if (cost == VT3D_COST_ZSAD)
-        return dispatch_r<VT3D_COST_ZSAD, ST>(radius, a, smax, kind, tiled, lds, pass, s);
-    return dispatch_r<VT3D_COST_ZNCC, ST>(radius, a, smax, kind, tiled, lds, pass, s);
+    if (cost == VT3D_COST_ZSAD) {
+        if (mode == kSweepTiledBox) return false;
+        return dispatch_r<VT3D_COST_ZSAD, ST>(radius, a, smax, kind, mode, lds, pass, s);
+    }
+    return dispatch_r<VT3D_COST_ZNCC, ST>(radius, a, smax, kind, mode, lds, pass, s);
 }
 
 // One launch of the primary sweep (kind 0), the reciprocal kernel (1) or the completion
 // proposals of pass `pass` (2) over the problem group a.group_p0 .. a.group_p1.
-bool launch_group(int storage, int cost, int radius, int smax, int kind, bool tiled,
+bool launch_group(int storage, int cost, int radius, int smax, int kind, int mode,
                   size_t lds, int pass, const Args &a, hipStream_t s) {
     switch (storage) {
     case VT3D_STORAGE_F32:
-        return dispatch_cost<VT3D_STORAGE_F32>(cost, radius, a, smax, kind, tiled, lds, pass, s);
+        return dispatch_cost<VT3D_STORAGE_F32>(cost, radius, a, smax, kind, mode, lds, pass, s);
     case VT3D_STORAGE_F16:
-        return dispatch_cost<VT3D_STORAGE_F16>(cost, radius, a, smax, kind, tiled, lds, pass, s);
+        return dispatch_cost<VT3D_STORAGE_F16>(cost, radius, a, smax, kind, mode, lds, pass, s);
     case VT3D_STORAGE_BF16:
-        return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, kind, tiled, lds, pass, s);
+        return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, kind, mode, lds, pass, s);
     default: return false;
     }
 }
@@ -1872,13 +1992,16 @@ size_t texel_bytes(int storage) {
 
 // LDS bytes of the tiled sweep for a tile width (sites per tile row), grid stride and source
 // bucket: two buffers of warped values (FP32) and the region's pixel coordinates (two FP16)
-// per pixel of the padded region, then each wave's table of region warps.
-size_t tile_lds_bytes(int tile_w, int stride, int radius, int smax) {
+// per pixel of the padded region, each wave's table of region warps, and the ZNCC row sums
+// (two FP64 per region row and site column) when the sweep uses box sums.
+size_t tile_lds_bytes(int tile_w, int stride, int radius, int smax, bool box) {
     const int P = 2 * radius + 1;
     const int TY = kBlock / tile_w;
-    const int npad = region_pad(((tile_w - 1) * stride + P) * ((TY - 1) * stride + P), radius);
+    const int RH = (TY - 1) * stride + P;
+    const int npad = region_pad(((tile_w - 1) * stride + P) * RH, radius);
     return (2ull * 4ull + 4ull) * (size_t)npad +
-           (size_t)(kBlock / 64) * (size_t)warp_table_of(smax) * sizeof(RegionWarp);
+           (size_t)(kBlock / 64) * (size_t)warp_table_of(smax) * sizeof(RegionWarp) +
+           (box ? (size_t)RH * (size_t)tile_w * 16ull : 0ull);
 }
 
 void launch_gate(const Args &a, hipStream_t s) {
