@@ -25,8 +25,9 @@ def _same(a, b, what):
 
 def _mix(product):
     """Six batched-config keyframes (three frame sizes, 8 sources, 96 hypotheses, 7x7 ZNCC),
-    the plain CPU-size config (4 sources, 64 hypotheses), the 2-source ZSAD plane fixture on
-    the plain-sweep entry, a remap that returns None and a skipped solve."""
+    the plain CPU-size config (4 sources, 64 hypotheses), the same at stride 1 with 5x5
+    patches (the box-sum sweep, a launch group of its own), the 2-source ZSAD plane fixture
+    on the plain-sweep entry, a remap that returns None and a skipped solve."""
     c3 = scenes.config_c3(count=6)
     c1 = scenes.config_c1()
     pf = scenes.plane_fixture()
@@ -35,6 +36,10 @@ def _mix(product):
         jobs.append(dict(frames=p.frames, cameras=p.cameras, sparse_points=p.sparse,
                          focal=p.focal, reference_camera_index=p.reference,
                          settings=dict(p.settings)))
+    jobs.append(dict(frames=c1.frames, cameras=c1.cameras, sparse_points=c1.sparse,
+                     focal=c1.focal, reference_camera_index=c1.reference,
+                     settings=dict(c1.settings, grid_stride=1, patch_radius=2,
+                                   depth_hypotheses=32)))
     ref = {"preset": "reference"}
     jobs.append(dict(frames=pf.frames, cameras=pf.cameras, sparse_points=pf.sparse,
                      focal=pf.focal, reference_camera_index=-1, settings=ref))
