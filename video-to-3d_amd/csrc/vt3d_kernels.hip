@@ -541,11 +541,13 @@ struct alignas(16) RegionWarp {
     float fxc, fyc;       // fractional part of the centre's source position
     int ixc, iyc;         // its integer part
     int ok;
-    int pad[2];
+    int inside;  // every pixel's source position lies in [1, w-2) x [1, h-2)
+    int pad;
 };
 
 __device__ __forceinline__ RegionWarp region_warp(const double (&h)[9], int X0, int Y0, int RW,
-                                                  int RH, uint32_t sw, uint32_t sh) {
+                                                  int RH, uint32_t sw, uint32_t sh,
+                                                  bool corners) {
     const double hx = 0.5 * (double)(RW - 1), hy = 0.5 * (double)(RH - 1);
     const double cx = (double)X0 + hx, cy = (double)Y0 + hy;
     const double nc = fma(h[0], cx, fma(h[1], cy, h[2]));
@@ -562,7 +564,7 @@ __device__ __forceinline__ RegionWarp region_warp(const double (&h)[9], int X0, 
     const double reach = (fabs(ax) + fabs(bx) + fabs(ay) + fabs(by)) * (hx + hy);
     w.ok = pos && fabs(g) * hx + fabs(e) * hy <= 0.25 && reach < 1.0e5 && xc > -1.0e5 &&
            xc < (double)sw + 1.0e5 && yc > -1.0e5 && yc < (double)sh + 1.0e5;
-    w.pad[0] = w.pad[1] = 0;
+    w.pad = 0;
     const double fx = w.ok ? floor(xc) : 0.0, fy = w.ok ? floor(yc) : 0.0;
     w.ixc = (int)fx;
     w.iyc = (int)fy;
@@ -577,6 +579,25 @@ __device__ __forceinline__ RegionWarp region_warp(const double (&h)[9], int X0, 
     w.y0 = (float)(-(ay * hx + by * hy));
     w.yu = (float)ay;
     w.yv = (float)by;
+    // With the denominator positive over the region the map is linear-fractional, so along
+    // any line the source position is monotonic and its extremes over the region are at the
+    // corners. One pixel of margin on each side covers the FP32 evaluation of every pixel
+    // (off by ~1e-5 px, at most eps times the reach), so when the corners clear it no pixel
+    // needs its own bounds test.
+    // (`corners` false skips the test and leaves `inside` 0.)
+    bool inside = corners && w.ok;
+    if (inside) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double du = (c & 1) ? hx : -hx, dv = (c & 2) ? hy : -hy;
+            const double rden = recip_f64(fma(g, du, fma(e, dv, 1.0)));  // in [0.75, 1.25]
+            const double x = fma(fma(ax, du, bx * dv), rden, xc);
+            const double y = fma(fma(ay, du, by * dv), rden, yc);
+            inside = inside && x >= 1.0 && x < (double)sw - 2.0 && y >= 1.0 &&
+                     y < (double)sh - 2.0;
+        }
+    }
+    w.inside = inside ? 1 : 0;
     return w;
 }
 
@@ -601,8 +622,9 @@ __device__ __forceinline__ void region_coords(half2_t *__restrict__ uv, int RW, 
 // `npad` is a multiple of NB * kBlock (region_pad). The texel offset is formed in bytes: the
 // column times the texel size plus the row times the row pitch in one 24-bit multiply-add
 // (frames are at most 65535 px wide, so the pitch stays below 2^24); an x outside [0, w-2]
-// wraps to a large unsigned value and fails the same test as before scaling.
-template <int NB, int ST>
+// wraps to a large unsigned value and fails the same test as before scaling. Without CHECK
+// (the step's region lies inside the texture, RegionWarp::inside) the tests are skipped.
+template <int NB, int ST, bool CHECK>
 __device__ __forceinline__ void warp_region_rel(float *__restrict__ L, const RegionWarp &w,
                                                 const half2_t *__restrict__ uv, int npad,
                                                 const TexRef<ST, true> &tex, float shift) {
@@ -624,7 +646,7 @@ __device__ __forceinline__ void warp_region_rel(float *__restrict__ L, const Reg
             const float qy = fmaf(fmaf(w.yu, u, fmaf(w.yv, v, w.y0)), rinv, w.fyc);
             const uint32_t xb = ((uint32_t)floor_to_int(qx) << kShift) + xc;
             const uint32_t iy = (uint32_t)(w.iyc + floor_to_int(qy));
-            const bool valid = xb <= xmax && iy <= ymax;
+            const bool valid = !CHECK || (xb <= xmax && iy <= ymax);
             ok[j] = valid;
             tx[j] = __builtin_amdgcn_fractf(qx);
             ty[j] = __builtin_amdgcn_fractf(qy);
@@ -1157,7 +1179,8 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
             if (hyp < h1) {
                 double h[9];
                 load_h(pd.Hp + ((size_t)ks * D + hyp) * 9, h);
-                table[lane] = region_warp(h, X0, Y0, RW, RH, pd.src[ks].tex.w, pd.src[ks].tex.h);
+                table[lane] = region_warp(h, X0, Y0, RW, RH, pd.src[ks].tex.w, pd.src[ks].tex.h,
+                                          !BOX);
             }
         }
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -1178,7 +1201,14 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
                     float *L = lds + buf * npad;
                     const RegionWarp rw = table[s * HC + j];
                     if (__builtin_amdgcn_readfirstlane(rw.ok)) {
-                        warp_region_rel<warp_batch<R>(), ST>(L, rw, uv, npad, tex, shift);
+                        // (the box-sum kernel skips the corner test, which measured 2% slower
+                        // there, so it keeps only the checked warp loop)
+                        if (!BOX && __builtin_amdgcn_readfirstlane(rw.inside))
+                            warp_region_rel<warp_batch<R>(), ST, false>(L, rw, uv, npad, tex,
+                                                                        shift);
+                        else
+                            warp_region_rel<warp_batch<R>(), ST, true>(L, rw, uv, npad, tex,
+                                                                       shift);
                     } else {
                         // the rare FP64 step gathers two pixels at a time, which keeps the
                         // kernel's registers at what the FP32 step needs
