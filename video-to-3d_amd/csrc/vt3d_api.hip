@@ -2099,11 +2099,35 @@ void admit(vt3d_server *sv, Slot &sl, std::vector<std::unique_ptr<ServerReq>> ta
     b->stream = sl.stream;
     std::vector<std::unique_ptr<ServerReq>> kept, back;
     std::vector<vt3d_completion> out;
+    // Host selection of every taken request, on up to 16 threads: each request is independent
+    // and reads only the session's frame table, and while the worker prepares, the GPU has
+    // only the other slots' batches to run. (A request that then does not fit the budget is
+    // prepared again with a later batch.)
+    const size_t nt = taken.size();
+    std::vector<std::unique_ptr<Prepared>> prs(nt);
+    std::vector<int> prc(nt, VT3D_OK);
+    {
+        const size_t n_threads = std::min<size_t>(
+            {nt, (size_t)std::max(1u, std::thread::hardware_concurrency()), (size_t)16});
+        auto work = [&](size_t t) {
+            for (size_t i = t; i < nt; i += n_threads) {
+                prs[i].reset(new Prepared());
+                prc[i] = prepare_request(s, taken[i]->q, taken[i]->p, *prs[i]);
+            }
+        };
+        if (n_threads <= 1) {
+            work(0);
+        } else {
+            std::vector<std::thread> pool;
+            for (size_t t = 0; t < n_threads; ++t) pool.emplace_back(work, t);
+            for (std::thread &th : pool) th.join();
+        }
+    }
     uint64_t bytes = 0;
-    for (size_t i = 0; i < taken.size(); ++i) {
+    for (size_t i = 0; i < nt; ++i) {
         std::unique_ptr<ServerReq> &r = taken[i];
-        std::unique_ptr<Prepared> pr(new Prepared());
-        const int rc = prepare_request(s, r->q, r->p, *pr);
+        std::unique_ptr<Prepared> &pr = prs[i];
+        const int rc = prc[i];
         if (rc) {
             out.push_back(failed(*r, rc));
             continue;
