@@ -148,6 +148,11 @@ def test_stride1_box_sums_match_oracle(product, oracle, radius):
     same = ((d.samples["status"] == g.samples["status"])
             & (d.samples["best_hypothesis"] == g.samples["best_hypothesis"]))
     assert len(g.samples) > 20000 and same.mean() >= 0.999
+    # integer luma is exact in FP16 and BF16 texels: the same bits as FP32 storage
+    for storage in ("f16", "bf16"):
+        r = _gpu(product, c1, st, kernel="tiled", storage=storage)
+        assert r.samples.tobytes() == g.samples.tobytes(), storage
+        assert r.points.tobytes() == g.points.tobytes(), storage
 
 
 def test_c2_is_deterministic_and_storage_independent(product):
