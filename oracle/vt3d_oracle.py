@@ -1,7 +1,8 @@
 """ctypes binding of the FP64 CPU oracle (libvt3d_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the CPU-baseline
-leg of bench.py, as the checker of the HIP path. The product never imports it.
+leg of bench.py, as the reference the HIP path is tested against. The product never
+imports it.
 """
 from __future__ import annotations
 
