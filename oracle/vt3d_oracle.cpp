@@ -4,8 +4,8 @@
 // CPU-baseline leg of the benchmark. It restates crates/video-to-3d-core/src/dense_legacy.rs
 // line for line in double precision (every function below cites the lines it follows),
 // generalised by vt3d_sweep_params so that the reference constants (dense_legacy.rs:6-21)
-// are one preset and the ZNCC configurations another. Built with -ffp-contract=off so that every
-// multiply and add rounds separately, as Rust's f64 arithmetic does.
+// are one preset and the ZNCC configurations another. Built with -ffp-contract=off so that
+// every multiply and add rounds separately, as Rust's f64 arithmetic does.
 #include "vt3d_oracle.h"
 
 #include <algorithm>
