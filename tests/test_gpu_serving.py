@@ -20,8 +20,10 @@ def c4(product):
     sess = product.Session(0)
     sess.upload(frames, "f16")
     requests = []
-    for a in arrivals:
+    for i, a in enumerate(arrivals):
         rq = scenes.c4_request(a, clips, base)
+        if i % 8 == 3:  # some requests at stride 1: the box-sum sweep in the same batches
+            rq["settings"] = dict(rq["settings"], grid_stride=1)
         rq["sweep"] = product.params(rq["settings"])
         requests.append(rq)
     # each request alone: a batch of one on the session (before any server runs on it)
@@ -48,6 +50,7 @@ def test_served_results_match_alone(product, c4):
     sess, requests, arrivals, alone = c4
     sizes = {(r["settings"]["patch_radius"], r["settings"]["max_source_views"]) for r in requests}
     assert len(sizes) > 3, "the mix spans patch sizes and source counts"
+    assert {r["settings"]["grid_stride"] for r in requests} == {1, 2}
     server = product.Server(sess, max_batch=8, max_inflight=2)
     try:
         out = serving.saturate(server, requests, keep_results=True)
