@@ -1009,16 +1009,15 @@ __global__ __launch_bounds__(kBlock) void k_compact_sites(Args a) {
 
 // ---- k_sweep -------------------------------------------------------------------------------
 
-// Inserts c into an ascending register list (min stays, max moves on): a sorting network
-// with static indices, so the list never spills.
+// Inserts c into an ascending register list with static indices, so the list never spills;
+// the largest of the N + 1 values drops off the end. Entry k of the merged list is the median
+// of the old entries k - 1 and k and c (the old list is ascending), so filling from the top
+// down takes one median-of-three per entry and one minimum.
 template <int N>
 __device__ __forceinline__ void sorted_insert(float (&l)[N], float c) {
 #pragma unroll
-    for (int k = 0; k < N; ++k) {
-        const float lo = fminf(l[k], c);
-        c = fmaxf(l[k], c);
-        l[k] = lo;
-    }
+    for (int k = N - 1; k > 0; --k) l[k] = __builtin_amdgcn_fmed3f(l[k - 1], l[k], c);
+    l[0] = fminf(l[0], c);
 }
 
 // Support (sources passing the photometric gate) and error (quantile 0.5 of their sorted
