@@ -1177,6 +1177,22 @@ int launch_batch(vt3d_batch &b) {
     return VT3D_OK;
 }
 
+// Runs work(t) for every share t in [0, n): share 0 on the calling thread, the others on
+// threads of their own. A thread that cannot be started (std::thread throws) leaves its share,
+// and the later ones, to the calling thread, so the work always completes.
+template <class F>
+void run_shares(size_t n, const F &work) {
+    std::vector<std::thread> pool;
+    size_t started = 1;
+    try {
+        for (; started < n; ++started) pool.emplace_back(work, started);
+    } catch (...) {
+    }
+    work(0);
+    for (size_t t = started; t < n; ++t) work(t);
+    for (std::thread &th : pool) th.join();
+}
+
 // Waits for a launched batch, downloads its packed outputs and builds one result per
 // request (results[i] stays NULL where the reference remap returns None).
 int finish_batch(vt3d_batch &b, vt3d_dense_result **results) {
@@ -1386,12 +1402,9 @@ int finish_batch(vt3d_batch &b, vt3d_dense_result **results) {
     if (n_threads <= 1) {
         for (size_t i = 0; i < nreq && rcs[0] == VT3D_OK; ++i) rcs[0] = build(i);
     } else {
-        std::vector<std::thread> pool;
-        for (size_t t = 0; t < n_threads; ++t)
-            pool.emplace_back([&, t]() {
-                for (size_t i = t; i < nreq && rcs[t] == VT3D_OK; i += n_threads) rcs[t] = build(i);
-            });
-        for (std::thread &th : pool) th.join();
+        run_shares(n_threads, [&](size_t t) {
+            for (size_t i = t; i < nreq && rcs[t] == VT3D_OK; i += n_threads) rcs[t] = build(i);
+        });
     }
     for (int e : rcs)
         if (e != VT3D_OK) return e;
@@ -2115,13 +2128,7 @@ void admit(vt3d_server *sv, Slot &sl, std::vector<std::unique_ptr<ServerReq>> ta
                 prc[i] = prepare_request(s, taken[i]->q, taken[i]->p, *prs[i]);
             }
         };
-        if (n_threads <= 1) {
-            work(0);
-        } else {
-            std::vector<std::thread> pool;
-            for (size_t t = 0; t < n_threads; ++t) pool.emplace_back(work, t);
-            for (std::thread &th : pool) th.join();
-        }
+        run_shares(std::max<size_t>(1, n_threads), work);
     }
     uint64_t bytes = 0;
     for (size_t i = 0; i < nt; ++i) {
