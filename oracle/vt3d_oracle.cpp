@@ -995,26 +995,35 @@ vt3d_dense_result *plane_sweep(const vt3d_frame *frames, size_t n_frames,
         for (size_t ri = 0; ri < rows.size(); ++ri)
             in_band[ri] = ri >= (size_t)g_band_first && ri < (size_t)g_band_first + g_band_count;
     std::vector<std::vector<SampleOut>> row_out(rows.size());
-    auto run_row = [&](size_t ri) {
-        if (!in_band[ri]) return;
-        uint32_t y = rows[ri];
-        for (uint32_t x = border; x < w - border; x += (uint32_t)stride) {
-            SampleOut o;
-            if (sw.sample(x, y, &o)) row_out[ri].push_back(o);
-        }
-    };
+    // Rows are polled one by one (dense_legacy.rs:273), then the grid sites of a chunk of
+    // polled rows are evaluated in parallel (samples are independent) and kept in row-major
+    // order, so a narrow band of rows still uses every thread.
     int nt = std::max(1, threads);
-    size_t band = (size_t)nt;
-    for (size_t r0 = 0; r0 < rows.size(); r0 += band) {
-        size_t r1 = std::min(rows.size(), r0 + band);
+    const size_t chunk = (size_t)std::max(nt, 8);
+    std::vector<std::pair<size_t, uint32_t>> sites;
+    std::vector<SampleOut> outs;
+    std::vector<char> kept;
+    for (size_t r0 = 0; r0 < rows.size(); r0 += chunk) {
+        size_t r1 = std::min(rows.size(), r0 + chunk);
         for (size_t ri = r0; ri < r1; ++ri)
             if (ctl.is_canceled()) return skipped(VT3D_SKIP_CANCELED);
+        sites.clear();
+        for (size_t ri = r0; ri < r1; ++ri)
+            if (in_band[ri])
+                for (uint32_t x = border; x < w - border; x += (uint32_t)stride)
+                    sites.push_back({ri, x});
+        outs.assign(sites.size(), SampleOut{});
+        kept.assign(sites.size(), 0);
         if (nt > 1) {
-#pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
-            for (long ri = (long)r0; ri < (long)r1; ++ri) run_row((size_t)ri);
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nt)
+            for (long k = 0; k < (long)sites.size(); ++k)
+                kept[k] = sw.sample(sites[k].second, rows[sites[k].first], &outs[k]) ? 1 : 0;
         } else {
-            for (size_t ri = r0; ri < r1; ++ri) run_row(ri);
+            for (size_t k = 0; k < sites.size(); ++k)
+                kept[k] = sw.sample(sites[k].second, rows[sites[k].first], &outs[k]) ? 1 : 0;
         }
+        for (size_t k = 0; k < sites.size(); ++k)
+            if (kept[k]) row_out[sites[k].first].push_back(outs[k]);
     }
 
     // Serial reduction in row-major order (dense_legacy.rs:363-412).

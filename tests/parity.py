@@ -12,9 +12,11 @@ from __future__ import annotations
 
 import numpy as np
 
-# Tolerances. ZSAD costs are intensity differences (0..255): FP32 bilinear and sums carry
-# about 1e-5 absolute error. ZNCC costs are in [0, 2] with about 1e-6 error.
-TAU_COST = {0: 1.0e-3, 1: 1.0e-4}
+# Decision margin below which a disagreement is a near tie. ZNCC costs (1 - ZNCC) lie in
+# [0, 2]: the margin is the north star's 1e-5. ZSAD costs are mean intensity differences on
+# the 0..255 scale, 127.5 times the ZNCC range, where the same relative margin would be
+# 1.275e-3; 1e-3 is used.
+TAU_COST = {0: 1.0e-3, 1: 1.0e-5}
 # Reverse-sweep ties that only move a reciprocal observation: the FP32 cost error must
 # exceed the gap to flip them, so a tighter bound (a few times the cost error) applies.
 TAU_POS = {0: 2.0e-4, 1: 2.0e-5}
@@ -37,6 +39,15 @@ def compare(gpu, oracle, slack, cost_kind: int, max_unexplained: int = 0):
     report = {"samples": int(len(gs)), "agree": float(agree.mean()) if len(gs) else 1.0,
               "near_ties": int(near.sum()), "disagree": int((~agree).sum()),
               "unexplained": int(unexplained.sum())}
+    # Disagreements whose oracle decision margin is at least the north star's 1e-5 and that
+    # no edge or relative-test slack explains, and the margins of every disagreement.
+    edge = (slack["px"] < TAU_PX) | (slack["rel"] < TAU_REL)
+    report["disagree_margin_ge_1e-5"] = int((~agree & ~edge & (slack["cost"] >= 1e-5)).sum())
+    report["disagree_cost_slack"] = sorted(float(v) for v in slack["cost"][~agree])[:12]
+    report["disagree_edge"] = int((~agree & edge).sum())
+    both = agree & (gs["best_hypothesis"] >= 0) & np.isfinite(os_["best_cost"])
+    ce = np.abs(gs["best_cost"][both].astype(np.float64) - os_["best_cost"][both].astype(np.float64))
+    report["best_cost_abs_err_max"] = float(ce.max()) if ce.size else 0.0
     assert report["unexplained"] <= max_unexplained, (report, _describe(gs, os_, slack,
                                                                         np.nonzero(unexplained)[0]))
     # Points: accepted on both sides with the same hypothesis and reciprocal set, and no near
@@ -57,6 +68,7 @@ def compare(gpu, oracle, slack, cost_kind: int, max_unexplained: int = 0):
     gp = _points_by_site(gpu)
     op = _points_by_site(oracle)
     worst = 0.0
+    rgb_diff = 0
     for i in np.nonzero(same)[0]:
         key = (int(gs["x"][i]), int(gs["y"][i]))
         a, b = gp[key], op[key]
@@ -64,8 +76,12 @@ def compare(gpu, oracle, slack, cost_kind: int, max_unexplained: int = 0):
         vb = np.array([b["x"], b["y"], b["z"]], dtype=np.float64)
         rel = np.linalg.norm(va - vb) / max(np.linalg.norm(vb), 1e-12)
         worst = max(worst, rel)
+        # sample_rgb (dense_legacy.rs:1159-1166): the reference frame's bytes at the site
+        rgb_diff += int((a["r"], a["g"], a["b"]) != (b["r"], b["g"], b["b"]))
     report["point_rel_max"] = worst
+    report["rgb_mismatch"] = rgb_diff
     assert worst <= POINT_REL, report
+    assert rgb_diff == 0, report
     return report
 
 
@@ -122,3 +138,25 @@ def compare_completed(g, o, what):
         assert g.stats[k] == o.stats[k], (what, k)
     ge, oe = g.stats["median_photometric_error"], o.stats["median_photometric_error"]
     assert (ge is None) == (oe is None) and (ge is None or abs(ge - oe) <= 1e-3), (what, ge, oe)
+
+
+def band(g, o):
+    """The GPU result restricted to the grid rows the oracle evaluated (a row band)."""
+    import copy
+
+    ys = np.unique(o.samples["y"])
+    gb = copy.copy(g)
+    keep = np.isin(g.samples["y"], ys)
+    gb.samples = g.samples[keep]
+    pk = np.isin(g.grid_sites["y"], ys)
+    gb.points, gb.grid_sites = g.points[pk], g.grid_sites[pk]
+    gb.stats = dict(g.stats, sampled_pixels=int(keep.sum()))
+    return gb, ys
+
+
+def check_counters(s):
+    """Counter identities of dense_legacy.rs:1282-1308."""
+    assert s["reciprocal_checked_points"] == (s["reciprocal_consistent_points"]
+                                              + s["reciprocal_rejected_points"])
+    assert s["reciprocal_consistent_points"] == s["accepted_points"] + s["fusion_rejected_points"]
+    assert s["fusion_input_observations"] >= 2 * s["reciprocal_consistent_points"]

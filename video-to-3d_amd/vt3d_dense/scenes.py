@@ -311,6 +311,30 @@ def config_c1(seed: int = SEED_C1) -> Problem:
                    list(depths))
 
 
+def colour_view(planes: Sequence[Plane], cam: Camera, width: int, height: int, focal: float,
+                seed: int) -> np.ndarray:
+    """One RGBA view whose channels carry independent value-noise textures (seeds seed,
+    seed + 1, seed + 2, each multi-view consistent) and whose alpha is per-pixel noise that
+    the dense pass must ignore: a swapped channel or a wrong luma weight changes the luma."""
+    out = np.empty((height, width, 4), np.uint8)
+    for c in range(3):
+        out[..., c] = render(planes, cam, width, height, focal, seed + c,
+                             background=(40, 96, 170)[c])[0]
+    rng = np.random.default_rng(seed ^ 0xA1FA)
+    out[..., 3] = rng.integers(0, 256, (height, width), dtype=np.uint8)
+    return out
+
+
+def config_c1_colour(seed: int = SEED_C1) -> Problem:
+    """The arc scene of `config_c1` rendered in colour (RGBA frames, see `colour_view`)."""
+    c1 = config_c1(seed)
+    W, H = c1.frames[0].shape[1], c1.frames[0].shape[0]
+    planes = arc_planes(seed)
+    frames = [colour_view(planes, cam, W, H, c1.focal, seed) for cam in c1.cameras]
+    return Problem("c1-arc-360x202-rgba", frames, c1.cameras, c1.sparse, c1.focal,
+                   c1.reference, dict(c1.settings), c1.depths)
+
+
 def room_planes(seed: int, low_texture: float = 0.0) -> List[Plane]:
     """Inside of a 5 x 2.6 x 5 m box room (x right, y down, z forward; floor y = 0)."""
     ex, ey, ez = np.eye(3)
