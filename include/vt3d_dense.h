@@ -263,6 +263,28 @@ int32_t vt3d_dense_reference_patch(const vt3d_frame *frames, size_t n_frames,
 
 void vt3d_dense_result_free(vt3d_dense_result *result);
 
+/* No counterpart in the reference. vt3d_dense_plane_sweep for callers that cannot allocate
+ * device memory (code under HIP graph capture setup, tight serving loops, harnesses that
+ * forbid hipMalloc): the same solve and the same results, with every device byte (frames,
+ * tables, per-sample buffers) taken from a fixed 128 MiB scratch array that each device
+ * holds from library load until the process ends. It calls no hipMalloc, hipFree or
+ * pinned-host allocator; its stream and events are created once per device by its first
+ * call there. A solve that needs more scratch returns VT3D_E_BUDGET with the size in
+ * vt3d_last_error() (a 640x360 reference with 8 sources fits).
+ *
+ * The caller owns `out` and its arrays: out->points, out->grid_sites and out->attributes
+ * hold `point_capacity` entries, out->samples `sample_capacity` (emit_samples only; may be
+ * NULL otherwise). Every field of *out is written; the arrays only when they are large
+ * enough: otherwise VT3D_E_BUDGET, with n_points / n_samples set to the sizes needed.
+ * Calls on one device share its scratch and run one at a time. */
+int32_t vt3d_dense_plane_sweep_noalloc(const vt3d_frame *frames, size_t n_frames,
+                                       const vt3d_camera *cameras, size_t n_cameras,
+                                       const double *sparse_xyz, size_t n_sparse, double focal,
+                                       int32_t fine_grid, const vt3d_sweep_params *params,
+                                       vt3d_poll_fn poll, void *poll_user,
+                                       vt3d_dense_result *out, size_t point_capacity,
+                                       size_t sample_capacity);
+
 /* ---- multi-reference dense estimate --------------------------------------------------- */
 
 #define VT3D_MAX_REFERENCE_VIEWS 3     /* MAX_REFERENCE_VIEWS, dense.rs:8 */

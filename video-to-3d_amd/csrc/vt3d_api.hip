@@ -70,15 +70,37 @@ namespace {
 constexpr size_t kMinVisibleSparsePoints = 8;  // dense_legacy.rs:8
 constexpr size_t kMaxTileLds = 64 * 1024;      // LDS budget of the tiled sweep's region
 
-// Grow-only device buffer.
+// A fixed device range handed out front to back (the no-allocation entry point's scratch).
+struct Arena {
+    uint8_t *base = nullptr;
+    size_t size = 0;
+    size_t used = 0;
+};
+
+// Grow-only device buffer: hipMalloc'd, or carved from an arena (never freed) when one is
+// bound.
 struct DevBuf {
     void *p = nullptr;
     size_t cap = 0;
+    Arena *arena = nullptr;
     DevBuf() = default;
     DevBuf(const DevBuf &) = delete;
     DevBuf &operator=(const DevBuf &) = delete;
     int ensure(size_t bytes) {
         if (bytes <= cap) return VT3D_OK;
+        if (arena) {
+            const size_t off = (arena->used + 255) & ~(size_t)255;
+            const size_t want = std::max<size_t>(bytes, 256);
+            if (off > arena->size || want > arena->size - off) {
+                set_error("the call needs more than the " + std::to_string(arena->size) +
+                          " bytes of scratch (at least " + std::to_string(off + want) + ")");
+                return VT3D_E_BUDGET;
+            }
+            p = arena->base + off;
+            cap = want;
+            arena->used = off + want;
+            return VT3D_OK;
+        }
         if (p) (void)hipFree(p);
         p = nullptr;
         cap = 0;
@@ -94,23 +116,40 @@ struct DevBuf {
     }
     template <class T> T *as() const { return reinterpret_cast<T *>(p); }
     ~DevBuf() {
-        if (p) (void)hipFree(p);
+        if (p && !arena) (void)hipFree(p);
     }
 };
 
-// Grow-only pinned host buffer.
+// Grow-only host staging buffer: pinned, or ordinary heap memory when `pageable` (the
+// no-allocation entry point allocates no pinned memory).
 struct HostBuf {
     void *p = nullptr;
     size_t cap = 0;
+    bool pageable = false;
     HostBuf() = default;
     HostBuf(const HostBuf &) = delete;
     HostBuf &operator=(const HostBuf &) = delete;
-    int ensure(size_t bytes) {
-        if (bytes <= cap) return VT3D_OK;
-        if (p) (void)hipHostFree(p);
+    void release() {
+        if (p) {
+            if (pageable) std::free(p);
+            else (void)hipHostFree(p);
+        }
         p = nullptr;
         cap = 0;
+    }
+    int ensure(size_t bytes) {
+        if (bytes <= cap) return VT3D_OK;
+        release();
         const size_t want = std::max<size_t>(bytes, 256);
+        if (pageable) {
+            p = std::malloc(want);
+            if (!p) {
+                set_error("out of host memory");
+                return VT3D_E_DEVICE;
+            }
+            cap = want;
+            return VT3D_OK;
+        }
         hipError_t e = hipHostMalloc(&p, want, hipHostMallocDefault);
         if (e != hipSuccess) {
             set_error(std::string("hipHostMalloc: ") + hipGetErrorString(e));
@@ -121,9 +160,7 @@ struct HostBuf {
         return VT3D_OK;
     }
     template <class T> T *as() const { return reinterpret_cast<T *>(p); }
-    ~HostBuf() {
-        if (p) (void)hipHostFree(p);
-    }
+    ~HostBuf() { release(); }
 };
 
 struct DevView {
@@ -131,6 +168,8 @@ struct DevView {
     uint32_t w = 0, h = 0;
     int storage = -1;
     bool valid = false;
+    DevView() = default;
+    explicit DevView(Arena *a) { rgba.arena = luma.arena = quad.arena = a; }
 };
 
 // Device buffers and stage events of one batch pipeline.
@@ -152,17 +191,25 @@ struct Workspace {
             if (!e) VT3D_HIP(hipEventCreate(&e));
         return VT3D_OK;
     }
+    template <class F> void each_buffer(F &&f) {
+        DevBuf *all[] = {&prob, &src, &Hp, &Hr, &depth_p, &depth_r, &block_problem,
+                         &site_flag, &samples, &site_map, &partial, &decision, &recip,
+                         &recip_list, &recip_count, &out, &block_counts, &block_offsets, &meta,
+                         &points, &sites, &attrs, &errors, &supports, &fobs, &diag, &site_pt,
+                         &prop_site, &prop, &prop_flag, &c_points, &c_sites, &c_attrs,
+                         &c_errors, &c_supports};
+        for (DevBuf *b : all) f(*b);
+    }
     // Device bytes held by the buffers.
-    size_t device_bytes() const {
-        const DevBuf *all[] = {&prob, &src, &Hp, &Hr, &depth_p, &depth_r, &block_problem,
-                               &site_flag, &samples, &site_map, &partial, &decision, &recip,
-                               &recip_list, &recip_count, &out, &block_counts, &block_offsets, &meta, &points, &sites,
-                               &attrs, &errors, &supports, &fobs, &diag, &site_pt, &prop_site,
-                               &prop, &prop_flag, &c_points, &c_sites, &c_attrs, &c_errors,
-                               &c_supports};
+    size_t device_bytes() {
         size_t t = 0;
-        for (const DevBuf *b : all) t += b->cap;
+        each_buffer([&](DevBuf &b) { t += b.cap; });
         return t;
+    }
+    // Carves every buffer from `a` and stages through heap memory instead of pinned memory.
+    void bind(Arena *a) {
+        each_buffer([&](DevBuf &b) { b.arena = a; });
+        meta_host.pageable = out_host.pageable = true;
     }
     ~Workspace() {
         for (hipEvent_t &e : ev)
@@ -176,6 +223,7 @@ struct Workspace {
 struct vt3d_session {
     int device = 0;
     hipStream_t stream = nullptr;
+    vt3d::Arena *arena = nullptr;  // set: frames and buffers are carved from this range
     int storage = VT3D_STORAGE_F32;
     std::vector<std::unique_ptr<vt3d::DevView>> views;  // uploaded frames by frame index
     vt3d::Workspace ws;                                  // one-shot and session solves
@@ -560,7 +608,7 @@ int prepare_sweep(vt3d_session *s, FrameSource &frames, const vt3d_camera *cams_
     // Device views of the reference and the sources.
     auto view_of = [&](uint64_t fi, DevView **outv) -> int {
         if (frames.host) {
-            pr.owned.emplace_back(new DevView());
+            pr.owned.emplace_back(new DevView(s->arena));
             int rc = upload_view(s, *pr.owned.back(), frames.host[frames.map[fi]], p.storage);
             *outv = pr.owned.back().get();
             return rc;
@@ -2241,6 +2289,82 @@ void serve(vt3d_server *sv) {
 
 using namespace vt3d;
 
+// ---- vt3d_dense_plane_sweep_noalloc ---------------------------------------------------------
+
+// Every device byte of a no-allocation call (frames, tables, per-sample buffers) comes from
+// this array. In host code its name only stands in for the symbol: the address each device
+// holds it at comes from hipGetSymbolAddress, once per device, and only that address is
+// used. Each device that loads the library keeps the array until the process ends. Calls on
+// one device share it (and keep the last call's contents), so the library runs them one at a
+// time; every buffer the pipeline reads before writing is cleared by the pipeline itself.
+static constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
+static __device__ __align__(256) uint8_t scratch[kNoAllocScratchBytes];
+
+namespace vt3d {
+namespace {
+
+// Per-device state of the no-allocation entry point, set up by its first call on a device.
+struct NoAllocDevice {
+    std::mutex mu;         // one call at a time on this device's scratch
+    uint8_t *ptr = nullptr;  // the device address of `scratch`
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[8] = {};
+};
+
+int noalloc_device(int dev, NoAllocDevice **out) {
+    static std::mutex setup;
+    static std::vector<std::unique_ptr<NoAllocDevice>> per_device;
+    std::lock_guard<std::mutex> lock(setup);
+    if ((size_t)dev >= per_device.size()) per_device.resize((size_t)dev + 1);
+    if (!per_device[dev]) {
+        std::unique_ptr<NoAllocDevice> nd(new NoAllocDevice());
+        VT3D_HIP(hipSetDevice(dev));
+        void *ptr = nullptr;
+        VT3D_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        nd->ptr = static_cast<uint8_t *>(ptr);
+        VT3D_HIP(hipStreamCreateWithFlags(&nd->stream, hipStreamNonBlocking));
+        for (hipEvent_t &e : nd->ev) VT3D_HIP(hipEventCreate(&e));
+        per_device[dev] = std::move(nd);
+    }
+    *out = per_device[dev].get();
+    return VT3D_OK;
+}
+
+// Copies a library result into the caller's arrays.
+int copy_to_caller(const vt3d_dense_result &r, vt3d_dense_result *out, size_t point_capacity,
+                   size_t sample_capacity) {
+    out->stats = r.stats;
+    out->skip_code = r.skip_code;
+    std::memcpy(out->skip_reason, r.skip_reason, sizeof(out->skip_reason));
+    out->n_points = r.n_points;
+    out->primary_points = r.primary_points;
+    out->n_samples = r.n_samples;
+    if (r.n_points > point_capacity || r.n_samples > sample_capacity) {
+        set_error("the solve has " + std::to_string(r.n_points) + " points and " +
+                  std::to_string(r.n_samples) + " samples; the caller's arrays hold " +
+                  std::to_string(point_capacity) + " and " + std::to_string(sample_capacity));
+        return VT3D_E_BUDGET;
+    }
+    if (r.n_points && (!out->points || !out->grid_sites || !out->attributes)) {
+        set_error("null output array");
+        return VT3D_E_INVALID;
+    }
+    if (r.n_samples && !out->samples) {
+        set_error("null sample array");
+        return VT3D_E_INVALID;
+    }
+    if (r.n_points) {
+        std::memcpy(out->points, r.points, sizeof(vt3d_point3) * r.n_points);
+        std::memcpy(out->grid_sites, r.grid_sites, sizeof(vt3d_grid_site) * r.n_points);
+        std::memcpy(out->attributes, r.attributes, sizeof(vt3d_point_attributes) * r.n_points);
+    }
+    if (r.n_samples) std::memcpy(out->samples, r.samples, sizeof(vt3d_sample_diag) * r.n_samples);
+    return VT3D_OK;
+}
+
+}  // namespace
+}  // namespace vt3d
+
 extern "C" {
 
 int32_t vt3d_abi_version(void) { return VT3D_ABI_VERSION; }
@@ -2330,6 +2454,54 @@ int32_t vt3d_dense_reference_patch(const vt3d_frame *frames, size_t n_frames,
     for (size_t i = 0; i < n_frames; ++i) fs.map.push_back(i);
     return solve_one(s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal,
                      (int64_t)reference_camera_index, fine_grid != 0, p, poll, poll_user, out);
+}
+
+int32_t vt3d_dense_plane_sweep_noalloc(const vt3d_frame *frames, size_t n_frames,
+                                       const vt3d_camera *cameras, size_t n_cameras,
+                                       const double *sparse_xyz, size_t n_sparse, double focal,
+                                       int32_t fine_grid, const vt3d_sweep_params *params,
+                                       vt3d_poll_fn poll, void *poll_user,
+                                       vt3d_dense_result *out, size_t point_capacity,
+                                       size_t sample_capacity) {
+    using namespace vt3d;
+    if (!out) {
+        set_error("out is null");
+        return VT3D_E_INVALID;
+    }
+    if ((n_cameras && !cameras) || (n_sparse && !sparse_xyz) || (n_frames && !frames)) {
+        set_error("null input array with nonzero length");
+        return VT3D_E_INVALID;
+    }
+    vt3d_sweep_params p;
+    int rc = resolve_params(params, &p);
+    if (rc) return rc;
+    int dev = 0;
+    VT3D_HIP(hipGetDevice(&dev));
+    NoAllocDevice *nd = nullptr;
+    if ((rc = noalloc_device(dev, &nd))) return rc;
+    std::lock_guard<std::mutex> lock(nd->mu);
+    Arena arena{nd->ptr, kNoAllocScratchBytes, 0};
+    vt3d_dense_result *res = nullptr;
+    {
+        // A session whose frames, buffers and events are the device's own; it borrows them
+        // for this call (its destructor frees nothing on the device).
+        vt3d_session s;
+        s.device = dev;
+        s.stream = nd->stream;
+        s.arena = &arena;
+        s.ws.bind(&arena);
+        for (int k = 0; k < 8; ++k) s.ws.ev[k] = nd->ev[k];
+        FrameSource fs{frames, nullptr, {}};
+        for (size_t i = 0; i < n_frames; ++i) fs.map.push_back(i);
+        rc = solve_one(&s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal, -1,
+                       fine_grid != 0, p, poll, poll_user, &res);
+        // work an error left queued finishes before the scratch can be handed out again
+        (void)hipStreamSynchronize(nd->stream);
+        for (hipEvent_t &e : s.ws.ev) e = nullptr;
+    }
+    if (!rc) rc = copy_to_caller(*res, out, point_capacity, sample_capacity);
+    vt3d_dense_result_free(res);
+    return rc;
 }
 
 void vt3d_dense_result_free(vt3d_dense_result *r) {

@@ -73,6 +73,10 @@ def _bind(lib: C.CDLL) -> None:
                                                         abi.POLL_FN, C.c_void_p,
                                                         P(P(abi.DenseResult))]
     lib.vt3d_dense_reference_patch.restype = C.c_int32
+    lib.vt3d_dense_plane_sweep_noalloc.argtypes = common + [
+        C.c_int32, P(abi.SweepParams), abi.POLL_FN, C.c_void_p, P(abi.DenseResult), C.c_size_t,
+        C.c_size_t]
+    lib.vt3d_dense_plane_sweep_noalloc.restype = C.c_int32
     lib.vt3d_dense_result_free.argtypes = [P(abi.DenseResult)]
     lib.vt3d_dense_result_free.restype = None
     lib.vt3d_dense_estimate.argtypes = common + [C.c_int32, C.c_int32, P(abi.SweepParams),
@@ -149,6 +153,7 @@ EXPORTED_SYMBOLS = (
     "vt3d_dense_estimate", "vt3d_multi_result_free", "vt3d_pack_dense_points",
     "vt3d_pack_grid_sites", "vt3d_pack_point_attributes", "vt3d_unpack_dense_points",
     "vt3d_dense_warning", "vt3d_dense_mesh", "vt3d_mesh_result_free", "vt3d_mesh_warning",
+    "vt3d_dense_plane_sweep_noalloc",
 )
 
 
@@ -213,6 +218,53 @@ def estimate_depth_points_with_control(frames, cameras, sparse_points, focal: fl
                                     C.byref(out))
     _check(rc)
     return _take(out)
+
+
+def estimate_depth_points_noalloc(frames, cameras, sparse_points, focal: float,
+                                  fine_grid: bool = False,
+                                  control: Optional[Callable[[], bool]] = None,
+                                  sweep: Optional[abi.SweepParams] = None,
+                                  point_capacity: int = 0, sample_capacity: int = 0):
+    """estimate_depth_points_with_control through vt3d_dense_plane_sweep_noalloc: the library
+    allocates no device memory; the result arrays are numpy arrays of the given capacities
+    (default: one entry per grid site of the first frame). Returns (return code,
+    DenseAnalysis); a code below zero leaves the stats and the counts needed."""
+    lib = library()
+    inp = abi.Inputs(_rgba(frames), cameras, sparse_points)
+    if not point_capacity:
+        h, w = (frames[0].shape[:2] if len(frames) else (0, 0))
+        point_capacity = max(1, h * w)
+    if not sample_capacity and sweep is not None and sweep.emit_samples:
+        sample_capacity = point_capacity
+    pts = np.zeros(point_capacity, abi.POINT_DTYPE)
+    sites = np.zeros(point_capacity, abi.SITE_DTYPE)
+    attrs = np.zeros(point_capacity, abi.ATTR_DTYPE)
+    samples = np.zeros(max(1, sample_capacity), abi.SAMPLE_DTYPE)
+    res = abi.DenseResult()
+    res.points = pts.ctypes.data_as(C.POINTER(abi.Point3))
+    res.grid_sites = sites.ctypes.data_as(C.POINTER(abi.GridSite))
+    res.attributes = attrs.ctypes.data_as(C.POINTER(abi.PointAttributes))
+    res.samples = samples.ctypes.data_as(C.POINTER(abi.SampleDiag)) if sample_capacity \
+        else C.POINTER(abi.SampleDiag)()
+    poll = _poll(control)
+    rc = lib.vt3d_dense_plane_sweep_noalloc(
+        inp.frames, inp.n_frames, inp.cameras, inp.n_cameras, inp.sparse_ptr, inp.n_sparse,
+        float(focal), int(fine_grid), C.byref(sweep) if sweep is not None else None, poll, None,
+        C.byref(res), point_capacity, sample_capacity)
+    if rc < 0:
+        return rc, abi.unpack_result(_counts_only(res))
+    return rc, abi.unpack_result(res)  # copies of the caller-side arrays
+
+
+def _counts_only(res: abi.DenseResult) -> abi.DenseResult:
+    """The stats and counts of a result whose arrays were not written."""
+    q = abi.DenseResult()
+    C.pointer(q)[0] = res
+    q.points = C.POINTER(abi.Point3)()
+    q.grid_sites = C.POINTER(abi.GridSite)()
+    q.attributes = C.POINTER(abi.PointAttributes)()
+    q.samples = C.POINTER(abi.SampleDiag)()
+    return q
 
 
 def reconstruct_reference_patch(frames, cameras, sparse_points, focal: float,
