@@ -173,6 +173,7 @@ struct DevView {
 };
 
 // Device buffers and stage events of one batch pipeline.
+constexpr int kWorkspaceBuffers = 35;
 struct Workspace {
     DevBuf prob, src, Hp, Hr, depth_p, depth_r, block_problem;
     DevBuf site_flag, samples, site_map, partial, decision, recip, recip_list, recip_count, out;
@@ -198,7 +199,20 @@ struct Workspace {
                          &points, &sites, &attrs, &errors, &supports, &fobs, &diag, &site_pt,
                          &prop_site, &prop, &prop_flag, &c_points, &c_sites, &c_attrs,
                          &c_errors, &c_supports};
+        static_assert(sizeof(all) / sizeof(all[0]) == kWorkspaceBuffers, "buffer list");
         for (DevBuf *b : all) f(*b);
+    }
+    void caps(uint64_t (&c)[kWorkspaceBuffers]) {
+        int k = 0;
+        each_buffer([&](DevBuf &b) { c[k++] = b.cap; });
+    }
+    // Frees every device buffer (the workspace's stream must be idle).
+    void release_device() {
+        each_buffer([&](DevBuf &b) {
+            if (b.p && !b.arena) (void)hipFree(b.p);
+            b.p = nullptr;
+            b.cap = 0;
+        });
     }
     // Device bytes held by the buffers.
     size_t device_bytes() {
@@ -897,7 +911,12 @@ int build_plan(const std::vector<Prepared *> &reqs, const vt3d_tuning &tuning, P
             for (int hh = 0; hh < pr.D; ++hh) {
                 const double d = hyp_depth(hh, pr.D, pr.src_min[si], pr.src_max[si]);
                 pl.dr.push_back(d);
-                for (int e = 0; e < 9; ++e) pl.Hr.push_back(d * Ar[e] + Br[e]);
+                double rec[kHrStride] = {};
+                for (int e = 0; e < 9; ++e) rec[e] = d * Ar[e] + Br[e];
+                const float fl[6] = {(float)rec[0], (float)rec[1], (float)rec[3],
+                                     (float)rec[4], (float)rec[6], (float)rec[7]};
+                std::memcpy(rec + kHrFloat, fl, sizeof(fl));
+                pl.Hr.insert(pl.Hr.end(), rec, rec + kHrStride);
             }
         }
     }
@@ -1636,25 +1655,60 @@ int prepare_request(vt3d_session *s, const vt3d_request &q, const vt3d_sweep_par
                                q.fine_grid != 0, p, ctl, pr);
 }
 
-// Upper bound of the device bytes one prepared request adds to a batch's buffers: its slots
-// in every per-site array, its hypothesis-block partials at the blocking it would get alone
-// (a batch never splits a request's hypotheses finer than that) and its tables.
-uint64_t request_bytes(const Prepared &pr, const vt3d_tuning &tuning) {
-    if (pr.state != 0) return 0;
-    const uint64_t blocks = std::max<uint64_t>(1, ((uint64_t)pr.nx * pr.ny + kBlock - 1) / kBlock);
-    const uint64_t cap = blocks * kBlock;
-    const uint64_t target = tuning.target_waves > 0 ? (uint64_t)tuning.target_waves : 4096;
-    const uint64_t n_hb = std::min<uint64_t>((uint64_t)pr.D, (target + 4 * blocks - 1) / (4 * blocks));
-    const uint64_t per_slot = 1 + 4 + 4 + sizeof(Decision) + sizeof(SampleOut) +
-                              sizeof(vt3d_point3) + sizeof(vt3d_grid_site) +
-                              sizeof(vt3d_point_attributes) + 12 +
-                              (pr.p.emit_samples ? sizeof(vt3d_sample_diag) : 0);
-    const uint64_t tables = sizeof(ProblemDev) + (uint64_t)pr.S * sizeof(SourceDev) +
-                            2ull * pr.S * pr.D * 9 * 8 + 8ull * pr.D * (1 + pr.S);
-    return cap * per_slot + cap * n_hb * sizeof(Top2) +
-           cap * (uint64_t)pr.S * (sizeof(Recip) + 4) + 4ull * kMaxRecipSources +
-           blocks * kCounters * 8 + tables + 64 * 4;
-}
+// Device bytes that the workspace buffers (in Workspace::each_buffer order) need at most for
+// a batch of problems: the sizes upload_plan ensures, with each problem's hypothesis blocking
+// bounded by the blocking it gets alone (a batch never splits a request's hypotheses finer).
+struct BatchNeeds {
+    uint64_t n = 0, slots = 0, partial = 0, pairs = 0, srcs = 0, sd = 0, d = 0;
+    int passes = 0;
+    bool emit = false;
+    void add(const Prepared &pr, const vt3d_tuning &tuning) {
+        if (pr.state != 0) return;
+        const uint64_t blocks =
+            std::max<uint64_t>(1, ((uint64_t)pr.nx * pr.ny + kBlock - 1) / kBlock);
+        const uint64_t cap = blocks * kBlock;
+        const uint64_t target = tuning.target_waves > 0 ? (uint64_t)tuning.target_waves : 4096;
+        const uint64_t n_hb =
+            std::min<uint64_t>((uint64_t)pr.D, (target + 4 * blocks - 1) / (4 * blocks));
+        ++n;
+        slots += cap;
+        partial += n_hb * cap;
+        pairs += (uint64_t)pr.S * cap;
+        srcs += (uint64_t)pr.S;
+        sd += (uint64_t)pr.S * pr.D;
+        d += (uint64_t)pr.D;
+        passes = std::max(passes, pr.p.surface_completion_passes);
+        emit = emit || pr.p.emit_samples;
+    }
+    void sizes(uint64_t (&b)[kWorkspaceBuffers]) const {
+        const uint64_t nb = slots / kBlock, cs = (uint64_t)passes * slots;
+        const MetaLayout ml{n, (size_t)passes, 2 + 2 * (size_t)passes};
+        const uint64_t completion = passes > 0 ? 1 : 0;
+        const uint64_t want[kWorkspaceBuffers] = {
+            sizeof(ProblemDev) * n, sizeof(SourceDev) * srcs, 8 * 9 * sd, 8 * kHrStride * sd,
+            8 * d, 8 * sd, 4 * nb, slots, 4 * slots, 4 * slots, sizeof(Top2) * partial,
+            sizeof(Decision) * slots, sizeof(Recip) * pairs, 4 * pairs,
+            4ull * kMaxRecipSources * n, sizeof(SampleOut) * slots, 4ull * kCounters * nb,
+            4ull * kCounters * nb, 4 * ml.words(), sizeof(vt3d_point3) * slots,
+            sizeof(vt3d_grid_site) * slots, sizeof(vt3d_point_attributes) * slots, 4 * slots,
+            4 * slots, 4 * slots, emit ? sizeof(vt3d_sample_diag) * slots : 0,
+            completion * sizeof(SitePoint) * slots, completion * sizeof(Proposal) * slots,
+            completion * sizeof(Proposal) * slots, completion * slots,
+            sizeof(vt3d_point3) * cs, sizeof(vt3d_grid_site) * cs,
+            sizeof(vt3d_point_attributes) * cs, 4 * cs, 4 * cs};
+        // DevBuf::ensure allocates at least 256 bytes
+        for (int k = 0; k < kWorkspaceBuffers; ++k)
+            b[k] = n && want[k] ? std::max<uint64_t>(want[k], 256) : 0;
+    }
+    // Device bytes of a workspace whose buffers hold `caps` once it has grown for this batch.
+    uint64_t footprint(const uint64_t (&caps)[kWorkspaceBuffers]) const {
+        uint64_t b[kWorkspaceBuffers];
+        sizes(b);
+        uint64_t t = 0;
+        for (int k = 0; k < kWorkspaceBuffers; ++k) t += std::max(caps[k], b[k]);
+        return t;
+    }
+};
 
 // ---- multi-reference estimate (dense.rs) -------------------------------------------------------
 
@@ -2091,6 +2145,7 @@ struct vt3d_server {
     std::deque<vt3d_completion> done;
     std::thread worker;
     bool stop = false;
+    bool failed = false;  // a device error ended the device work; later requests are refused
     uint64_t launches = 0;
     vt3d_server_stats stats{};
 };
@@ -2114,6 +2169,25 @@ vt3d_completion failed(const ServerReq &r, int status) {
     c.status = status;
     c.latency_ms = ms_between(r.arrival, Clock::now());
     return c;
+}
+
+// After a device error: the slot's requests and every queued one complete with
+// VT3D_E_DEVICE, and the server takes no more work.
+void abandon(vt3d_server *sv, Slot *sl) {
+    std::vector<vt3d_completion> out;
+    std::deque<std::unique_ptr<ServerReq>> queued;
+    {
+        std::lock_guard<std::mutex> lk(sv->mu);
+        sv->failed = true;
+        queued.swap(sv->pending);
+    }
+    if (sl) {
+        for (auto &r : sl->reqs) out.push_back(failed(*r, VT3D_E_DEVICE));
+        sl->reqs.clear();
+        sl->batch.reset();
+    }
+    for (auto &r : queued) out.push_back(failed(*r, VT3D_E_DEVICE));
+    complete(sv, out);
 }
 
 // Downloads a finished slot's batch and turns it into completions.
@@ -2148,6 +2222,7 @@ void retire(vt3d_server *sv, Slot &sl) {
     sl.batch.reset();
     sl.reqs.clear();
     complete(sv, out);
+    if (rc == VT3D_E_DEVICE) abandon(sv, nullptr);
 }
 
 // Prepares up to max_batch queued requests that fit the slot's budget and launches them on
@@ -2178,7 +2253,12 @@ void admit(vt3d_server *sv, Slot &sl, std::vector<std::unique_ptr<ServerReq>> ta
         };
         run_shares(std::max<size_t>(1, n_threads), work);
     }
-    uint64_t bytes = 0;
+    // Admission against the slot's real footprint: its buffers only grow, so the batch fits
+    // when every buffer's larger of held and needed bytes sums within the slot's share. A
+    // slot whose held buffers leave no room for the first request is emptied first.
+    uint64_t caps[kWorkspaceBuffers], none[kWorkspaceBuffers] = {};
+    sl.ws.caps(caps);
+    BatchNeeds acc;
     for (size_t i = 0; i < nt; ++i) {
         std::unique_ptr<ServerReq> &r = taken[i];
         std::unique_ptr<Prepared> &pr = prs[i];
@@ -2187,18 +2267,25 @@ void admit(vt3d_server *sv, Slot &sl, std::vector<std::unique_ptr<ServerReq>> ta
             out.push_back(failed(*r, rc));
             continue;
         }
-        const uint64_t need = request_bytes(*pr, s->tuning);
-        if (need > sv->slot_budget) {
+        BatchNeeds alone, trial = acc;
+        alone.add(*pr, s->tuning);
+        trial.add(*pr, s->tuning);
+        if (alone.footprint(none) > sv->slot_budget) {
             out.push_back(failed(*r, VT3D_E_BUDGET));
             std::lock_guard<std::mutex> lk(sv->mu);
             ++sv->stats.rejected_budget;
             continue;
         }
-        if (bytes + need > sv->slot_budget) {  // the rest waits for a later batch
-            for (size_t k = i; k < taken.size(); ++k) back.push_back(std::move(taken[k]));
-            break;
+        if (trial.footprint(caps) > sv->slot_budget) {
+            if (kept.empty()) {
+                sl.ws.release_device();
+                sl.ws.caps(caps);
+            } else {  // the rest waits for a later batch
+                for (size_t k = i; k < taken.size(); ++k) back.push_back(std::move(taken[k]));
+                break;
+            }
         }
-        bytes += need;
+        acc = trial;
         b->prepared.push_back(std::move(pr));
         kept.push_back(std::move(r));
     }
@@ -2216,6 +2303,7 @@ void admit(vt3d_server *sv, Slot &sl, std::vector<std::unique_ptr<ServerReq>> ta
     if (rc) {
         for (auto &r : kept) out.push_back(failed(*r, rc));
         complete(sv, out);
+        if (rc == VT3D_E_DEVICE) abandon(sv, nullptr);
         return;
     }
     sl.batch = std::move(b);
@@ -2241,9 +2329,16 @@ void serve(vt3d_server *sv) {
             Slot *oldest = nullptr;
             for (auto &x : sv->slots)
                 if (x->busy && (!oldest || x->seq < oldest->seq)) oldest = x.get();
-            if (!oldest || hipEventQuery(oldest->ws.ev[5]) != hipSuccess) break;
+            if (!oldest) break;
+            const hipError_t q = hipEventQuery(oldest->ws.ev[5]);
+            if (q == hipErrorNotReady) break;
             lk.unlock();
-            retire(sv, *oldest);
+            if (q == hipSuccess) {
+                retire(sv, *oldest);
+            } else {
+                set_error(std::string("serving batch: ") + hipGetErrorString(q));
+                abandon(sv, oldest);
+            }
             lk.lock();
             oldest->busy = false;
             progressed = true;
@@ -2255,6 +2350,12 @@ void serve(vt3d_server *sv) {
             if (!x->busy && !free_slot) free_slot = x.get();
         }
         if (sv->stop && sv->pending.empty() && !any_busy) break;
+        if (sv->failed && !sv->pending.empty()) {
+            lk.unlock();
+            abandon(sv, nullptr);
+            lk.lock();
+            continue;
+        }
         if (free_slot && !sv->pending.empty()) {
             std::vector<std::unique_ptr<ServerReq>> taken;
             while (!sv->pending.empty() && (int)taken.size() < sv->max_batch) {
@@ -2701,6 +2802,10 @@ int32_t vt3d_server_submit(vt3d_server *sv, const vt3d_request *q, uint64_t tag)
     {
         std::lock_guard<std::mutex> lk(sv->mu);
         if (sv->stop) return VT3D_E_INVALID;
+        if (sv->failed) {
+            set_error("the server stopped after a device error");
+            return VT3D_E_DEVICE;
+        }
         sv->pending.push_back(std::move(r));
         ++sv->stats.submitted;
     }

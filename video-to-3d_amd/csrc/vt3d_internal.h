@@ -34,6 +34,11 @@ struct ViewTex {
     uint32_t w, h;
 };
 
+// One reverse-sweep homography record: the 9 FP64 entries (row-major), then the FP32
+// copies of entries 0, 1, 3, 4, 6 and 7 that the per-hypothesis warp set-up uses.
+constexpr int kHrStride = 12;
+constexpr int kHrFloat = 9;  // first FP32 entry, in doubles
+
 // Per selected source view: camera, texture, the direct-check homography pieces
 // (H(d) = d * A + B maps reference pixel (u, v, 1) to source homogeneous coordinates) and the
 // source's landmark envelope for the reverse sweep.
@@ -62,7 +67,7 @@ struct ProblemDev {
     // tables
     const SourceDev *src;   // [S]
     const double *Hp;       // [S][D][9] primary homographies
-    const double *Hr;       // [S][D][9] reverse homographies (source -> reference)
+    const double *Hr;       // [S][D][kHrStride] reverse homographies (source -> reference)
     const double *depth_p;  // [D]
     const double *depth_r;  // [S][D]
     // layout

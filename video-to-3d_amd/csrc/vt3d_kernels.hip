@@ -371,14 +371,9 @@ __device__ __forceinline__ float finish_cost(const float (&rp)[T], const RefStat
 // Interior taps therefore lie inside the corners' box; should FP32 rounding at a box edge
 // put one a texel outside the texture, the gather reads zeros (buffer form) or the last
 // texel (global form) instead of memory beyond the texture.
-template <int R, int COST, int ST, bool BUF>
-__device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
-                                            const RefStats &st, const double *__restrict__ H,
-                                            double cx, double cy, const TexRef<ST, BUF> &tex) {
-    constexpr int P = 2 * R + 1;
-    constexpr int T = P * P;
-    const uint32_t sw = tex.w, sh = tex.h;
-    const Warp w = make_warp(H, cx, cy, sw, sh);
+// Validity of a warped patch from its four corner taps (see patch_cost).
+template <int R>
+__device__ __forceinline__ bool corners_inside(const Warp &w, uint32_t sw, uint32_t sh) {
     bool ok = w.ok;
     const float fr = (float)R;
 #pragma unroll
@@ -393,7 +388,18 @@ __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 *
              qx < (float)((int)sw - 1 - w.ixc) && qy >= (float)(-w.iyc) &&
              qy < (float)((int)sh - 1 - w.iyc);
     }
-    if (!ok) return kInfF;
+    return ok;
+}
+
+// The taps of a valid patch: each warped relative to the centre's cell, gathered, and
+// reduced to the patch cost.
+template <int R, int COST, int ST, bool BUF>
+__device__ __forceinline__ float patch_taps(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
+                                            const RefStats &st, const Warp &w,
+                                            const TexRef<ST, BUF> &tex) {
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    const uint32_t sw = tex.w;
     const uint32_t base = (uint32_t)(w.iyc * (int)sw + w.ixc);
     const uint32_t base_bytes = base * TexRef<ST, true>::kTexelBytes;
     const float pivot = st.mean;
@@ -444,7 +450,77 @@ __device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 *
     return finish_cost<COST, T>(rp, st, s1, s2, srs, sv);
 }
 
+// patch_error (dense_legacy.rs:1031-1095) for one (sample, hypothesis, source): every tap of
+// the reference patch is warped into the source texture and sampled bilinearly; a tap outside
+// [0, w-1) x [0, h-1) or with source depth <= 1e-4 makes the source not count (+inf).
+// ZSAD: mean |(r - r_mean) - (s - s_mean)|; ZNCC: 1 - ZNCC.
+//
+// Validity is decided on the four corner taps before any texel is read. Over a patch the
+// source depth is affine in (dx, dy), so its minimum is at a corner; with that depth positive
+// the warp is linear-fractional, so the extremes of the source x and y are at corners too.
+// Interior taps therefore lie inside the corners' box; should FP32 rounding at a box edge
+// put one a texel outside the texture, the gather reads zeros (buffer form) or the last
+// texel (global form) instead of memory beyond the texture.
+template <int R, int COST, int ST, bool BUF>
+__device__ __forceinline__ float patch_cost(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
+                                            const RefStats &st, const double *__restrict__ H,
+                                            double cx, double cy, const TexRef<ST, BUF> &tex) {
+    const Warp w = make_warp(H, cx, cy, tex.w, tex.h);
+    if (!corners_inside<R>(w, tex.w, tex.h)) return kInfF;
+    return patch_taps<R, COST, ST, BUF>(rp, st, w, tex);
+}
+
+// The warp of a reverse-sweep patch (k_recip): the centre in FP64 as make_warp computes it,
+// the tap-offset terms in FP32 from the record's FP32 entries (hf: H0, H1, H3, H4, H6, H7)
+// and the FP32 centre. A tap offset is at most R pixels, so the terms' FP32 rounding moves a
+// tap by under 1e-6 px. Returns the warp and whether its patch is inside for certain.
+template <int R>
+__device__ __forceinline__ Warp make_warp_rev(const double (&H)[9], const float (&hf)[6],
+                                              double cx, double cy, uint32_t sw, uint32_t sh,
+                                              float midx, float midy, bool *certain) {
+    const double nc = fma(H[0], cx, fma(H[1], cy, H[2]));
+    const double mc = fma(H[3], cx, fma(H[4], cy, H[5]));
+    const double dc = fma(H[6], cx, fma(H[7], cy, H[8]));
+    Warp w;
+    w.ok = dc > 1.0e-4;
+    const double rd = recip_f64(w.ok ? dc : 1.0);
+    // clamped, so that every conversion below stays in range whatever the centre
+    const double xc = fmin(fmax(nc * rd, -4096.0), (double)sw + 4096.0);
+    const double yc = fmin(fmax(mc * rd, -4096.0), (double)sh + 4096.0);
+    w.ok = w.ok && xc > -4096.0 && xc < (double)sw + 4096.0 && yc > -4096.0 &&
+           yc < (double)sh + 4096.0;
+    const double fx = floor(xc), fy = floor(yc);
+    w.ixc = (int)fx;
+    w.iyc = (int)fy;
+    w.fxc = (float)(xc - fx);
+    w.fyc = (float)(yc - fy);
+    const float rdf = (float)rd, xcf = (float)xc, ycf = (float)yc;
+    w.g = hf[4] * rdf;
+    w.e = hf[5] * rdf;
+    w.ax = fmaf(-xcf, hf[4], hf[0]) * rdf;
+    w.bx = fmaf(-xcf, hf[5], hf[1]) * rdf;
+    w.ay = fmaf(-ycf, hf[4], hf[2]) * rdf;
+    w.by = fmaf(-ycf, hf[5], hf[3]) * rdf;
+    w.dc = (float)dc;
+    // A bound on every tap: the denominator 1 + g dx + e dy is at least dmin, and the offset
+    // (ax dx + bx dy) / den at most R (|ax| + |bx|) / dmin. A patch whose bounds lie 1e-3 px
+    // inside [0, w-1) x [0, h-1), with every tap's depth above 1e-4, passes the corner test
+    // as well (FP32 corners sit within 1e-5 px of these bounds' exact values).
+    const float fr = (float)R;
+    const float dmin = fmaf(-fr, fabsf(w.g) + fabsf(w.e), 1.f);
+    const float reach = fr * __builtin_amdgcn_rcpf(dmin);
+    const float ex = (fabsf(w.ax) + fabsf(w.bx)) * reach;
+    const float ey = (fabsf(w.ay) + fabsf(w.by)) * reach;
+    const float cx_out = fabsf(xcf - midx), cy_out = fabsf(ycf - midy);
+    *certain = w.ok && dmin > 0.5f && w.dc * dmin > 1.0001e-4f &&
+               cx_out + ex < midx - 1.0e-3f && cy_out + ey < midy - 1.0e-3f;
+    // The centre is a tap: one 1e-3 px outside the bounds leaves the patch out for certain.
+    if (cx_out > midx + 1.0e-3f || cy_out > midy + 1.0e-3f) w.ok = false;
+    return w;
+}
+
 typedef const __attribute__((address_space(1))) double gdouble;
+typedef const __attribute__((address_space(1))) float gfloat;
 
 // A homography row-major in registers (loads through the global address space, so the
 // compiler keeps it out of the loops that write LDS).
@@ -1446,11 +1522,23 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     Recip out{{0.0, 0.0, 0.0}, 0, -1};
     float best_c = kInfF;
     int best_h = -1;
-    const double *Hs = pd.Hr + (size_t)s * D * 9;
+    const double *Hs = pd.Hr + (size_t)s * D * kHrStride;
+    const float midx = 0.5f * (float)(pd.ref.w - 1), midy = 0.5f * (float)(pd.ref.h - 1);
     for (int hyp = 0; hyp < D; ++hyp) {
+        const double *rec = Hs + (size_t)hyp * kHrStride;
         double h[9];
-        load_h(Hs + (size_t)hyp * 9, h);
-        const float c = patch_cost<R, COST, ST, true>(rp, st, h, xs, ys, ref_tex);
+        load_h(rec, h);
+        float hf[6];
+        const gfloat *rf = (const gfloat *)(rec + kHrFloat);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) hf[k] = rf[k];
+        bool certain;
+        const Warp w = make_warp_rev<R>(h, hf, xs, ys, pd.ref.w, pd.ref.h, midx, midy, &certain);
+        // the exact corner test only where the bound leaves the patch in doubt (near an edge)
+        bool valid = certain;
+        if (!certain && w.ok) valid = corners_inside<R>(w, pd.ref.w, pd.ref.h);
+        float c = kInfF;
+        if (valid) c = patch_taps<R, COST, ST, true>(rp, st, w, ref_tex);
         // cost ascending, then depth (hypothesis) ascending: strict < keeps the first
         if (c <= maxc && c < best_c) {
             best_c = c;
