@@ -38,6 +38,7 @@ extern "C" {
 #define VT3D_E_NO_DEVICE (-3)
 #define VT3D_E_BUDGET (-4)
 #define VT3D_E_BUSY (-5)
+#define VT3D_E_CANCELED (-6) /* a content hash polled a cancel ("reconstruction canceled") */
 
 /* FrameInput (crates/video-to-3d-core/src/reconstruction/types.rs:16-21):
  * RGBA8 row-major, 4*width bytes per row. */
@@ -188,7 +189,8 @@ enum vt3d_skip_code {
     VT3D_SKIP_NO_ENVELOPE = 6,         /* dense_legacy.rs:196-200 */
     VT3D_SKIP_NO_REFERENCE_FRAME = 7,  /* dense_legacy.rs:205-207 */
     VT3D_SKIP_NO_SOURCES = 8,          /* dense_legacy.rs:232-236 */
-    VT3D_SKIP_CANCELED = 9             /* dense_legacy.rs:273-275 */
+    VT3D_SKIP_CANCELED = 9,            /* dense_legacy.rs:273-275 */
+    VT3D_SKIP_WORKING_SET = 10         /* dense.rs:385-404 (budget; formatted reason) */
 };
 
 /* DenseAnalysis (dense_legacy.rs:61-68), library-owned. points/grid_sites/attributes are
@@ -285,6 +287,37 @@ int32_t vt3d_dense_plane_sweep_noalloc(const vt3d_frame *frames, size_t n_frames
                                        vt3d_dense_result *out, size_t point_capacity,
                                        size_t sample_capacity);
 
+/* ---- content identities (reconstruction_stages.rs:458-510) --------------------------- */
+
+/* ReconstructionOptions (crates/video-to-3d-core/src/reconstruction/types.rs:25-35). */
+typedef struct vt3d_reconstruction_options {
+    uint64_t max_features;
+    uint32_t min_feature_distance;
+    uint32_t descriptor_radius;
+    uint32_t match_radius;
+    float max_descriptor_distance;
+    float ratio_threshold;
+    int32_t has_focal_length_pixels; /* Option<f32> */
+    float focal_length_pixels;
+    uint32_t pad;
+    uint64_t max_dense_working_set_bytes;
+} vt3d_reconstruction_options;
+
+/* "sha256:" + 64 lowercase hex digits + NUL. */
+#define VT3D_CONTENT_HASH_LEN 72
+
+/* ReconstructionOptions::default (reconstruction/types.rs:37-50). */
+void vt3d_reconstruction_options_default(vt3d_reconstruction_options *options);
+/* request_content_hash_with_control (reconstruction_stages.rs:463-510): the DenseStageArtifact's
+ * request_hash. Polls once per 64 KiB of each frame's RGBA; a cancel returns VT3D_E_CANCELED.
+ * Host only, no device needed. */
+int32_t vt3d_request_content_hash(const vt3d_frame *frames, size_t n_frames,
+                                  const vt3d_reconstruction_options *options, vt3d_poll_fn poll,
+                                  void *poll_user, char out[VT3D_CONTENT_HASH_LEN]);
+/* ContentHash::of_bytes (scene_project.rs:463-465), e.g. the DenseStageArtifact's cameras_hash
+ * over the camera stage's JSON document. */
+void vt3d_content_hash(const uint8_t *bytes, size_t n, char out[VT3D_CONTENT_HASH_LEN]);
+
 /* ---- multi-reference dense estimate --------------------------------------------------- */
 
 #define VT3D_MAX_REFERENCE_VIEWS 3     /* MAX_REFERENCE_VIEWS, dense.rs:8 */
@@ -330,8 +363,25 @@ typedef struct vt3d_reference_patch_stats {
     float search_max_depth;
 } vt3d_reference_patch_stats;
 
-/* The multi-reference DenseAnalysis of dense.rs (DenseStats, dense.rs:100-146, without the
- * working-set fields). `combined` holds the merged stats and points: every patch's primary
+/* DenseWorkingSetEstimate (dense.rs:14-29): bytes of every container the reconstruction
+ * holds while dense depth runs, as estimate_working_set (dense.rs:423-605) charges them. The
+ * byte counts are the Rust size_of values of the x86_64 layout. */
+typedef struct vt3d_working_set {
+    uint64_t frame_bytes;
+    uint64_t retry_frame_bytes;
+    uint64_t remapped_frame_bytes;
+    uint64_t luminance_bytes;
+    uint64_t sparse_pipeline_bytes;
+    uint64_t dense_sample_bytes;
+    uint64_t topology_bytes;
+    uint64_t mesh_builder_bytes;
+    uint64_t surface_fusion_bytes;
+    uint64_t retained_candidate_bytes;
+    uint64_t packed_output_bytes;
+    uint64_t total_bytes;
+} vt3d_working_set;
+
+/* The multi-reference DenseAnalysis of dense.rs (DenseStats, dense.rs:100-146). `combined` holds the merged stats and points: every patch's primary
  * points in patch order, then every patch's completed points (combine_patches,
  * dense.rs:842-995); combined.stats.source_frames lists the participating frames other than
  * the primary reference; the medians and search depths are set only for a single patch. */
@@ -345,6 +395,14 @@ typedef struct vt3d_multi_result {
     vt3d_reference_patch_stats patches[VT3D_MAX_REFERENCE_VIEWS];
     int32_t omitted_references; /* the view limit stopped the run while candidates remained */
     int32_t fine_grid;          /* the grid spacing that ran */
+    /* Set by vt3d_dense_estimate_with_budget only (zero otherwise): the plan's estimate, the
+     * budget, the whole input's estimate with every reference view, and the reference views
+     * the budget allowed when that limit left a candidate out (dense.rs:132-139). */
+    vt3d_working_set working_set_estimate;
+    uint64_t working_set_budget_bytes;
+    uint64_t full_working_set_bytes;
+    int32_t has_reference_view_limit;
+    int32_t reference_view_limit;
 } vt3d_multi_result;
 
 /* Replaces estimate_depth_points_limited (dense.rs:244-330): reference candidates in the
@@ -364,6 +422,24 @@ int32_t vt3d_dense_estimate(const vt3d_frame *frames, size_t n_frames,
                             const vt3d_sweep_params *params, vt3d_poll_fn poll, void *poll_user,
                             vt3d_multi_result **out);
 void vt3d_multi_result_free(vt3d_multi_result *result);
+
+/* estimate_working_set (dense.rs:423-605): host arithmetic, no device needed. */
+void vt3d_estimate_working_set(const vt3d_frame *frames, size_t n_frames, uint64_t max_features,
+                               uint32_t descriptor_radius, size_t reference_views,
+                               int32_t fine_grid, vt3d_working_set *out);
+
+/* Replaces estimate_depth_points_with_budget_and_control (dense.rs:354-420): the finer grid
+ * only where its full three-view plan fits the budget, else the coarse plan with three, two
+ * or one reference views, whichever fits first; vt3d_dense_estimate then runs that plan.
+ * When even one view does not fit, no device work runs and the analysis is skipped with
+ * VT3D_SKIP_WORKING_SET and the reference's formatted reason. */
+int32_t vt3d_dense_estimate_with_budget(const vt3d_frame *frames, size_t n_frames,
+                                        const vt3d_camera *cameras, size_t n_cameras,
+                                        const double *sparse_xyz, size_t n_sparse, double focal,
+                                        uint64_t working_set_budget_bytes, uint64_t max_features,
+                                        uint32_t descriptor_radius,
+                                        const vt3d_sweep_params *params, vt3d_poll_fn poll,
+                                        void *poll_user, vt3d_multi_result **out);
 
 /* ---- mesh consumer: reference-grid triangulation of the dense points -------------------- */
 

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <limits>
@@ -1585,6 +1586,12 @@ void legacy_into(vt3d_dense_result *r, vt3d_multi_result *m) {
 
 }  // namespace
 
+int32_t estimate_limited(const vt3d_frame *frames, size_t n_frames, const vt3d_camera *cameras,
+                         size_t n_cameras, const double *sparse_xyz, size_t n_sparse,
+                         double focal, int32_t reference_views, bool fine,
+                         const vt3d_sweep_params &pp, Control &ctl, int32_t threads,
+                         vt3d_multi_result **out);
+
 extern "C" {
 
 int32_t vt3d_oracle_estimate(const vt3d_frame *frames, size_t n_frames,
@@ -1602,7 +1609,6 @@ int32_t vt3d_oracle_estimate(const vt3d_frame *frames, size_t n_frames,
     vt3d_oracle_params_reference(&def);
     const vt3d_sweep_params &pp = params ? *params : def;
     if (!valid_params(pp)) return VT3D_E_INVALID;
-    const Params p = to_params(pp);
     Control ctl{poll, poll_user};
     // estimate_depth_points_with_budget_and_control, dense.rs:366-372 (no working-set budget):
     // finer only where the rule gives frame 0 a smaller stride, and then only with landmarks
@@ -1614,6 +1620,20 @@ int32_t vt3d_oracle_estimate(const vt3d_frame *frames, size_t n_frames,
         fine_grid < 0 ? finer && distributed_landmarks(frames, n_frames, cameras, n_cameras,
                                                        sparse_xyz, n_sparse, focal, ctl)
                       : fine_grid != 0;
+    return estimate_limited(frames, n_frames, cameras, n_cameras, sparse_xyz, n_sparse, focal,
+                            reference_views, fine, pp, ctl, threads, out);
+}
+
+}  // extern "C"
+
+// estimate_depth_points_limited, dense.rs:244-330, with the grid decided and the control's
+// state carried in.
+int32_t estimate_limited(const vt3d_frame *frames, size_t n_frames, const vt3d_camera *cameras,
+                         size_t n_cameras, const double *sparse_xyz, size_t n_sparse,
+                         double focal, int32_t reference_views, bool fine,
+                         const vt3d_sweep_params &pp, Control &ctl, int32_t threads,
+                         vt3d_multi_result **out) {
+    const Params p = to_params(pp);
     vt3d_multi_result *m = (vt3d_multi_result *)std::calloc(1, sizeof(vt3d_multi_result));
     m->fine_grid = fine;
     auto legacy = [&]() {
@@ -1756,6 +1776,8 @@ int32_t vt3d_oracle_estimate(const vt3d_frame *frames, size_t n_frames,
     *out = m;
     return VT3D_OK;
 }
+
+extern "C" {
 
 void vt3d_oracle_multi_result_free(vt3d_multi_result *m) {
     if (!m) return;
@@ -2177,6 +2199,259 @@ void vt3d_oracle_mesh_result_free(vt3d_mesh_result *r) {
     if (!r) return;
     std::free(r->triangles);
     std::free(r);
+}
+
+}  // extern "C"
+
+// ---- working-set budget (dense.rs:354-605) ---------------------------------------------------
+//
+// size_of values of the Rust types on x86_64 (usize = 8). Each is the fields' sum rounded to
+// the largest field alignment; Option<T> of a type without a niche adds an aligned tag; a
+// field-less enum is one byte with a niche for its Option.
+namespace ws {
+
+constexpr size_t usz = 8, u32 = 4, i16 = 2, i32 = 4, i64 = 8, f32 = 4, f64 = 8, boolean = 1,
+                 u8 = 1;
+constexpr size_t vec = 24, vector3 = 24, vector2 = 16;
+constexpr size_t point3 = 4 * f32 + 3 * u8 + 1;              // 19 -> 20
+constexpr size_t dense_grid_site = 2 * u32;
+constexpr size_t evidence_point_attributes = 2 * f32;
+constexpr size_t mesh_triangle = 3 * usz + f32 + 4;         // 28 -> 32
+constexpr size_t option_usize = 2 * usz;
+constexpr size_t option_f64 = 2 * f64;
+constexpr size_t option_vector3 = vector3 + 8;
+constexpr size_t option_usize_f64 = usz + f64 + 8;
+constexpr size_t option_evidence_origin = 1;
+constexpr size_t triangle_geometry = 3 * vector3 + vector3 + 3 * f64;
+constexpr size_t option_triangle_geometry = triangle_geometry + 8;
+constexpr size_t image_removal = usz + usz + f64;
+constexpr size_t projected_triangle = 3 * (vector2 + f64);
+constexpr size_t kept_triangle = 3 * vector3 + usz + 3 * usz + usz;
+constexpr size_t triangle_source_corners = usz + 3 * usz;
+constexpr size_t half_edge = 4 * u32;
+constexpr size_t seam = 28;                                 // [u32; 6] + u8 -> 28
+constexpr size_t option_seam = seam + 4;
+constexpr size_t edge_use = 2 * u32;
+
+constexpr size_t MAX_REFERENCE_VIEWS = 3, MAX_SOURCE_VIEWS_PER_REFERENCE = 4,
+                 DENSE_GRID_BORDER = 3, MAX_SCALE_SAMPLES = 4096, MAX_SEAM_VERTICES = 6,
+                 MAX_SEAMS_PER_VERTEX = 16;
+
+size_t sadd(size_t a, size_t b) { return a > SIZE_MAX - b ? SIZE_MAX : a + b; }
+size_t smul(size_t a, size_t b) { return (a && b > SIZE_MAX / a) ? SIZE_MAX : a * b; }
+size_t ssub(size_t a, size_t b) { return a > b ? a - b : 0; }
+
+// btree_entry_bytes / hash_entry_bytes, dense.rs:607-613.
+constexpr size_t btree_entry_bytes(size_t p) { return p + 4 * usz + 32; }
+constexpr size_t hash_entry_bytes(size_t p) { return p + 4 * usz + 32; }
+
+// grid_axis_samples, dense.rs:615-618.
+size_t grid_axis_samples(size_t extent, size_t stride) {
+    size_t usable = ssub(extent, smul(DENSE_GRID_BORDER, 2));
+    return sadd(usable, ssub(stride, 1)) / stride;
+}
+
+// surface_merge_stitch.rs:128-152.
+size_t stitch_working_set_bytes(size_t vertices, size_t triangles) {
+    size_t per_vertex = boolean;
+    size_t per_triangle = 3 * half_edge + option_seam + MAX_SEAM_VERTICES * (2 * u32) + 3 * u32 +
+                          mesh_triangle + 3 * hash_entry_bytes(2 * u32 + edge_use) + usz +
+                          hash_entry_bytes((i32 + 4 + 3 * i64) + vec);
+    size_t scratch = 2 * MAX_SEAMS_PER_VERTEX *
+                     (btree_entry_bytes(usz) +
+                      MAX_SEAM_VERTICES * (btree_entry_bytes(usz) + 2 * (4 * usz)));
+    return sadd(sadd(smul(vertices, per_vertex), smul(triangles, per_triangle)), scratch);
+}
+
+// surface_merge.rs:1080-1121.
+size_t merge_working_set_bytes(size_t vertices, size_t triangles) {
+    size_t per_vertex = 3 * usz + boolean + option_evidence_origin + 2 * (2 * usz) +
+                        hash_entry_bytes(2 * usz) + 2 * hash_entry_bytes(2 * usz) +
+                        hash_entry_bytes(usz + (option_usize + 8));
+    size_t per_triangle = usz + option_usize + boolean + option_triangle_geometry + 2 * usz +
+                          image_removal + projected_triangle + usz + 3 * usz + 2 * usz +
+                          hash_entry_bytes((i32 + 4 + 2 * i64) + vec) +
+                          hash_entry_bytes((i32 + 4 + 3 * i64) + vec) + hash_entry_bytes(2 * usz) +
+                          (usz + 3 * usz) + triangle_source_corners;
+    size_t ownership = sadd(smul(triangles, 3 * f64 + kept_triangle + btree_entry_bytes(usz)),
+                            MAX_SCALE_SAMPLES * f64);
+    return sadd(sadd(smul(vertices, per_vertex), smul(triangles, per_triangle)),
+                std::max(ownership, stitch_working_set_bytes(vertices, triangles)));
+}
+
+// estimate_working_set, dense.rs:423-605.
+vt3d_working_set estimate_working_set(const vt3d_frame *frames, size_t n_frames,
+                                      size_t max_features, uint32_t descriptor_radius,
+                                      size_t reference_views, bool fine_grid) {
+    size_t frame_bytes = 0, luminance_pixels = 0, largest_frame_pixels = 0;
+    for (size_t i = 0; i < n_frames; ++i) {
+        frame_bytes = sadd(frame_bytes, (size_t)frames[i].width * frames[i].height * 4);
+        luminance_pixels = sadd(luminance_pixels, smul(frames[i].width, frames[i].height));
+        largest_frame_pixels =
+            std::max(largest_frame_pixels, smul(frames[i].width, frames[i].height));
+    }
+    size_t grid_samples_per_reference = 0;
+    if (n_frames) {
+        size_t stride = grid_stride_rule(frames[0].width, frames[0].height, fine_grid);
+        grid_samples_per_reference = smul(grid_axis_samples(frames[0].width, stride),
+                                          grid_axis_samples(frames[0].height, stride));
+    }
+    size_t max_grid_samples = smul(grid_samples_per_reference, reference_views);
+    size_t max_mesh_triangles = smul(max_grid_samples, 8);
+    size_t retry_frame_bytes = frame_bytes;
+    size_t remapped_frame_bytes = frame_bytes;
+    size_t luminance_bytes =
+        sadd(luminance_pixels, smul(largest_frame_pixels, 1 + MAX_SOURCE_VIEWS_PER_REFERENCE));
+    size_t descriptor_side = sadd(smul(descriptor_radius, 2), 1);
+    size_t descriptor_bytes = smul(smul(descriptor_side, descriptor_side), i16);
+    size_t feature_count = smul(n_frames, max_features);
+    size_t adjacent_match_count = smul(ssub(n_frames, 1), max_features);
+    size_t observation_count = smul(adjacent_match_count, 2);
+    size_t feature_record_bytes = 3 * u32 + vec;
+    size_t sparse_pipeline_bytes = smul(feature_count, sadd(feature_record_bytes, descriptor_bytes));
+    sparse_pipeline_bytes = sadd(sparse_pipeline_bytes, smul(adjacent_match_count, 3 * usz));
+    sparse_pipeline_bytes = sadd(sparse_pipeline_bytes, smul(adjacent_match_count, vec));
+    sparse_pipeline_bytes = sadd(sparse_pipeline_bytes, smul(observation_count, 2 * usz));
+    sparse_pipeline_bytes =
+        sadd(sparse_pipeline_bytes, smul(observation_count, hash_entry_bytes(3 * usz)));
+    sparse_pipeline_bytes =
+        sadd(sparse_pipeline_bytes,
+             smul(adjacent_match_count,
+                  point3 + usz + f32 + vector3 + 2 * f64 + 4 * usz));
+    size_t retained_sample_bytes =
+        point3 + dense_grid_site + evidence_point_attributes + 2 * f64 + 3 * usz;
+    size_t dense_sample_bytes = smul(smul(max_grid_samples, retained_sample_bytes), 2);
+    size_t topology_bytes = smul(max_mesh_triangles, mesh_triangle);
+    size_t mesh_builder_bytes =
+        smul(max_grid_samples, btree_entry_bytes(2 * i32 + usz + 6 * f64 + f32));
+    mesh_builder_bytes =
+        sadd(mesh_builder_bytes, smul(smul(max_grid_samples, 4), btree_entry_bytes(2 * i32)));
+    mesh_builder_bytes =
+        sadd(mesh_builder_bytes, smul(max_mesh_triangles, btree_entry_bytes(3 * usz)));
+    mesh_builder_bytes = sadd(mesh_builder_bytes, smul(max_mesh_triangles, mesh_triangle));
+    size_t per_vertex_fusion_bytes = option_usize + vector3 + f64 + usz + vec + option_vector3 +
+                                     option_f64 + option_usize_f64 + option_usize + 2 * boolean +
+                                     3 * i64 + vec + 4 * usz + usz + 2 * usz + vector3;
+    size_t per_triangle_fusion_bytes = 6 * usz + 3 * f64 + boolean + usz;
+    size_t surface_fusion_bytes =
+        std::max(sadd(smul(max_grid_samples, per_vertex_fusion_bytes),
+                      smul(max_mesh_triangles, per_triangle_fusion_bytes)),
+                 merge_working_set_bytes(max_grid_samples, max_mesh_triangles));
+    size_t retained_grid_samples = smul(grid_samples_per_reference, MAX_REFERENCE_VIEWS);
+    size_t retained_candidate_bytes =
+        sadd(smul(retained_grid_samples, point3 + dense_grid_site + evidence_point_attributes),
+             smul(smul(retained_grid_samples, 8), mesh_triangle));
+    size_t packed_output_bytes =
+        sadd(smul(max_grid_samples, 6 * f32 + 3 * u8 + 2 * u32), smul(max_mesh_triangles, 4 * u32));
+    vt3d_working_set w{};
+    w.frame_bytes = frame_bytes;
+    w.retry_frame_bytes = retry_frame_bytes;
+    w.remapped_frame_bytes = remapped_frame_bytes;
+    w.luminance_bytes = luminance_bytes;
+    w.sparse_pipeline_bytes = sparse_pipeline_bytes;
+    w.dense_sample_bytes = dense_sample_bytes;
+    w.topology_bytes = topology_bytes;
+    w.mesh_builder_bytes = mesh_builder_bytes;
+    w.surface_fusion_bytes = surface_fusion_bytes;
+    w.retained_candidate_bytes = retained_candidate_bytes;
+    w.packed_output_bytes = packed_output_bytes;
+    w.total_bytes = sadd(sadd(sadd(sadd(sadd(sadd(sadd(sadd(sadd(sadd(frame_bytes,
+        retry_frame_bytes), remapped_frame_bytes), luminance_bytes), sparse_pipeline_bytes),
+        dense_sample_bytes), topology_bytes), mesh_builder_bytes), surface_fusion_bytes),
+        retained_candidate_bytes), packed_output_bytes);
+    return w;
+}
+
+}  // namespace ws
+
+extern "C" {
+
+void vt3d_oracle_estimate_working_set(const vt3d_frame *frames, size_t n_frames,
+                                      uint64_t max_features, uint32_t descriptor_radius,
+                                      size_t reference_views, int32_t fine_grid,
+                                      vt3d_working_set *out) {
+    *out = ws::estimate_working_set(frames, n_frames, max_features, descriptor_radius,
+                                    reference_views, fine_grid != 0);
+}
+
+// estimate_depth_points_with_budget_and_control, dense.rs:354-420.
+int32_t vt3d_oracle_estimate_with_budget(const vt3d_frame *frames, size_t n_frames,
+                                         const vt3d_camera *cameras, size_t n_cameras,
+                                         const double *sparse_xyz, size_t n_sparse, double focal,
+                                         uint64_t working_set_budget_bytes,
+                                         uint64_t max_features, uint32_t descriptor_radius,
+                                         const vt3d_sweep_params *params, vt3d_poll_fn poll,
+                                         void *poll_user, int32_t threads,
+                                         vt3d_multi_result **out) {
+    if (!out) return VT3D_E_INVALID;
+    *out = nullptr;
+    vt3d_sweep_params def;
+    vt3d_oracle_params_reference(&def);
+    const vt3d_sweep_params &pp = params ? *params : def;
+    if (!valid_params(pp)) return VT3D_E_INVALID;
+    Control control{poll, poll_user};
+    auto estimate = [&](size_t views, bool fine) {
+        return ws::estimate_working_set(frames, n_frames, max_features, descriptor_radius,
+                                        views, fine);
+    };
+    const bool finer_spacing_available =
+        n_frames > 0 && grid_stride_rule(frames[0].width, frames[0].height, true) <
+                            grid_stride_rule(frames[0].width, frames[0].height, false);
+    bool fine_grid = false;
+    vt3d_working_set full{};
+    if (finer_spacing_available && distributed_landmarks(frames, n_frames, cameras, n_cameras,
+                                                         sparse_xyz, n_sparse, focal, control)) {
+        vt3d_working_set f = estimate(ws::MAX_REFERENCE_VIEWS, true);
+        if (f.total_bytes <= working_set_budget_bytes) {
+            fine_grid = true;
+            full = f;
+        }
+    }
+    if (!fine_grid) full = estimate(ws::MAX_REFERENCE_VIEWS, false);
+    size_t reference_views = 0;
+    vt3d_working_set working_set_estimate{};
+    if (full.total_bytes <= working_set_budget_bytes) {
+        reference_views = ws::MAX_REFERENCE_VIEWS;
+        working_set_estimate = full;
+    } else {
+        for (size_t views = ws::MAX_REFERENCE_VIEWS - 1; views >= 1; --views) {
+            vt3d_working_set limited = estimate(views, false);
+            if (limited.total_bytes <= working_set_budget_bytes) {
+                reference_views = views;
+                working_set_estimate = limited;
+                break;
+            }
+        }
+    }
+    if (!reference_views) {
+        vt3d_working_set single = estimate(1, false);
+        vt3d_multi_result *m = (vt3d_multi_result *)std::calloc(1, sizeof(vt3d_multi_result));
+        char reason[256];
+        std::snprintf(reason, sizeof(reason),
+                      "estimated dense working set of %zu bytes exceeds the configured %zu byte "
+                      "budget even for one reference view",
+                      (size_t)single.total_bytes, (size_t)working_set_budget_bytes);
+        set_reason(m->combined.skip_reason, sizeof(m->combined.skip_reason), reason);
+        m->combined.skip_code = VT3D_SKIP_WORKING_SET;
+        m->working_set_estimate = single;
+        m->working_set_budget_bytes = working_set_budget_bytes;
+        m->full_working_set_bytes = full.total_bytes;
+        *out = m;
+        return VT3D_OK;
+    }
+    int32_t rc = estimate_limited(frames, n_frames, cameras, n_cameras, sparse_xyz, n_sparse,
+                                  focal, (int32_t)reference_views, fine_grid, pp, control,
+                                  threads, out);
+    if (rc) return rc;
+    vt3d_multi_result *m = *out;
+    m->working_set_estimate = working_set_estimate;
+    m->working_set_budget_bytes = working_set_budget_bytes;
+    m->full_working_set_bytes = full.total_bytes;
+    if (reference_views < ws::MAX_REFERENCE_VIEWS && m->omitted_references) {
+        m->has_reference_view_limit = 1;
+        m->reference_view_limit = (int32_t)reference_views;
+    }
+    return VT3D_OK;
 }
 
 }  // extern "C"

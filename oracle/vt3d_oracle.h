@@ -61,6 +61,22 @@ int32_t vt3d_oracle_estimate(const vt3d_frame *frames, size_t n_frames,
                              int32_t threads, vt3d_multi_result **out);
 void vt3d_oracle_multi_result_free(vt3d_multi_result *result);
 
+/* estimate_working_set (dense.rs:423-605). */
+void vt3d_oracle_estimate_working_set(const vt3d_frame *frames, size_t n_frames,
+                                      uint64_t max_features, uint32_t descriptor_radius,
+                                      size_t reference_views, int32_t fine_grid,
+                                      vt3d_working_set *out);
+/* estimate_depth_points_with_budget_and_control (dense.rs:354-420); free with
+ * vt3d_oracle_multi_result_free. */
+int32_t vt3d_oracle_estimate_with_budget(const vt3d_frame *frames, size_t n_frames,
+                                         const vt3d_camera *cameras, size_t n_cameras,
+                                         const double *sparse_xyz, size_t n_sparse, double focal,
+                                         uint64_t working_set_budget_bytes,
+                                         uint64_t max_features, uint32_t descriptor_radius,
+                                         const vt3d_sweep_params *params, vt3d_poll_fn poll,
+                                         void *poll_user, int32_t threads,
+                                         vt3d_multi_result **out);
+
 /* Same contract as vt3d_dense_mesh. Free with vt3d_oracle_mesh_result_free. */
 int32_t vt3d_oracle_dense_mesh(const vt3d_point3 *points, size_t n_points,
                                const vt3d_grid_site *sites, size_t n_sites,

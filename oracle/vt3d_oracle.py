@@ -61,6 +61,14 @@ def library() -> C.CDLL:
     lib.vt3d_oracle_estimate.restype = C.c_int32
     lib.vt3d_oracle_multi_result_free.argtypes = [P(abi.MultiResult)]
     lib.vt3d_oracle_multi_result_free.restype = None
+    lib.vt3d_oracle_estimate_working_set.argtypes = [P(abi.Frame), C.c_size_t, C.c_uint64,
+                                                     C.c_uint32, C.c_size_t, C.c_int32,
+                                                     P(abi.WorkingSet)]
+    lib.vt3d_oracle_estimate_working_set.restype = None
+    lib.vt3d_oracle_estimate_with_budget.argtypes = common + [
+        C.c_uint64, C.c_uint64, C.c_uint32, P(abi.SweepParams), abi.POLL_FN, C.c_void_p,
+        C.c_int32, P(P(abi.MultiResult))]
+    lib.vt3d_oracle_estimate_with_budget.restype = C.c_int32
     lib.vt3d_oracle_distributed_landmarks.argtypes = common + [abi.POLL_FN, C.c_void_p]
     lib.vt3d_oracle_distributed_landmarks.restype = C.c_int32
     lib.vt3d_oracle_reference_order.argtypes = common + [P(C.c_size_t)]
@@ -212,6 +220,42 @@ def estimate(frames, cameras, sparse_points, focal: float, reference_views: int 
                                   int(reference_views), fine,
                                   C.byref(sweep) if sweep is not None else None, _poll(control),
                                   None, int(threads), C.byref(out))
+    if rc < 0:
+        raise ValueError(f"oracle rejected the input ({rc})")
+    m = abi.unpack_multi(out.contents)
+    lib.vt3d_oracle_multi_result_free(out)
+    return m
+
+
+def estimate_working_set(frames, max_features: int = 320, descriptor_radius: int = 3,
+                         reference_views: int = 3, fine_grid: bool = False) -> dict:
+    """Oracle of estimate_working_set (dense.rs:423-605); frames as arrays or (w, h)."""
+    dims = [f if isinstance(f, tuple) else (f.shape[1], f.shape[0]) for f in frames]
+    fr = (abi.Frame * max(1, len(dims)))()
+    for i, (w_, h_) in enumerate(dims):
+        fr[i].width, fr[i].height = int(w_), int(h_)
+    w = abi.WorkingSet()
+    library().vt3d_oracle_estimate_working_set(fr, len(dims), int(max_features),
+                                               int(descriptor_radius), int(reference_views),
+                                               int(bool(fine_grid)), C.byref(w))
+    return w.as_dict()
+
+
+def estimate_with_budget(frames, cameras, sparse_points, focal: float,
+                         working_set_budget_bytes: int, max_features: int = 320,
+                         descriptor_radius: int = 3,
+                         control: Optional[Callable[[], bool]] = None,
+                         sweep: Optional[abi.SweepParams] = None,
+                         threads: int = 1) -> abi.MultiAnalysis:
+    """Oracle of estimate_depth_points_with_budget_and_control (dense.rs:354-420)."""
+    lib = library()
+    inp = abi.Inputs(_rgba(frames), cameras, sparse_points)
+    out = C.POINTER(abi.MultiResult)()
+    rc = lib.vt3d_oracle_estimate_with_budget(
+        inp.frames, inp.n_frames, inp.cameras, inp.n_cameras, inp.sparse_ptr, inp.n_sparse,
+        float(focal), int(working_set_budget_bytes), int(max_features), int(descriptor_radius),
+        C.byref(sweep) if sweep is not None else None, _poll(control), None, int(threads),
+        C.byref(out))
     if rc < 0:
         raise ValueError(f"oracle rejected the input ({rc})")
     m = abi.unpack_multi(out.contents)

@@ -52,6 +52,8 @@ STRUCTS = {
     "vt3d_reference_patch_stats": abi.ReferencePatchStats, "vt3d_multi_result": abi.MultiResult,
     "vt3d_mesh_triangle": abi.MeshTriangle, "vt3d_mesh_patch_stats": abi.MeshPatchStats,
     "vt3d_mesh_stats": abi.MeshStats, "vt3d_mesh_result": abi.MeshResult,
+    "vt3d_working_set": abi.WorkingSet,
+    "vt3d_reconstruction_options": abi.ReconstructionOptions,
 }
 
 

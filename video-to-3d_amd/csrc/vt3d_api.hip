@@ -1976,7 +1976,7 @@ int combine_patches(std::vector<vt3d_dense_result *> &patches,
 int dense_estimate(vt3d_session *s, const vt3d_camera *cams_in, size_t n_cams,
                    const double *sparse_xyz, size_t n_sparse, double f, int reference_views,
                    int fine_req, const vt3d_sweep_params &p, vt3d_poll_fn poll, void *user,
-                   vt3d_multi_result *m) {
+                   vt3d_multi_result *m, bool pre_canceled = false) {
     FrameSource fs{nullptr, s, {}};
     for (size_t i = 0; i < s->views.size(); ++i) fs.map.push_back(i);
     std::vector<Cam> cams(n_cams);
@@ -1985,6 +1985,7 @@ int dense_estimate(vt3d_session *s, const vt3d_camera *cams_in, size_t n_cams,
     for (size_t i = 0; i < n_sparse; ++i)
         sparse[i] = {sparse_xyz[3 * i], sparse_xyz[3 * i + 1], sparse_xyz[3 * i + 2]};
     Control ctl{poll, user};
+    ctl.canceled = pre_canceled;  // a cancel the caller's own polls saw stays (sticky)
     // dense.rs:366-372: the finer grid only where it is finer for the first frame, and the
     // landmark check (which polls) only then.
     const bool finer = fs.count() > 0 && grid_stride_rule(fs.w(0), fs.h(0), true) <
@@ -2884,6 +2885,96 @@ int32_t vt3d_dense_estimate(const vt3d_frame *frames, size_t n_frames,
     if (rc) {
         vt3d_multi_result_free(m);
         return rc;
+    }
+    *out = m;
+    return VT3D_OK;
+}
+
+int32_t vt3d_dense_estimate_with_budget(const vt3d_frame *frames, size_t n_frames,
+                                        const vt3d_camera *cameras, size_t n_cameras,
+                                        const double *sparse_xyz, size_t n_sparse, double focal,
+                                        uint64_t budget, uint64_t max_features,
+                                        uint32_t descriptor_radius,
+                                        const vt3d_sweep_params *params, vt3d_poll_fn poll,
+                                        void *poll_user, vt3d_multi_result **out) {
+    if (!out) return VT3D_E_INVALID;
+    *out = nullptr;
+    vt3d_sweep_params p;
+    int rc;
+    if ((rc = resolve_params(params, &p))) return rc;
+    if ((n_frames && !frames) || (n_cameras && !cameras) || (n_sparse && !sparse_xyz)) {
+        set_error("null input array with nonzero length");
+        return VT3D_E_INVALID;
+    }
+    auto estimate = [&](size_t views, bool fine) {
+        vt3d_working_set w{};
+        vt3d_estimate_working_set(frames, n_frames, max_features, descriptor_radius, views,
+                                  fine ? 1 : 0, &w);
+        return w;
+    };
+    // dense.rs:366-384. The landmark check reads only frame sizes, so it runs on the host
+    // frames, before anything goes to the device.
+    FrameSource fs{frames, nullptr, {}};
+    for (size_t i = 0; i < n_frames; ++i) fs.map.push_back(i);
+    std::vector<Cam> cams(n_cameras);
+    for (size_t i = 0; i < n_cameras; ++i) cams[i] = to_cam(cameras[i]);
+    std::vector<V3> sparse(n_sparse);
+    for (size_t i = 0; i < n_sparse; ++i)
+        sparse[i] = {sparse_xyz[3 * i], sparse_xyz[3 * i + 1], sparse_xyz[3 * i + 2]};
+    Control ctl{poll, poll_user};
+    const bool finer = n_frames > 0 && grid_stride_rule(frames[0].width, frames[0].height, true) <
+                                           grid_stride_rule(frames[0].width, frames[0].height, false);
+    bool fine = false;
+    vt3d_working_set full{};
+    if (finer && has_distributed_landmarks(fs, cams, sparse, focal, ctl)) {
+        full = estimate(VT3D_MAX_REFERENCE_VIEWS, true);
+        fine = full.total_bytes <= budget;
+    }
+    if (!fine) full = estimate(VT3D_MAX_REFERENCE_VIEWS, false);
+    size_t views = 0;
+    vt3d_working_set plan{};
+    if (full.total_bytes <= budget) {
+        views = VT3D_MAX_REFERENCE_VIEWS;
+        plan = full;
+    } else {
+        for (size_t v = VT3D_MAX_REFERENCE_VIEWS - 1; v >= 1 && !views; --v) {
+            const vt3d_working_set w = estimate(v, false);
+            if (w.total_bytes <= budget) {
+                views = v;
+                plan = w;
+            }
+        }
+    }
+    vt3d_multi_result *m = (vt3d_multi_result *)std::calloc(1, sizeof(vt3d_multi_result));
+    if (!m) return VT3D_E_DEVICE;
+    if (!views) {
+        m->working_set_budget_bytes = budget;
+        m->full_working_set_bytes = full.total_bytes;
+        // dense.rs:385-404: nothing runs; the skip reason names the one-view estimate
+        m->working_set_estimate = estimate(1, false);
+        m->combined.skip_code = VT3D_SKIP_WORKING_SET;
+        std::snprintf(m->combined.skip_reason, sizeof(m->combined.skip_reason),
+                      "estimated dense working set of %llu bytes exceeds the configured %llu "
+                      "byte budget even for one reference view",
+                      (unsigned long long)m->working_set_estimate.total_bytes,
+                      (unsigned long long)budget);
+        *out = m;
+        return VT3D_OK;
+    }
+    vt3d_session *s = thread_session(&rc);
+    if (!s || (rc = vt3d_session_upload_frames(s, frames, n_frames, p.storage)) ||
+        (rc = dense_estimate(s, cameras, n_cameras, sparse_xyz, n_sparse, focal, (int)views,
+                             fine ? 1 : 0, p, poll, poll_user, m, ctl.canceled))) {
+        vt3d_multi_result_free(m);
+        return rc;
+    }
+    m->working_set_estimate = plan;
+    m->working_set_budget_bytes = budget;
+    m->full_working_set_bytes = full.total_bytes;
+    // dense.rs:416-418: the limit is reported only when it left a candidate out
+    if (views < VT3D_MAX_REFERENCE_VIEWS && m->omitted_references) {
+        m->has_reference_view_limit = 1;
+        m->reference_view_limit = (int32_t)views;
     }
     *out = m;
     return VT3D_OK;

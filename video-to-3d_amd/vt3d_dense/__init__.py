@@ -82,6 +82,21 @@ def _bind(lib: C.CDLL) -> None:
     lib.vt3d_dense_estimate.argtypes = common + [C.c_int32, C.c_int32, P(abi.SweepParams),
                                                  abi.POLL_FN, C.c_void_p, P(P(abi.MultiResult))]
     lib.vt3d_dense_estimate.restype = C.c_int32
+    lib.vt3d_estimate_working_set.argtypes = [P(abi.Frame), C.c_size_t, C.c_uint64, C.c_uint32,
+                                              C.c_size_t, C.c_int32, P(abi.WorkingSet)]
+    lib.vt3d_estimate_working_set.restype = None
+    lib.vt3d_dense_estimate_with_budget.argtypes = common + [
+        C.c_uint64, C.c_uint64, C.c_uint32, P(abi.SweepParams), abi.POLL_FN, C.c_void_p,
+        P(P(abi.MultiResult))]
+    lib.vt3d_dense_estimate_with_budget.restype = C.c_int32
+    lib.vt3d_reconstruction_options_default.argtypes = [P(abi.ReconstructionOptions)]
+    lib.vt3d_reconstruction_options_default.restype = None
+    lib.vt3d_request_content_hash.argtypes = [P(abi.Frame), C.c_size_t,
+                                              P(abi.ReconstructionOptions), abi.POLL_FN,
+                                              C.c_void_p, C.c_char_p]
+    lib.vt3d_request_content_hash.restype = C.c_int32
+    lib.vt3d_content_hash.argtypes = [C.c_char_p, C.c_size_t, C.c_char_p]
+    lib.vt3d_content_hash.restype = None
     lib.vt3d_multi_result_free.argtypes = [P(abi.MultiResult)]
     lib.vt3d_multi_result_free.restype = None
     u8p = P(C.c_uint8)
@@ -153,7 +168,9 @@ EXPORTED_SYMBOLS = (
     "vt3d_dense_estimate", "vt3d_multi_result_free", "vt3d_pack_dense_points",
     "vt3d_pack_grid_sites", "vt3d_pack_point_attributes", "vt3d_unpack_dense_points",
     "vt3d_dense_warning", "vt3d_dense_mesh", "vt3d_mesh_result_free", "vt3d_mesh_warning",
-    "vt3d_dense_plane_sweep_noalloc",
+    "vt3d_dense_plane_sweep_noalloc", "vt3d_estimate_working_set",
+    "vt3d_dense_estimate_with_budget", "vt3d_reconstruction_options_default",
+    "vt3d_request_content_hash", "vt3d_content_hash",
 )
 
 
@@ -307,6 +324,75 @@ def estimate_depth_points_limited(frames, cameras, sparse_points, focal: float,
     owner = _ResultOwner(out, lib.vt3d_multi_result_free)
     m = abi.unpack_multi(out.contents, owner)
     return m, m.omitted_references
+
+
+def reconstruction_options(**overrides) -> abi.ReconstructionOptions:
+    """ReconstructionOptions::default (reconstruction/types.rs:37-50) with overrides;
+    focal_length_pixels=None clears the focal."""
+    o = abi.ReconstructionOptions()
+    library().vt3d_reconstruction_options_default(C.byref(o))
+    for k, v in overrides.items():
+        if k == "focal_length_pixels":
+            o.has_focal_length_pixels = 0 if v is None else 1
+            o.focal_length_pixels = 0.0 if v is None else float(v)
+        else:
+            setattr(o, k, v)
+    return o
+
+
+def request_content_hash(frames, options: Optional[abi.ReconstructionOptions] = None,
+                         control: Optional[Callable[[], bool]] = None) -> str:
+    """request_content_hash (reconstruction_stages.rs:458-510): "sha256:..." of the frames
+    ((H, W) gray or (H, W, 4) RGBA uint8) and options. Raises Vt3dError on a cancel."""
+    lib = library()
+    inp = abi.Inputs(_rgba(frames), [], np.zeros((0, 3)))
+    opts = options if options is not None else reconstruction_options()
+    buf = C.create_string_buffer(72)
+    _check(lib.vt3d_request_content_hash(inp.frames, inp.n_frames, C.byref(opts),
+                                         _poll(control), None, buf))
+    return buf.value.decode()
+
+
+def content_hash(data: bytes) -> str:
+    """ContentHash::of_bytes (scene_project.rs:463-465)."""
+    buf = C.create_string_buffer(72)
+    library().vt3d_content_hash(bytes(data), len(data), buf)
+    return buf.value.decode()
+
+
+def estimate_working_set(frames, max_features: int = 320, descriptor_radius: int = 3,
+                         reference_views: int = abi.MAX_REFERENCE_VIEWS,
+                         fine_grid: bool = False) -> dict:
+    """estimate_working_set (dense.rs:423-605) of frames ((H, W) or (H, W, 4) arrays, or
+    (width, height) pairs): the DenseWorkingSetEstimate as a dict. Host arithmetic."""
+    dims = [f if isinstance(f, tuple) else (f.shape[1], f.shape[0]) for f in frames]
+    fr = (abi.Frame * max(1, len(dims)))()
+    for i, (w_, h_) in enumerate(dims):
+        fr[i].width, fr[i].height = int(w_), int(h_)
+    w = abi.WorkingSet()
+    library().vt3d_estimate_working_set(fr, len(dims), int(max_features),
+                                        int(descriptor_radius), int(reference_views),
+                                        int(bool(fine_grid)), C.byref(w))
+    return w.as_dict()
+
+
+def estimate_depth_points_with_budget(frames, cameras, sparse_points, focal: float,
+                                      working_set_budget_bytes: int, max_features: int = 320,
+                                      descriptor_radius: int = 3,
+                                      control: Optional[Callable[[], bool]] = None,
+                                      sweep: Optional[abi.SweepParams] = None):
+    """estimate_depth_points_with_budget_and_control (dense.rs:354-420) on the GPU."""
+    lib = library()
+    inp = abi.Inputs(_rgba(frames), cameras, sparse_points)
+    out = C.POINTER(abi.MultiResult)()
+    poll = _poll(control)
+    rc = lib.vt3d_dense_estimate_with_budget(
+        inp.frames, inp.n_frames, inp.cameras, inp.n_cameras, inp.sparse_ptr, inp.n_sparse,
+        float(focal), int(working_set_budget_bytes), int(max_features), int(descriptor_radius),
+        C.byref(sweep) if sweep is not None else None, poll, None, C.byref(out))
+    _check(rc)
+    owner = _ResultOwner(out, lib.vt3d_multi_result_free)
+    return abi.unpack_multi(out.contents, owner)
 
 
 def estimate_depth_points(frames, cameras, sparse_points, focal: float,

@@ -185,6 +185,29 @@ class ReferencePatchStats(C.Structure):
         ("search_min_depth", C.c_float), ("search_max_depth", C.c_float)]
 
 
+class ReconstructionOptions(C.Structure):
+    """ReconstructionOptions (crates/video-to-3d-core/src/reconstruction/types.rs:25-35)."""
+    _fields_ = [("max_features", C.c_uint64), ("min_feature_distance", C.c_uint32),
+                ("descriptor_radius", C.c_uint32), ("match_radius", C.c_uint32),
+                ("max_descriptor_distance", C.c_float), ("ratio_threshold", C.c_float),
+                ("has_focal_length_pixels", C.c_int32), ("focal_length_pixels", C.c_float),
+                ("pad", C.c_uint32), ("max_dense_working_set_bytes", C.c_uint64)]
+
+
+WORKING_SET_FIELDS = ("frame_bytes", "retry_frame_bytes", "remapped_frame_bytes",
+                      "luminance_bytes", "sparse_pipeline_bytes", "dense_sample_bytes",
+                      "topology_bytes", "mesh_builder_bytes", "surface_fusion_bytes",
+                      "retained_candidate_bytes", "packed_output_bytes", "total_bytes")
+
+
+class WorkingSet(C.Structure):
+    """DenseWorkingSetEstimate (crates/video-to-3d-core/src/dense.rs:14-29)."""
+    _fields_ = [(n, C.c_uint64) for n in WORKING_SET_FIELDS]
+
+    def as_dict(self) -> Dict[str, int]:
+        return {n: int(getattr(self, n)) for n in WORKING_SET_FIELDS}
+
+
 class MultiResult(C.Structure):
     _fields_ = [("combined", DenseResult), ("n_reference_frames", C.c_size_t),
                 ("reference_frames", C.c_uint64 * MAX_REFERENCE_VIEWS),
@@ -192,7 +215,10 @@ class MultiResult(C.Structure):
                 ("attempts", ReferenceAttempt * (MAX_REFERENCE_ATTEMPTS + 1)),
                 ("n_patches", C.c_size_t),
                 ("patches", ReferencePatchStats * MAX_REFERENCE_VIEWS),
-                ("omitted_references", C.c_int32), ("fine_grid", C.c_int32)]
+                ("omitted_references", C.c_int32), ("fine_grid", C.c_int32),
+                ("working_set_estimate", WorkingSet), ("working_set_budget_bytes", C.c_uint64),
+                ("full_working_set_bytes", C.c_uint64),
+                ("has_reference_view_limit", C.c_int32), ("reference_view_limit", C.c_int32)]
 
 
 class MeshTriangle(C.Structure):
@@ -339,6 +365,12 @@ def unpack_multi(m: MultiResult, owner=None) -> MultiAnalysis:
     st["reference_frames"] = [int(m.reference_frames[i]) for i in range(int(m.n_reference_frames))]
     st["reference_attempts"] = [_attempt(m.attempts[i]) for i in range(int(m.n_attempts))]
     st["reference_patches"] = [_patch(m.patches[i]) for i in range(int(m.n_patches))]
+    # dense.rs:132-139 (set by the budgeted entry only)
+    st["working_set_estimate"] = m.working_set_estimate.as_dict()
+    st["working_set_budget_bytes"] = int(m.working_set_budget_bytes)
+    st["full_working_set_bytes"] = int(m.full_working_set_bytes)
+    st["reference_view_limit"] = (int(m.reference_view_limit) if m.has_reference_view_limit
+                                  else None)
     return MultiAnalysis(combined, bool(m.omitted_references), bool(m.fine_grid))
 
 
