@@ -21,10 +21,13 @@ Four legs, all with frames resident in HBM; units = texture-gated samples x hypo
   FP32 and FP16 storage, with the FP16 result's error against the FP64 oracle on a band of
   grid rows (chosen hypothesis, photometric cost, fused points).
 
-Prints one JSON line. `roofline` prices the batched leg's primary sweep kernel against the
-FP32 vector peak at 24*P^2 FLOPs per patch cost, one patch cost per (sample, hypothesis,
-source kept by that keyframe's selection); `cpu_baseline` times the FP64 oracle on whole
-keyframes of the batched workload (about 10 s of work) on the host cores.
+Prints one JSON line. `roofline` prices the batched leg's dominant kernel, the reverse sweep
+(k_recip), at 24*P^2 FLOPs per (listed pair, hypothesis) patch cost over its launch time
+(HIP events on the batch's stream), against the FP32 vector peak (157.3 TFLOP/s, packed) and
+the non-packed issue limit this build runs at (78.6); the primary sweep's figure is labelled
+effective (naive-count), and the whole step's is given too. `cpu_baseline` times the FP64
+oracle on whole keyframes of the batched workload (about 10 s of work) on the process's
+cores, and on one thread.
 """
 from __future__ import annotations
 
@@ -38,25 +41,48 @@ import time
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "video-to-3d_amd"))
 
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3          # MI355X vector FP32, packed (v_pk_fma_f32)
+PEAK_FP32_NON_PACKED_TFLOPS = 78.6  # one wave64 FMA per SIMD per 4 cycles: this build's ceiling
 PEAK_HBM_GBS = 8000.0
 
 
 def _host_threads() -> int:
+    """The process's cores: its CPU affinity, capped by OMP_NUM_THREADS where set (the GPU
+    box gives each GPU's processes a share of 16)."""
     try:
         n = len(os.sched_getaffinity(0))
     except AttributeError:
         n = os.cpu_count() or 1
-    return max(1, min(n, 16))
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    if cap.isdigit() and int(cap) > 0:
+        n = min(n, int(cap))
+    return max(1, n)
+
+
+def _profile(name: str):
+    """A committed rocprofv3 --pmc summary (profiles/<name>.json), or None."""
+    path = os.path.join(REPO, "profiles", f"{name}.json")
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f)
 
 
 def _traffic(name: str):
     """HBM bytes per launch of a kernel from the committed rocprofv3 --pmc summary, if any."""
-    path = os.path.join(REPO, "profiles", f"{name}_traffic.json")
-    if not os.path.exists(path):
-        return None
-    with open(path) as f:
-        return json.load(f).get("bytes_per_launch")
+    prof = _profile(f"{name}_traffic")
+    return prof.get("bytes_per_launch") if prof else None
+
+
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(problem, settings, rows: int, threads: int, gpu_samples=None):
@@ -201,14 +227,25 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
     # session stream; the graph replays the same launches).
     eager = sess.batch(reqs, use_graph=False)
     eager.run()
-    sweep_ms, recip_ms, eager_ms = [], [], []
+    sweep_ms, recip_ms, rev_ms, eager_ms, pairs = [], [], [], [], 0
     for _ in range(max(3, steps // 2)):
         eager.run()
         tm = eager.timing()
         sweep_ms.append(tm.sweep_ms)
         recip_ms.append(tm.reciprocal_ms)
+        rev_ms.append(tm.reverse_sweep_ms)
         eager_ms.append(tm.total_ms)
+        pairs = int(tm.reciprocal_pairs)
     eager.close()
+    # Planning included: every step builds its batch (host selection, layout, FP64 tables)
+    # and runs it eagerly, as the plain leg's solve does for its single request.
+    planned = []
+    for _ in range(max(2, steps // 3)):
+        t1 = time.perf_counter()
+        b2 = sess.batch(reqs, use_graph=False)
+        b2.run()
+        planned.append(time.perf_counter() - t1)
+        b2.close()
     sess.close()
     walls_ms = [w * 1e3 for w in walls]
     per_step_units = units // max(1, steps)
@@ -225,7 +262,13 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
         "device_ms": {"graph_total": statistics.mean(dev_ms),
                       "eager_total": statistics.mean(eager_ms),
                       "sweep": statistics.mean(sweep_ms),
-                      "reciprocal": statistics.mean(recip_ms)},
+                      "reciprocal": statistics.mean(recip_ms),
+                      "reverse_sweep": statistics.mean(rev_ms)},
+        "planning_included": {"value": per_step_units / statistics.mean(planned),
+                              "ms_per_step": 1e3 * statistics.mean(planned),
+                              "what": "batch built (host selection, layout, tables) and run "
+                                      "eagerly every step, as the plain leg's solve is"},
+        "reciprocal_pairs": pairs,
         "units_per_step": per_step_units,
         "solves": len(problems),
         "sampled_pixels": int(sum(r.stats["sampled_pixels"] for r in ok)),
@@ -421,13 +464,52 @@ def main() -> int:
         prec = precision_leg(V, scenes, args.precision_rows, _host_threads(), log)
     triples, P = bat.pop("sample_hyp_sources"), bat.pop("P")
     cpu_problems = bat.pop("cpu_problems")
-    flops_per_launch = 24 * P * P * triples
+    D = 96
     mean_sources = triples / max(1, bat["units_per_step"])
-    sweep_s = bat["device_ms"]["sweep"] * 1e-3
-    achieved = flops_per_launch / sweep_s / 1e12
     kernel = ("k_sweep_tile<3, ZNCC, F16, 8>"
               if not bat["tuning"] or bat["tuning"]["choice"]["kernel"] == "tiled"
               else "k_sweep<3, ZNCC, F16, 8>")
+    # 24 P^2 FLOPs per patch cost (9 warp, 10 bilinear, 5 cost per tap). The reverse sweep
+    # evaluates one patch cost per (listed pair, hypothesis) and shares nothing; the primary
+    # sweep's count is naive: its tiles warp each pixel once for all the patches covering it.
+    fpc = 24 * P * P
+    rev_s = bat["device_ms"]["reverse_sweep"] * 1e-3
+    sweep_s = bat["device_ms"]["sweep"] * 1e-3
+    step_s = bat["device_ms"]["graph_total"] * 1e-3
+    rev_flops = fpc * D * bat["reciprocal_pairs"]
+    sweep_flops = fpc * triples
+    rev_tf = rev_flops / rev_s / 1e12
+    sweep_tf = sweep_flops / sweep_s / 1e12
+    step_tf = (rev_flops + sweep_flops) / step_s / 1e12
+    rev_traffic = _traffic("k_recip_c3")
+    sweep_traffic = _traffic("k_sweep_tile_c3")
+    issue = _profile("k_recip_k_sweep_tile_c3_valu_issue")
+
+    def frac(tf):
+        return {"achieved": tf, "frac": tf / PEAK_FP32_TFLOPS,
+                "frac_non_packed": tf / PEAK_FP32_NON_PACKED_TFLOPS}
+
+    roofline = {
+        "bound": "valu", "kernel": "k_recip<3, ZNCC, F16> (reverse sweep)",
+        "achieved": rev_tf, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+        "frac": rev_tf / PEAK_FP32_TFLOPS, "traffic": rev_traffic,
+        "peak_non_packed": PEAK_FP32_NON_PACKED_TFLOPS,
+        "frac_non_packed": rev_tf / PEAK_FP32_NON_PACKED_TFLOPS,
+        "share_of_step": rev_s / step_s,
+        "flops_per_pair_hypothesis": fpc, "pairs_per_launch": bat["reciprocal_pairs"],
+        "hypotheses": D, "launch_ms": rev_s * 1e3,
+        "hbm_gbs": rev_traffic / rev_s / 1e9 if rev_traffic else None,
+        "hbm_frac": rev_traffic / rev_s / 1e9 / PEAK_HBM_GBS if rev_traffic else None,
+        "valu_issue": issue.get("k_recip") if issue else None,
+        "primary_sweep": dict(frac(sweep_tf), kernel=kernel,
+                              label="effective (naive-count): tiles share each warp",
+                              patch_costs_per_launch=triples, launch_ms=sweep_s * 1e3,
+                              traffic=sweep_traffic,
+                              hbm_gbs=sweep_traffic / sweep_s / 1e9 if sweep_traffic else None,
+                              valu_issue=issue.get("k_sweep_tile") if issue else None),
+        "whole_step": dict(frac(step_tf), device_ms=step_s * 1e3,
+                           label="both sweeps' naive FLOPs over the graph replay"),
+    }
     line = {
         "metric": "depth-hyp×samples/s batched vs plain GPU + p50/p99 solve latency",
         "value": bat["value"],
@@ -446,7 +528,7 @@ def main() -> int:
                                "7x7 ZNCC, stride 2, FP16 storage, reciprocal + fusion, one "
                                "hipGraph replay per step",
                    "solves": bat["solves"], "sources": 8,
-                   "mean_sources_kept": round(mean_sources, 3), "hypotheses": 96, "patch": P,
+                   "mean_sources_kept": round(mean_sources, 3), "hypotheses": D, "patch": P,
                    "stride": 2, "storage": "f16",
                    "sampled_pixels": bat["sampled_pixels"],
                    "accepted_points": bat["accepted_points"]},
@@ -454,23 +536,20 @@ def main() -> int:
         "plain": plain,
         "batched_vs_plain": bat["value"] / plain["value"],
         "batched_vs_plain_device": bat["device_value"] / plain["device_value"],
+        "batched_planned_vs_plain": bat["planning_included"]["value"] / plain["value"],
         "latency_ms": (serv["open_loop"]["latency_ms"] if serv else plain["latency_ms"]),
         "latency_source": ("serving open loop, submit -> result on host" if serv
                            else "plain solve wall time"),
         "serving": serv,
         "large_solve": prec,
         "served_vs_plain": (serv["saturated"]["units_per_s"] / plain["value"] if serv else None),
-        "roofline": {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS,
-                     "unit": "TFLOP/s", "frac": achieved / PEAK_FP32_TFLOPS,
-                     "traffic": _traffic("k_sweep_tile_c3"),
-                     "kernel": kernel, "flops_per_patch_cost": 24 * P * P,
-                     "patch_costs_per_launch": triples,
-                     "units_per_launch": bat["units_per_step"]},
+        "roofline": roofline,
     }
     if not args.no_cpu_baseline:
-        # Whole keyframes of the batched workload, one after another, until about 10 s of
-        # CPU work; the first also checks the GPU's decisions on 4 of its rows.
         threads = _host_threads()
+        # All of this process's cores: whole keyframes of the batched workload, one after
+        # another, until about 10 s of CPU work; the first also checks the GPU's decisions on
+        # 4 of its rows.
         units, secs, names, agreement = 0.0, 0.0, [], None
         for k, prob in enumerate(cpu_problems):
             settings = dict(prob.settings, surface_completion_passes=0, storage="f32")
@@ -489,10 +568,45 @@ def main() -> int:
             names.append(f"{prob.name} ({cpu['sample']})")
             if secs >= 10.0:
                 break
-        line["cpu_baseline"] = {"value": units / secs, "unit": "depth-hyp*samples/s",
-                                "cores": threads, "kind": "port",
-                                "sample": f"{len(names)} keyframes of the batched workload in "
-                                          f"{secs:.1f} s: " + "; ".join(names)}
+        # One thread (the reference's own structure): bands of 2 grid rows of the first
+        # keyframe from its middle outwards, until about 10 s.
+        prob = cpu_problems[0]
+        settings = dict(prob.settings, surface_completion_passes=0, storage="f32")
+        u1, s1, bands = 0.0, 0.0, 0
+        sys.path.insert(0, os.path.join(REPO, "oracle"))
+        import vt3d_oracle as O
+
+        sw = O.params(settings)
+        ny = len(range(sw.patch_radius + 2, prob.frames[prob.reference].shape[0]
+                       - (sw.patch_radius + 2), sw.grid_stride))
+        first = ny // 2
+        while s1 < 10.0 and first + 2 <= ny:
+            O.set_row_band(first, 2)
+            try:
+                t0 = time.perf_counter()
+                res, _ = O.reference_patch(prob.frames, prob.cameras, prob.sparse, prob.focal,
+                                           prob.reference, sweep=sw, threads=1)
+                s1 += time.perf_counter() - t0
+            finally:
+                O.set_row_band(0, 0)
+            u1 += res.stats["sampled_pixels"] * res.stats["depth_hypotheses"]
+            first += 2
+            bands += 1
+        try:
+            affinity = len(os.sched_getaffinity(0))
+        except AttributeError:
+            affinity = os.cpu_count() or 1
+        line["cpu_baseline"] = {
+            "value": units / secs, "unit": "depth-hyp*samples/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{len(names)} keyframes of the batched workload in {secs:.1f} s: "
+                      + "; ".join(names),
+            "single_thread": {"value": u1 / s1, "cores": 1,
+                              "sample": f"{prob.name}: {bands} bands of 2 grid rows from row "
+                                        f"{ny // 2} ({int(u1)} units, {s1:.1f} s)"},
+            "nproc": os.cpu_count(), "affinity": affinity,
+            "omp_num_threads": os.environ.get("OMP_NUM_THREADS"), "cpu_model": _cpu_model(),
+            "thread_scaling": (units / secs) / (u1 / s1) if s1 > 0 else None}
         if agreement:
             line["parity_band"] = agreement
     else:

@@ -221,6 +221,8 @@ typedef struct vt3d_timing {
     double completion_ms;
     double download_ms;
     uint64_t units; /* sampled_pixels * depth_hypotheses */
+    double reverse_sweep_ms;   /* the reverse-sweep kernel alone (eager launches) */
+    uint64_t reciprocal_pairs; /* (sample, source) pairs it swept */
 } vt3d_timing;
 
 /* ---- parameters and strings ---------------------------------------------------------- */

@@ -44,6 +44,7 @@ void launch_compact_props(const Args &a, hipStream_t s);
 void launch_fill_u32(uint32_t *p, uint32_t n, hipStream_t s);
 void launch_complete_fuse(const Args &a, int pass, hipStream_t s);
 void launch_compact_additions(const Args &a, const Outputs &o, int pass, hipStream_t s);
+void launch_pair_totals(const Args &a, uint32_t *pairs, hipStream_t s);
 
 namespace {
 
@@ -174,6 +175,9 @@ struct DevView {
 
 // Device buffers and stage events of one batch pipeline.
 constexpr int kWorkspaceBuffers = 35;
+// Stage events: 0 launch, 1 gate, 2 sweep, 3 decide, 8 reverse sweep (k_recip), 4 fuse,
+// 7 completion, 5 end of the pipeline, 6 download done.
+constexpr int kStageEvents = 9;
 struct Workspace {
     DevBuf prob, src, Hp, Hr, depth_p, depth_r, block_problem;
     DevBuf site_flag, samples, site_map, partial, decision, recip, recip_list, recip_count, out;
@@ -183,7 +187,7 @@ struct Workspace {
     DevBuf c_points, c_sites, c_attrs, c_errors, c_supports;    // completed points, per pass
     HostBuf meta_host;
     HostBuf out_host;  // staging of the packed outputs
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[kStageEvents] = {};
     Workspace() = default;
     Workspace(const Workspace &) = delete;
     Workspace &operator=(const Workspace &) = delete;
@@ -966,7 +970,8 @@ struct MetaLayout {
     size_t out_offset() const { return n_props() + passes * n; }
     size_t diag_offset() const { return out_offset() + n + 1; }
     size_t comp_offset() const { return diag_offset() + n + 1; }
-    size_t words() const { return comp_offset() + passes * (n + 1); }
+    size_t pairs() const { return comp_offset() + passes * (n + 1); }  // listed reverse pairs
+    size_t words() const { return pairs() + n; }
 };
 
 MetaLayout meta_layout(const Plan &pl) {
@@ -1102,9 +1107,10 @@ Args batch_args(const vt3d_batch &b) {
 
 // The reciprocal check of every problem: list lengths cleared (a fill kernel, so a captured
 // graph holds no memset), the gates and pair lists of every group, then the reverse sweeps.
-void enqueue_recip(const Plan &pl, const Args &a, hipStream_t st) {
+void enqueue_recip(const Plan &pl, const Args &a, hipStream_t st, hipEvent_t before_sweep) {
     launch_fill_u32(a.recip_count, a.n_prob * (uint32_t)kMaxRecipSources, st);
-    for (int kind : {3, 1})
+    for (int kind : {3, 1}) {
+        if (kind == 1 && before_sweep) (void)hipEventRecord(before_sweep, st);
         for (const Group &g : pl.recip_groups) {
             Args ga = a;
             ga.group_p0 = g.p0;
@@ -1112,6 +1118,7 @@ void enqueue_recip(const Plan &pl, const Args &a, hipStream_t st) {
             ga.group_items = g.items;
             launch_group(g.storage, g.cost, g.radius, 0, kind, kSweepDirect, 0, 0, ga, st);
         }
+    }
 }
 
 // Surface-completion pass k of every problem: proposals, their compaction, the reciprocal
@@ -1135,7 +1142,7 @@ int enqueue_pass(vt3d_batch &b, int k) {
     launch_compact_props(a, st);
     Args ra = a;
     ra.n_samples = a.n_props + (size_t)k * n;
-    enqueue_recip(pl, ra, st);
+    enqueue_recip(pl, ra, st, nullptr);
     launch_complete_fuse(a, k, st);
     launch_scan(a, stage_complete(k), nullptr, st);
     launch_problem_offsets(a, 2 + k, st);
@@ -1190,7 +1197,8 @@ int enqueue(vt3d_batch &b, hipEvent_t *ev) {
     VT3D_HIP(hipGetLastError());
     if ((rc = mark(3))) return rc;
     launch_decide(a, st);
-    enqueue_recip(pl, a, st);
+    enqueue_recip(pl, a, st, ev ? ev[8] : nullptr);
+    launch_pair_totals(a, a.totals + meta_layout(pl).pairs(), st);
     VT3D_HIP(hipGetLastError());
     if ((rc = mark(4))) return rc;
     launch_fuse(a, b.outs.diag, st);
@@ -1234,12 +1242,12 @@ int launch_batch(vt3d_batch &b) {
                 VT3D_HIP(hipGraphInstantiate(&b.exec, b.graph, nullptr, nullptr, 0));
             }
             VT3D_HIP(hipGraphLaunch(b.exec, st));
-            for (int k : {1, 2, 3, 4, 7, 5}) VT3D_HIP(hipEventRecord(ws.ev[k], st));
+            for (int k : {1, 2, 3, 8, 4, 7, 5}) VT3D_HIP(hipEventRecord(ws.ev[k], st));
         } else if ((rc = enqueue(b, ws.ev))) {
             return rc;
         }
     } else {
-        for (int k : {1, 2, 3, 4, 7, 5}) VT3D_HIP(hipEventRecord(ws.ev[k], st));
+        for (int k : {1, 2, 3, 8, 4, 7, 5}) VT3D_HIP(hipEventRecord(ws.ev[k], st));
     }
     return VT3D_OK;
 }
@@ -1350,6 +1358,9 @@ int finish_batch(vt3d_batch &b, vt3d_dense_result **results) {
     tm.gate_ms = elapsed(ws.ev[1], ws.ev[2]);
     tm.sweep_ms = elapsed(ws.ev[2], ws.ev[3]);
     tm.reciprocal_ms = elapsed(ws.ev[3], ws.ev[4]);
+    tm.reverse_sweep_ms = elapsed(ws.ev[8], ws.ev[4]);
+    const uint32_t *pairs = meta + ml.pairs();
+    for (size_t k = 0; k < n; ++k) tm.reciprocal_pairs += pairs[k];
     tm.fuse_ms = elapsed(ws.ev[4], ws.ev[7]);
     tm.completion_ms = elapsed(ws.ev[7], ws.ev[5]);
     tm.download_ms = elapsed(ws.ev[5], ws.ev[6]);
@@ -2410,7 +2421,7 @@ struct NoAllocDevice {
     std::mutex mu;         // one call at a time on this device's scratch
     uint8_t *ptr = nullptr;  // the device address of `scratch`
     hipStream_t stream = nullptr;
-    hipEvent_t ev[8] = {};
+    hipEvent_t ev[kStageEvents] = {};
 };
 
 int noalloc_device(int dev, NoAllocDevice **out) {
@@ -2592,7 +2603,7 @@ int32_t vt3d_dense_plane_sweep_noalloc(const vt3d_frame *frames, size_t n_frames
         s.stream = nd->stream;
         s.arena = &arena;
         s.ws.bind(&arena);
-        for (int k = 0; k < 8; ++k) s.ws.ev[k] = nd->ev[k];
+        for (int k = 0; k < kStageEvents; ++k) s.ws.ev[k] = nd->ev[k];
         FrameSource fs{frames, nullptr, {}};
         for (size_t i = 0; i < n_frames; ++i) fs.map.push_back(i);
         rc = solve_one(&s, fs, cameras, n_cameras, sparse_xyz, n_sparse, focal, -1,

@@ -1561,6 +1561,15 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     a.recip[pd.recip_base + (size_t)s * pd.cap + i] = out;
 }
 
+// Listed reverse-sweep pairs of every problem (the primary pass), for the launch's accounting.
+__global__ __launch_bounds__(kBlock) void k_pair_totals(Args a, uint32_t *pairs) {
+    const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= a.n_prob) return;
+    uint32_t t = 0;
+    for (int s = 0; s < a.prob[p].S; ++s) t += a.recip_count[p * kMaxRecipSources + s];
+    pairs[p] = t;
+}
+
 __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t *p, uint32_t n) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -2112,6 +2121,11 @@ bool launch_group(int storage, int cost, int radius, int smax, int kind, int mod
         return dispatch_cost<VT3D_STORAGE_BF16>(cost, radius, a, smax, kind, mode, lds, pass, s);
     default: return false;
     }
+}
+
+void launch_pair_totals(const Args &a, uint32_t *pairs, hipStream_t s) {
+    hipLaunchKernelGGL(k_pair_totals, dim3((a.n_prob + kBlock - 1) / kBlock), dim3(kBlock), 0, s,
+                       a, pairs);
 }
 
 void launch_fill_u32(uint32_t *p, uint32_t n, hipStream_t s) {

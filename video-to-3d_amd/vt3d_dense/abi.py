@@ -121,7 +121,8 @@ class Timing(C.Structure):
     _fields_ = [("total_ms", C.c_double), ("upload_ms", C.c_double), ("gate_ms", C.c_double),
                 ("sweep_ms", C.c_double), ("reciprocal_ms", C.c_double), ("fuse_ms", C.c_double),
                 ("completion_ms", C.c_double), ("download_ms", C.c_double),
-                ("units", C.c_uint64)]
+                ("units", C.c_uint64), ("reverse_sweep_ms", C.c_double),
+                ("reciprocal_pairs", C.c_uint64)]
 
 
 class Tuning(C.Structure):
