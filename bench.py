@@ -164,7 +164,8 @@ def plain_leg(V, scenes, steps: int, warmup: int, storage: str):
 
     for _ in range(warmup):
         solve()
-    walls, units, sweep_ms, recip_ms, dev_ms = [], [], [], [], []
+    walls, units, sweep_ms, recip_ms, rev_ms, dev_ms = [], [], [], [], [], []
+    pairs = 0
     for _ in range(steps):
         t0 = time.perf_counter()
         res = solve()
@@ -173,7 +174,9 @@ def plain_leg(V, scenes, steps: int, warmup: int, storage: str):
         units.append(tm.units)
         sweep_ms.append(tm.sweep_ms)
         recip_ms.append(tm.reciprocal_ms)
+        rev_ms.append(tm.reverse_sweep_ms)
         dev_ms.append(tm.total_ms)
+        pairs = int(tm.reciprocal_pairs)
     sess.close()
     walls_ms = [w * 1e3 for w in walls]
     return {
@@ -182,7 +185,10 @@ def plain_leg(V, scenes, steps: int, warmup: int, storage: str):
         "ms_per_step": statistics.mean(walls_ms),
         "latency_ms": {"p50": _pct(walls_ms, 0.5), "p99": _pct(walls_ms, 0.99)},
         "device_ms": {"total": statistics.mean(dev_ms), "sweep": statistics.mean(sweep_ms),
-                      "reciprocal": statistics.mean(recip_ms)},
+                      "reciprocal": statistics.mean(recip_ms),
+                      "reverse_sweep": statistics.mean(rev_ms)},
+        "reciprocal_pairs": pairs,
+        "source_views": int(res.stats["source_views"]),
         "workload": "640x360 ref + 8 sources, 128 hyps 0.5-8 m, 7x7 ZNCC, stride 2, "
                     f"{storage.upper()} storage, reciprocal + fusion",
         "sampled_pixels": int(res.stats["sampled_pixels"]),
@@ -283,7 +289,7 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
     }
 
 
-def serving_leg(V, scenes, n_requests: int, warmup: int, log):
+def serving_leg(V, scenes, n_requests: int, warmup: int, log, inflight: int = 2):
     """The serving stream: Poisson arrivals at 200/s from 16 clips (long edge 360-640,
     4-12 sources, 64-192 hypotheses, 5x5 / 7x7 / 9x9 ZNCC, stride 2, FP16 storage), admitted by
     continuous batching under a 64 GB HBM budget; then the same requests all at once."""
@@ -304,7 +310,7 @@ def serving_leg(V, scenes, n_requests: int, warmup: int, log):
         rq = scenes.c4_request(a, clips, base)
         rq["sweep"] = V.params(rq["settings"])
         requests.append(rq)
-    server = V.Server(sess, max_batch=64, max_inflight=2, hbm_budget_bytes=64 << 30)
+    server = V.Server(sess, max_batch=64, max_inflight=inflight, hbm_budget_bytes=64 << 30)
     try:
         opened = serving.open_loop(server, requests, [a["arrival"] for a in arrivals],
                                    warmup=warmup)
@@ -321,7 +327,7 @@ def serving_leg(V, scenes, n_requests: int, warmup: int, log):
             "workload": f"{n_requests} requests after {warmup} warm-up, Poisson 200/s from "
                         "16 clips (long edge 360-640, 4-12 sources, 64-192 hyps, 5x5/7x7/9x9 "
                         "ZNCC, stride 2, FP16), 64 GB HBM budget, up to 64 requests per "
-                        "launch, 2 launches in flight"}
+                        f"launch, {inflight} launches in flight"}
 
 
 def precision_leg(V, scenes, rows: int, threads: int, log):
@@ -428,6 +434,8 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="measured requests of the serving stream (0: skip the serving leg)")
     ap.add_argument("--serving-warmup", type=int, default=200)
+    ap.add_argument("--serving-inflight", type=int, default=2,
+                    help="batches the server keeps in flight (1-8)")
     ap.add_argument("--precision-rows", type=int, default=2,
                     help="grid rows of the large solve checked against the FP64 oracle "
                          "(0: skip the large-solve leg)")
@@ -458,7 +466,8 @@ def main() -> int:
     log(f"batched leg: {bat['value']:.4g} units/s")
     serv = None
     if args.serving_requests > 0:
-        serv = serving_leg(V, scenes, args.serving_requests, args.serving_warmup, log)
+        serv = serving_leg(V, scenes, args.serving_requests, args.serving_warmup, log,
+                           args.serving_inflight)
     prec = None
     if args.precision_rows > 0:
         prec = precision_leg(V, scenes, args.precision_rows, _host_threads(), log)
