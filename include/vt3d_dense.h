@@ -589,13 +589,27 @@ typedef struct vt3d_request {
 typedef struct vt3d_batch vt3d_batch;
 
 /* Host selection, FP64 tables and device layout of a batch on the session's frames. The
- * session must outlive the batch; batches of one session run one at a time. */
+ * session must outlive the batch. Each batch has a HIP stream of its own, so batches of one
+ * session may be in flight together (submit / collect below); a batch itself runs one launch
+ * at a time. A destroyed batch's device buffers stay with the session for its next batches
+ * (four sets at most, freed by vt3d_session_destroy), so a stream of batches allocates no
+ * device memory once warm. */
 int32_t vt3d_batch_create(vt3d_session *session, const vt3d_request *requests, size_t n,
                           int32_t use_graph, vt3d_batch **out);
-/* Runs the batch. results[i] receives request i's result (free each with
- * vt3d_dense_result_free), or NULL where the reference remap returns None. */
+/* Runs the batch and waits for it. results[i] receives request i's result (free each with
+ * vt3d_dense_result_free), or NULL where the reference remap returns None. VT3D_E_INVALID
+ * while a submitted launch of the batch is not collected. */
 int32_t vt3d_batch_run(vt3d_batch *batch, vt3d_dense_result **results);
+/* vt3d_batch_run in two halves: submit enqueues the batch's launches on its stream and
+ * returns; collect waits for them, downloads the outputs and builds results as run does.
+ * Between the two the host may plan, submit or collect other batches, whose device work then
+ * overlaps this one's. submit on a submitted batch, and collect on one that is not, return
+ * VT3D_E_INVALID; so does vt3d_session_upload_frames while any batch of the session is
+ * submitted and not collected. */
+int32_t vt3d_batch_submit(vt3d_batch *batch);
+int32_t vt3d_batch_collect(vt3d_batch *batch, vt3d_dense_result **results);
 int32_t vt3d_batch_last_timing(const vt3d_batch *batch, vt3d_timing *timing);
+/* Waits for the batch's stream (a submitted launch is dropped uncollected) and frees it. */
 void vt3d_batch_destroy(vt3d_batch *batch);
 
 /* ---- serving: continuous batching on a session ----------------------------------------- */

@@ -172,3 +172,68 @@ def test_launch_shapes_agree_within_tolerance(product, mix):
             assert abs(len(g.points) - len(a.points)) <= max(2, len(a.points) // 200), i
         blocked = _alone_tuned(product, jobs[:1], {"tile_width": 32, "target_waves": 1})
         _same(blocked[0], alone[0], "hypothesis blocking")
+
+
+def test_batches_in_flight_together(product, mix):
+    """Two batches of one session submitted together (each on its own stream) give the same
+    bits as each solve alone, whichever is collected first; misuse is refused."""
+    jobs, alone = mix
+    frames, maps = product.pack_frames([j["frames"] for j in jobs])
+    sess = product.Session(0)
+    try:
+        sess.upload(frames, "f16")
+        reqs = [dict(j, frame_map=m) for j, m in zip(jobs, maps)]
+        first, second = list(range(0, 6)), list(range(6, len(jobs)))
+        a = sess.batch([reqs[i] for i in first], use_graph=True)
+        b = sess.batch([reqs[i] for i in second])
+        with pytest.raises(product.Vt3dError):
+            a.collect()  # nothing submitted
+        for order in ("ab", "ba"):
+            a.submit()
+            b.submit()
+            with pytest.raises(product.Vt3dError):
+                a.submit()  # already in flight
+            with pytest.raises(product.Vt3dError):
+                a.run()
+            with pytest.raises(product.Vt3dError):
+                sess.upload(frames, "f16")  # frames in use by batches in flight
+            got = {}
+            for k in order:
+                got[k] = (a if k == "a" else b).collect()
+            for i, g in zip(first, got["a"]):
+                _same(g, alone[i], f"{order}: request {i}")
+            for i, g in zip(second, got["b"]):
+                _same(g, alone[i], f"{order}: request {i}")
+        sess.upload(frames, "f16")  # nothing in flight
+        _same(a.run()[0], alone[0], "run after collect")
+        a.close()
+        b.close()
+    finally:
+        sess.close()
+
+
+def test_pooled_buffers_serve_later_batches(product, mix):
+    """A destroyed batch's buffers serve the session's next batches: a small batch after a
+    large one, a large one after a small one, and a third batch while two are alive."""
+    jobs, alone = mix
+    frames, maps = product.pack_frames([j["frames"] for j in jobs])
+    sess = product.Session(0)
+    try:
+        sess.upload(frames, "f16")
+        reqs = [dict(j, frame_map=m) for j, m in zip(jobs, maps)]
+        for sel in (list(range(len(jobs))), [6, 7], [0, 1, 2, 3, 4, 5, 7], [8]):
+            b = sess.batch([reqs[i] for i in sel])
+            for i, g in zip(sel, b.run()):
+                _same(g, alone[i], f"{sel}: request {i}")
+            b.close()
+        x, y = sess.batch([reqs[0]]), sess.batch(reqs[1:3])
+        z = sess.batch(reqs[3:7])
+        for bt, sel in ((x, [0]), (y, [1, 2]), (z, [3, 4, 5, 6])):
+            bt.submit()
+        for bt, sel in ((z, [3, 4, 5, 6]), (x, [0]), (y, [1, 2])):
+            for i, g in zip(sel, bt.collect()):
+                _same(g, alone[i], f"three alive: request {i}")
+        for bt in (x, y, z):
+            bt.close()
+    finally:
+        sess.close()

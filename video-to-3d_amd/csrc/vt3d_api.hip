@@ -7,6 +7,7 @@
 // stream. Everything per grid sample runs on the GPU; there is no CPU path for it. A single
 // solve is a batch of one.
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -188,6 +189,10 @@ struct Workspace {
     HostBuf meta_host;
     HostBuf out_host;  // staging of the packed outputs
     hipEvent_t ev[kStageEvents] = {};
+    // A created batch's stream, which stays with these buffers in the session's pool: a
+    // steady stream of batches then reuses the same few streams (and the hardware queues
+    // they map to) instead of creating one per batch.
+    hipStream_t stream = nullptr;
     Workspace() = default;
     Workspace(const Workspace &) = delete;
     Workspace &operator=(const Workspace &) = delete;
@@ -230,6 +235,10 @@ struct Workspace {
         meta_host.pageable = out_host.pageable = true;
     }
     ~Workspace() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
         for (hipEvent_t &e : ev)
             if (e) (void)hipEventDestroy(e);
     }
@@ -247,7 +256,35 @@ struct vt3d_session {
     vt3d::Workspace ws;                                  // one-shot and session solves
     vt3d_timing timing{};
     vt3d_tuning tuning{32, 4096};
+    // Buffers of destroyed batches, kept for the next batches (at most kPooledWorkspaces sets),
+    // so that a stream of batches allocates and frees no device memory once warm: hipFree
+    // would wait for every stream of the device, batches in flight included.
+    std::mutex pool_mu;
+    std::vector<std::unique_ptr<vt3d::Workspace>> pool;
+    std::atomic<int> inflight{0};  // batches submitted and not yet collected
 };
+
+namespace vt3d {
+namespace {
+
+constexpr size_t kPooledWorkspaces = 4;
+
+std::unique_ptr<Workspace> take_workspace(vt3d_session *s) {
+    std::lock_guard<std::mutex> lock(s->pool_mu);
+    if (s->pool.empty()) return std::unique_ptr<Workspace>(new Workspace());
+    std::unique_ptr<Workspace> w = std::move(s->pool.back());
+    s->pool.pop_back();
+    return w;
+}
+
+// `w`'s stream must be idle.
+void give_back_workspace(vt3d_session *s, std::unique_ptr<Workspace> w) {
+    std::lock_guard<std::mutex> lock(s->pool_mu);
+    if (s->pool.size() < kPooledWorkspaces) s->pool.push_back(std::move(w));
+}
+
+}  // namespace
+}  // namespace vt3d
 
 namespace vt3d {
 namespace {
@@ -935,9 +972,10 @@ struct vt3d_batch {
     vt3d_session *session = nullptr;
     std::vector<std::unique_ptr<vt3d::Prepared>> prepared;
     vt3d::Plan plan;
-    vt3d::Workspace own;           // the batch's buffers (one-shot solves use the session's)
-    vt3d::Workspace *ws = nullptr;
-    hipStream_t stream = nullptr;  // the session stream, or a serving slot's stream
+    std::unique_ptr<vt3d::Workspace> own;  // a created batch's buffers and stream (pooled)
+    vt3d::Workspace *ws = nullptr;         // own, the session's (one-shot) or a serving slot's
+    hipStream_t stream = nullptr;          // own's, the session's or a serving slot's
+    bool submitted = false;                // launched by vt3d_batch_submit, not yet collected
     bool uploaded = false;
     bool use_graph = false;
     hipGraph_t graph = nullptr;
@@ -951,8 +989,10 @@ struct vt3d_batch {
     int discard_pass = -1;         // pass whose completions are counted but dropped
     vt3d_timing timing{};
     ~vt3d_batch() {
+        if (own && own->stream) (void)hipStreamSynchronize(own->stream);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
+        if (own && session) vt3d::give_back_workspace(session, std::move(own));
     }
 };
 
@@ -2660,6 +2700,10 @@ int32_t vt3d_session_upload_frames(vt3d_session *s, const vt3d_frame *frames, si
     if (!s || (n && !frames)) return VT3D_E_INVALID;
     if (storage != VT3D_STORAGE_F32 && storage != VT3D_STORAGE_F16 && storage != VT3D_STORAGE_BF16)
         return VT3D_E_INVALID;
+    if (s->inflight.load() > 0) {
+        set_error("a batch of the session is in flight (submitted and not collected)");
+        return VT3D_E_INVALID;
+    }
     VT3D_HIP(hipSetDevice(s->device));
     s->views.resize(n);
     for (size_t i = 0; i < n; ++i) {
@@ -2712,8 +2756,11 @@ int32_t vt3d_batch_create(vt3d_session *s, const vt3d_request *reqs, size_t n,
     VT3D_HIP(hipSetDevice(s->device));
     std::unique_ptr<vt3d_batch> b(new vt3d_batch());
     b->session = s;
-    b->ws = &b->own;
-    b->stream = s->stream;
+    b->own = take_workspace(s);
+    b->ws = b->own.get();
+    // A stream of its own, so that batches of one session can be in flight together.
+    if (!b->own->stream) VT3D_HIP(hipStreamCreateWithFlags(&b->own->stream, hipStreamNonBlocking));
+    b->stream = b->own->stream;
     b->use_graph = use_graph != 0;
     for (size_t i = 0; i < n; ++i) {
         vt3d_sweep_params p;
@@ -2730,15 +2777,55 @@ int32_t vt3d_batch_create(vt3d_session *s, const vt3d_request *reqs, size_t n,
     return VT3D_OK;
 }
 
+namespace vt3d {
+namespace {
+
+void free_results(vt3d_batch *b, vt3d_dense_result **results) {
+    for (size_t i = 0; i < b->prepared.size(); ++i) {
+        vt3d_dense_result_free(results[i]);
+        results[i] = nullptr;
+    }
+}
+
+}  // namespace
+}  // namespace vt3d
+
 int32_t vt3d_batch_run(vt3d_batch *b, vt3d_dense_result **results) {
     if (!b || (!results && !b->prepared.empty())) return VT3D_E_INVALID;
-    int rc = run_batch(*b, results);
-    if (rc) {
-        for (size_t i = 0; i < b->prepared.size(); ++i) {
-            vt3d_dense_result_free(results[i]);
-            results[i] = nullptr;
-        }
+    if (b->submitted) {
+        set_error("the batch has a submitted launch; collect it first");
+        return VT3D_E_INVALID;
     }
+    int rc = run_batch(*b, results);
+    if (rc) free_results(b, results);
+    return rc;
+}
+
+int32_t vt3d_batch_submit(vt3d_batch *b) {
+    if (!b) return VT3D_E_INVALID;
+    if (b->submitted) {
+        set_error("the batch has a submitted launch; collect it first");
+        return VT3D_E_INVALID;
+    }
+    VT3D_HIP(hipSetDevice(b->session->device));
+    const int rc = launch_batch(*b);
+    if (rc) return rc;
+    b->submitted = true;
+    ++b->session->inflight;
+    return VT3D_OK;
+}
+
+int32_t vt3d_batch_collect(vt3d_batch *b, vt3d_dense_result **results) {
+    if (!b || (!results && !b->prepared.empty())) return VT3D_E_INVALID;
+    if (!b->submitted) {
+        set_error("the batch has no submitted launch to collect");
+        return VT3D_E_INVALID;
+    }
+    VT3D_HIP(hipSetDevice(b->session->device));
+    b->submitted = false;
+    --b->session->inflight;
+    const int rc = finish_batch(*b, results);
+    if (rc) free_results(b, results);
     return rc;
 }
 
@@ -2750,8 +2837,10 @@ int32_t vt3d_batch_last_timing(const vt3d_batch *b, vt3d_timing *t) {
 
 void vt3d_batch_destroy(vt3d_batch *b) {
     if (!b) return;
+    (void)hipSetDevice(b->session->device);
     if (b->stream) (void)hipStreamSynchronize(b->stream);
-    delete b;
+    if (b->submitted) --b->session->inflight;
+    delete b;  // its buffers go back to the session's pool
 }
 
 int32_t vt3d_server_create(vt3d_session *s, const vt3d_server_config *cfg, vt3d_server **out) {

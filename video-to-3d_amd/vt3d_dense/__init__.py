@@ -139,6 +139,10 @@ def _bind(lib: C.CDLL) -> None:
     lib.vt3d_batch_create.restype = C.c_int32
     lib.vt3d_batch_run.argtypes = [C.c_void_p, P(P(abi.DenseResult))]
     lib.vt3d_batch_run.restype = C.c_int32
+    lib.vt3d_batch_submit.argtypes = [C.c_void_p]
+    lib.vt3d_batch_submit.restype = C.c_int32
+    lib.vt3d_batch_collect.argtypes = [C.c_void_p, P(P(abi.DenseResult))]
+    lib.vt3d_batch_collect.restype = C.c_int32
     lib.vt3d_batch_last_timing.argtypes = [C.c_void_p, P(abi.Timing)]
     lib.vt3d_batch_last_timing.restype = C.c_int32
     lib.vt3d_batch_destroy.argtypes = [C.c_void_p]
@@ -163,6 +167,7 @@ EXPORTED_SYMBOLS = (
     "vt3d_session_destroy", "vt3d_session_upload_frames", "vt3d_session_solve",
     "vt3d_session_last_timing", "vt3d_last_error", "vt3d_plane_homography",
     "vt3d_session_set_tuning", "vt3d_batch_create", "vt3d_batch_run",
+    "vt3d_batch_submit", "vt3d_batch_collect",
     "vt3d_batch_last_timing", "vt3d_batch_destroy", "vt3d_server_create",
     "vt3d_server_submit", "vt3d_server_poll", "vt3d_server_get_stats", "vt3d_server_destroy",
     "vt3d_dense_estimate", "vt3d_multi_result_free", "vt3d_pack_dense_points",
@@ -569,7 +574,8 @@ def _fill_request(r: abi.Request, rq: dict, n_frames: int):
 
 class Batch:
     """Many solves on one session's frames, run as one kernel pipeline (optionally replayed
-    from a captured HIP graph). Build once, run many times."""
+    from a captured HIP graph) on a stream of the batch's own. Build once, run many times;
+    submit() and collect() split a run so that several batches can be in flight."""
 
     def __init__(self, session: Session, requests: Sequence[dict], use_graph: bool = False):
         lib = library()
@@ -590,6 +596,17 @@ class Batch:
         lib = library()
         outs = (C.POINTER(abi.DenseResult) * max(1, self.n))()
         _check(lib.vt3d_batch_run(self._h, outs))
+        return [_take(outs[i]) if outs[i] else None for i in range(self.n)]
+
+    def submit(self) -> None:
+        """Enqueues the batch's launches on its stream and returns (see vt3d_batch_submit)."""
+        _check(library().vt3d_batch_submit(self._h))
+
+    def collect(self) -> list:
+        """Waits for the submitted launch and returns what run() returns."""
+        lib = library()
+        outs = (C.POINTER(abi.DenseResult) * max(1, self.n))()
+        _check(lib.vt3d_batch_collect(self._h, outs))
         return [_take(outs[i]) if outs[i] else None for i in range(self.n)]
 
     def timing(self) -> abi.Timing:
