@@ -430,6 +430,8 @@ def precision_leg(V, scenes, rows: int, threads: int, log):
             res[storage] = r
             out[storage] = {"wall_ms": wall * 1e3, "device_ms": tm.total_ms,
                             "sweep_ms": tm.sweep_ms, "reciprocal_ms": tm.reciprocal_ms,
+                            "reverse_sweep_ms": tm.reverse_sweep_ms,
+                            "reciprocal_pairs": int(tm.reciprocal_pairs),
                             "units": int(tm.units), "units_per_s": tm.units / wall,
                             "sampled_pixels": int(r.stats["sampled_pixels"]),
                             "accepted_points": int(r.stats["accepted_points"])}
@@ -458,7 +460,10 @@ def precision_leg(V, scenes, rows: int, threads: int, log):
            "oracle_s": oracle_s, "oracle_threads": threads}
     if len(g) == n and n > 0 and np.array_equal(g["x"], os_["x"]):
         agree = (g["status"] == os_["status"]) & (g["best_hypothesis"] == os_["best_hypothesis"])
-        near = (slack["cost"] < 1e-4) | (slack["px"] < 1e-3) | (slack["rel"] < 1e-5)
+        # the parity tests' criterion: a decision may differ only where the oracle's margin
+        # to it is below the FP32 error (cost 1e-5 for ZNCC, 1e-3 px at a texture edge, 1e-5
+        # relative in the 8% depth test)
+        near = (slack["cost"] < 1e-5) | (slack["px"] < 1e-3) | (slack["rel"] < 1e-5)
         err["hypothesis_agree"] = float(agree.mean())
         err["disagree_explained_by_near_tie"] = int((~agree & near).sum())
         err["disagree_unexplained"] = int((~agree & ~near).sum())

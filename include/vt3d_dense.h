@@ -15,8 +15,10 @@
  *    with result->stats.attempted == 0 and result->skip_code / skip_reason set to the exact
  *    reference string (dense_legacy.rs:157-236, 274). Negative values are ABI misuse
  *    (VT3D_E_INVALID) or a device error (VT3D_E_DEVICE); no CPU fallback exists.
- *  - Calls are synchronous on the calling thread. Each call uses its own HIP stream on the
- *    current device; concurrent calls from different threads are allowed.
+ *  - Calls are synchronous on the calling thread, except vt3d_batch_submit (which returns
+ *    once the batch's launches are enqueued) and the serving loop's worker. Each call uses
+ *    its own HIP stream on the current device; concurrent calls from different threads are
+ *    allowed.
  */
 #ifndef VT3D_DENSE_H
 #define VT3D_DENSE_H
