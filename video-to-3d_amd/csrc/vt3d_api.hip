@@ -187,6 +187,7 @@ struct Workspace {
     DevBuf site_pt, prop_site, prop, prop_flag;                 // surface completion
     DevBuf c_points, c_sites, c_attrs, c_errors, c_supports;    // completed points, per pass
     HostBuf meta_host;
+    HostBuf in_host;   // staging of the plan's tables on their way to the device
     HostBuf out_host;  // staging of the packed outputs
     hipEvent_t ev[kStageEvents] = {};
     // A created batch's stream, which stays with these buffers in the session's pool: a
@@ -232,7 +233,7 @@ struct Workspace {
     // Carves every buffer from `a` and stages through heap memory instead of pinned memory.
     void bind(Arena *a) {
         each_buffer([&](DevBuf &b) { b.arena = a; });
-        meta_host.pageable = out_host.pageable = true;
+        meta_host.pageable = in_host.pageable = out_host.pageable = true;
     }
     ~Workspace() {
         if (stream) {
@@ -1070,18 +1071,40 @@ int upload_plan(vt3d_batch &b) {
         pd.depth_p = ws.depth_p.as<double>() + reinterpret_cast<size_t>(pd.depth_p);
         pd.depth_r = ws.depth_r.as<double>() + reinterpret_cast<size_t>(pd.depth_r);
     }
-    VT3D_HIP(hipMemcpyAsync(ws.prob.p, pl.prob.data(), sizeof(ProblemDev) * n,
-                            hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(ws.src.p, pl.srcs.data(), sizeof(SourceDev) * pl.srcs.size(),
-                            hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(ws.Hp.p, pl.Hp.data(), 8 * pl.Hp.size(), hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(ws.Hr.p, pl.Hr.data(), 8 * pl.Hr.size(), hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(ws.depth_p.p, pl.dp.data(), 8 * pl.dp.size(), hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(ws.depth_r.p, pl.dr.data(), 8 * pl.dr.size(), hipMemcpyHostToDevice, st));
-    VT3D_HIP(hipMemcpyAsync(ws.block_problem.p, pl.block_problem.data(), 4ull * pl.n_blocks,
-                            hipMemcpyHostToDevice, st));
-    // The table vectors are pageable: wait for the copies before they can be freed or reused.
-    VT3D_HIP(hipStreamSynchronize(st));
+    // The tables travel through the workspace's pinned staging buffer, so the copies are
+    // asynchronous and the host does not wait for them: with other batches busy on the
+    // device, a copy can start long after it is enqueued. The staging belongs to the
+    // workspace, which serves one batch at a time, and a batch uploads once, before its first
+    // launch; the next write to it comes after the stream has drained. Pageable staging (the
+    // no-allocation path) is waited for.
+    struct Piece {
+        void *dst;
+        const void *src;
+        size_t bytes;
+    };
+    const Piece pieces[] = {
+        {ws.prob.p, pl.prob.data(), sizeof(ProblemDev) * n},
+        {ws.src.p, pl.srcs.data(), sizeof(SourceDev) * pl.srcs.size()},
+        {ws.Hp.p, pl.Hp.data(), 8 * pl.Hp.size()},
+        {ws.Hr.p, pl.Hr.data(), 8 * pl.Hr.size()},
+        {ws.depth_p.p, pl.dp.data(), 8 * pl.dp.size()},
+        {ws.depth_r.p, pl.dr.data(), 8 * pl.dr.size()},
+        {ws.block_problem.p, pl.block_problem.data(), 4ull * pl.n_blocks},
+    };
+    auto padded = [](size_t bytes) { return (bytes + 255) & ~(size_t)255; };
+    size_t staged = 0;
+    for (const Piece &q : pieces) staged += padded(q.bytes);
+    if ((rc = ws.in_host.ensure(staged))) return rc;
+    uint8_t *stage = ws.in_host.as<uint8_t>();
+    size_t at = 0;
+    for (const Piece &q : pieces) {
+        if (q.bytes) {
+            std::memcpy(stage + at, q.src, q.bytes);
+            VT3D_HIP(hipMemcpyAsync(q.dst, stage + at, q.bytes, hipMemcpyHostToDevice, st));
+        }
+        at += padded(q.bytes);
+    }
+    if (ws.in_host.pageable) VT3D_HIP(hipStreamSynchronize(st));
 
     Args &a = b.args;
     a = Args{};
