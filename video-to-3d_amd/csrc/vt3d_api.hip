@@ -2785,13 +2785,40 @@ int32_t vt3d_batch_create(vt3d_session *s, const vt3d_request *reqs, size_t n,
     if (!b->own->stream) VT3D_HIP(hipStreamCreateWithFlags(&b->own->stream, hipStreamNonBlocking));
     b->stream = b->own->stream;
     b->use_graph = use_graph != 0;
-    for (size_t i = 0; i < n; ++i) {
+    // Requests are prepared independently (host selection, envelopes, remaps; on session
+    // frames no device work), so a large batch prepares them on host threads. The call
+    // returns the first failing request's error, in request order, as a sequential loop would.
+    for (size_t i = 0; i < n; ++i) b->prepared.emplace_back(new Prepared());
+    std::vector<int> rcs(n, VT3D_OK);
+    std::vector<std::string> errs(n);
+    auto prep = [&](size_t i) {
         vt3d_sweep_params p;
-        int rc;
-        if ((rc = check_request(s, reqs[i], &p))) return rc;
-        b->prepared.emplace_back(new Prepared());
-        if ((rc = prepare_request(s, reqs[i], p, *b->prepared.back()))) return rc;
+        int rc = check_request(s, reqs[i], &p);
+        if (!rc) rc = prepare_request(s, reqs[i], p, *b->prepared[i]);
+        if (rc) {
+            rcs[i] = rc;
+            errs[i] = g_error;
+        }
+    };
+    const size_t n_threads =
+        n < 16 ? 1
+               : std::min<size_t>({n / 8, (size_t)std::max(1u, std::thread::hardware_concurrency()),
+                                   (size_t)16});
+    if (n_threads <= 1) {
+        for (size_t i = 0; i < n; ++i) {
+            prep(i);
+            if (rcs[i] != VT3D_OK) break;
+        }
+    } else {
+        run_shares(n_threads, [&](size_t t) {
+            for (size_t i = t; i < n; i += n_threads) prep(i);
+        });
     }
+    for (size_t i = 0; i < n; ++i)
+        if (rcs[i] != VT3D_OK) {
+            set_error(errs[i]);
+            return rcs[i];
+        }
     std::vector<Prepared *> reqp;
     for (auto &u : b->prepared) reqp.push_back(u.get());
     int rc = build_plan(reqp, s->tuning, b->plan);
