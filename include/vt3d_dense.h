@@ -565,6 +565,10 @@ int32_t vt3d_session_last_timing(const vt3d_session *session, vt3d_timing *timin
 typedef struct vt3d_tuning {
     int32_t tile_width;
     int32_t target_waves;
+    /* ZNCC reverse sweep: drop hypotheses that a bound over a patch's first rows of taps shows
+     * to cost more than a cost already reached (0 or 1: on, the default; -1: off). The
+     * results are the same either way. */
+    int32_t reverse_bound;
 } vt3d_tuning;
 int32_t vt3d_session_set_tuning(vt3d_session *session, const vt3d_tuning *tuning);
 

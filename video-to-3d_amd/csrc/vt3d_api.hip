@@ -256,7 +256,7 @@ struct vt3d_session {
     std::vector<std::unique_ptr<vt3d::DevView>> views;  // uploaded frames by frame index
     vt3d::Workspace ws;                                  // one-shot and session solves
     vt3d_timing timing{};
-    vt3d_tuning tuning{32, 4096};
+    vt3d_tuning tuning{32, 4096, 1};
     // Buffers of destroyed batches, kept for the next batches (at most kPooledWorkspaces sets),
     // so that a stream of batches allocates and frees no device memory once warm: hipFree
     // would wait for every stream of the device, batches in flight included.
@@ -868,6 +868,7 @@ int build_plan(const std::vector<Prepared *> &reqs, const vt3d_tuning &tuning, P
         pd.tiles = std::max<uint32_t>(1, (uint32_t)pd.tiles_x * ((pr.ny + ty - 1) / ty));
         pd.emit_samples = pr.p.emit_samples ? 1 : 0;
         pd.passes = pr.p.surface_completion_passes;
+        pd.recip_bound = tuning.reverse_bound != 0 ? 1 : 0;
         pl.max_passes = std::max(pl.max_passes, pd.passes);
         pl.emit = pl.emit || pd.emit_samples;
         for (uint32_t b = 0; b < pd.cap / kBlock; ++b) pl.block_problem.push_back(k);
@@ -2745,9 +2746,10 @@ int32_t vt3d_session_set_tuning(vt3d_session *s, const vt3d_tuning *t) {
         set_error("tile_width must be 16, 32 or 64");
         return VT3D_E_INVALID;
     }
-    if (t->target_waves < 0) return VT3D_E_INVALID;
+    if (t->target_waves < 0 || t->reverse_bound < -1 || t->reverse_bound > 1) return VT3D_E_INVALID;
     s->tuning.tile_width = t->tile_width ? t->tile_width : 32;
     s->tuning.target_waves = t->target_waves ? t->target_waves : 4096;
+    s->tuning.reverse_bound = t->reverse_bound < 0 ? 0 : 1;
     return VT3D_OK;
 }
 

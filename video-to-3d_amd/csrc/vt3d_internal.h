@@ -85,7 +85,7 @@ struct ProblemDev {
     int emit_samples;
     int passes;              // surface completion passes
     uint32_t prop_item0;     // first work item of the completion proposal kernel
-    uint32_t pad1;
+    int recip_bound;         // ZNCC reverse sweep: drop hypotheses a partial-patch bound excludes
 };
 
 // Best and comparable-second candidate of one sample over a block of hypotheses, in the

@@ -391,22 +391,26 @@ __device__ __forceinline__ bool corners_inside(const Warp &w, uint32_t sw, uint3
     return ok;
 }
 
-// The taps of a valid patch: each warped relative to the centre's cell, gathered, and
-// reduced to the patch cost.
-template <int R, int COST, int ST, bool BUF>
-__device__ __forceinline__ float patch_taps(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
-                                            const RefStats &st, const Warp &w,
-                                            const TexRef<ST, BUF> &tex) {
+// Running sums of a patch's taps: ZNCC Σd, Σd², Σ r̂ d (d = value - pivot); ZSAD the values.
+template <int T, int COST> struct TapSums {
+    float s1 = 0.f, s2 = 0.f, srs = 0.f;
+    float sv[COST == VT3D_COST_ZSAD ? T : 1];
+};
+
+// Rows [Y0, Y1) of a valid patch's taps (dy = Y0 - R .. Y1 - 1 - R), each warped relative to
+// the centre's cell, gathered and added to the sums. Splitting a patch into row ranges keeps
+// every addition in the order of the whole patch's loop.
+template <int R, int COST, int ST, bool BUF, int Y0, int Y1>
+__device__ __forceinline__ void tap_rows(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
+                                         float pivot, const Warp &w, const TexRef<ST, BUF> &tex,
+                                         TapSums<(2 * R + 1) * (2 * R + 1), COST> &ts) {
     constexpr int P = 2 * R + 1;
-    constexpr int T = P * P;
     const uint32_t sw = tex.w;
     const uint32_t base = (uint32_t)(w.iyc * (int)sw + w.ixc);
     const uint32_t base_bytes = base * TexRef<ST, true>::kTexelBytes;
-    const float pivot = st.mean;
-    float s1 = 0.f, s2 = 0.f, srs = 0.f;
-    float sv[COST == VT3D_COST_ZSAD ? T : 1];
+    float &s1 = ts.s1, &s2 = ts.s2, &srs = ts.srs;
 #pragma unroll
-    for (int dy = -R; dy <= R; ++dy) {
+    for (int dy = Y0 - R; dy < Y1 - R; ++dy) {
         const float fdy = (float)dy;
         const float row_den = fmaf(w.e, fdy, 1.f);
         const float row_x = w.bx * fdy;
@@ -437,7 +441,7 @@ __device__ __forceinline__ float patch_taps(const float (&rp)[(2 * R + 1) * (2 *
             const int k = (dy + R) * P + (dx + R);
             const float v = bicoef(kk[dx + R], tx[dx + R], ty[dx + R]);
             if constexpr (COST == VT3D_COST_ZSAD) {
-                sv[k] = v;
+                ts.sv[k] = v;
                 s1 += v;
             } else {
                 const float d = v - pivot;
@@ -447,7 +451,43 @@ __device__ __forceinline__ float patch_taps(const float (&rp)[(2 * R + 1) * (2 *
             }
         }
     }
-    return finish_cost<COST, T>(rp, st, s1, s2, srs, sv);
+}
+
+// The taps of a valid patch reduced to the patch cost.
+template <int R, int COST, int ST, bool BUF>
+__device__ __forceinline__ float patch_taps(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
+                                            const RefStats &st, const Warp &w,
+                                            const TexRef<ST, BUF> &tex) {
+    constexpr int P = 2 * R + 1;
+    constexpr int T = P * P;
+    TapSums<T, COST> ts;
+    tap_rows<R, COST, ST, BUF, 0, P>(rp, st.mean, w, tex, ts);
+    return finish_cost<COST, T>(rp, st, ts.s1, ts.s2, ts.srs, ts.sv);
+}
+
+// Whether a ZNCC patch cost certainly exceeds `thr`, from the sums over its first NS taps
+// (s = Σd, B = Σd², A = Σ r̂ d) and, of the unit, zero-mean reference vector r̂, the sum srS
+// and the centred energy E = Σ r̂² - srS² / NS over those taps. Whatever the unseen taps hold,
+// the patch's correlation is at most sqrt(c² + 1 - E), where c is the seen taps' centred
+// correlation (Cauchy-Schwarz over the unseen values, the patch mean included). The test
+// carries allowances for the FP32 rounding of the sums (3e-5 of their magnitudes, several
+// times the bound for NS <= 121 terms) and of r̂'s norm, and a 1e-4 margin on the cost, so a
+// hypothesis it drops costs more than `thr` in the kernel's own arithmetic as well.
+template <int NS>
+__device__ __forceinline__ bool zncc_exceeds(float s, float B, float A, float srS, float E,
+                                             float thr) {
+    const float lim = 1.f - thr - 1.0e-4f;  // the correlation below which the cost is > thr
+    if (!(lim > 0.f)) return false;
+    constexpr float kInvNS = 1.0f / (float)NS;
+    const float sm = s * kInvNS;
+    const float N0 = fmaf(-srS, sm, A);  // Σ r̂ (d - mean of the seen d)
+    const float V = fmaf(-s, sm, B);     // Σ (d - mean of the seen d)²
+    const float Vlo = fmaf(-3.0e-5f, B, V);
+    if (!(Vlo > 1.0e-6f * B) || !(Vlo > 0.f)) return false;
+    const float Nhi = fabsf(N0) + 3.0e-5f * (sqrtf(B) + fabsf(srS * sm));
+    const float rest = (1.0f + 1.0e-5f) - E;
+    // correlation² <= Nhi² / Vlo + rest < lim²
+    return fmaf(Nhi, Nhi, rest * Vlo) < lim * lim * Vlo;
 }
 
 // patch_error (dense_legacy.rs:1031-1095) for one (sample, hypothesis, source): every tap of
@@ -1425,6 +1465,11 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
 
 // ---- k_recip_pre / k_recip -----------------------------------------------------------------
 
+// Rows of a P x P patch (P = 2R + 1) summed before the reverse sweep's first and second test
+// of its partial-patch bound (1 <= first < second < P).
+__host__ __device__ constexpr int recip_bound_rows1(int R) { return R; }
+__host__ __device__ constexpr int recip_bound_rows2(int R) { return R + 2 < 2 * R + 1 ? R + 2 : 2 * R; }
+
 // reciprocal_depth_observations (dense_legacy.rs:795-856) in two launches over the same work
 // items, a work item being (problem, source, block of 256 samples), so every wave shares the
 // source and its reverse homographies. Sources are outer within a problem: each source's list
@@ -1524,7 +1569,30 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     int best_h = -1;
     const double *Hs = pd.Hr + (size_t)s * D * kHrStride;
     const float midx = 0.5f * (float)(pd.ref.w - 1), midy = 0.5f * (float)(pd.ref.h - 1);
-    for (int hyp = 0; hyp < D; ++hyp) {
+    // The sweep keeps the first minimum in hypothesis order, so a hypothesis that certainly
+    // costs more than a cost some hypothesis reaches cannot be it. With ZNCC, a bound from the
+    // patch's first rows of taps (zncc_exceeds, tested after K1 and after K2 rows) drops such
+    // hypotheses before their remaining rows; the threshold is the photometric gate, then the
+    // best cost found. Hypotheses that are not dropped are summed exactly as without the
+    // bound, so the result is the same.
+    // rows of taps before the first and the second test of the bound
+    constexpr int K1 = recip_bound_rows1(R), K2 = recip_bound_rows2(R);
+    constexpr int N1 = K1 * P, N2 = K2 * P;
+    float srS1 = 0.f, E1 = 0.f, srS2 = 0.f, E2 = 0.f;
+    const bool bounded = COST == VT3D_COST_ZNCC && pd.recip_bound != 0;  // uniform
+    if (bounded) {
+        float q = 0.f;
+#pragma unroll
+        for (int k = 0; k < N2; ++k) {
+            if (k == N1) E1 = fmaf(-srS1, srS1 * (1.0f / (float)N1), q);
+            srS2 += rp[k];
+            q = fmaf(rp[k], rp[k], q);
+            if (k < N1) srS1 = srS2;
+        }
+        E2 = fmaf(-srS2, srS2 * (1.0f / (float)N2), q);
+    }
+    // The cost of hypothesis `hyp`, or +inf where the bound shows it exceeds `thr`.
+    auto cost_of = [&](int hyp, float thr) -> float {
         const double *rec = Hs + (size_t)hyp * kHrStride;
         double h[9];
         load_h(rec, h);
@@ -1538,11 +1606,60 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
         bool valid = certain;
         if (!certain && w.ok) valid = corners_inside<R>(w, pd.ref.w, pd.ref.h);
         float c = kInfF;
-        if (valid) c = patch_taps<R, COST, ST, true>(rp, st, w, ref_tex);
-        // cost ascending, then depth (hypothesis) ascending: strict < keeps the first
-        if (c <= maxc && c < best_c) {
-            best_c = c;
-            best_h = hyp;
+        if (valid) {
+            if constexpr (COST == VT3D_COST_ZNCC) {
+                TapSums<T, COST> ts;
+                tap_rows<R, COST, ST, true, 0, K1>(rp, st.mean, w, ref_tex, ts);
+                if (!zncc_exceeds<N1>(ts.s1, ts.s2, ts.srs, srS1, E1, thr)) {
+                    tap_rows<R, COST, ST, true, K1, K2>(rp, st.mean, w, ref_tex, ts);
+                    if (!zncc_exceeds<N2>(ts.s1, ts.s2, ts.srs, srS2, E2, thr)) {
+                        tap_rows<R, COST, ST, true, K2, P>(rp, st.mean, w, ref_tex, ts);
+                        c = finish_cost<COST, T>(rp, st, ts.s1, ts.s2, ts.srs, ts.sv);
+                    }
+                }
+            } else {
+                c = patch_taps<R, COST, ST, true>(rp, st, w, ref_tex);
+            }
+        }
+        return c;
+    };
+    if (bounded) {
+        // Each lane walks its hypotheses outward from the one nearest the expected source
+        // depth zs, so the lanes of a wave meet their likely minima first and the far
+        // hypotheses, which the bound drops, together. The minimum is kept by (cost,
+        // hypothesis), the order the in-order sweep's first minimum has.
+        int h0 = 0;
+        const double inv_min = 1.0 / src.min_depth, inv_max = 1.0 / src.max_depth;
+        if (D > 1 && zs > 0.0 && isfinite(zs) && inv_min != inv_max) {
+            const double t = (inv_min - 1.0 / zs) / (inv_min - inv_max) * (double)(D - 1);
+            h0 = (int)fmin(fmax(rint(t), 0.0), (double)(D - 1));
+        }
+        const int m = min(h0, D - 1 - h0);
+        const bool right = h0 < D - 1 - h0;
+        float thr = st.flat ? kInfF : maxc;
+        for (int j = 0; j < D; ++j) {
+            int hyp;
+            if (j <= 2 * m) {
+                const int k = (j + 1) >> 1;
+                hyp = (j & 1) ? h0 + k : h0 - k;
+            } else {
+                hyp = right ? h0 + j - m : h0 - (j - m);
+            }
+            const float c = cost_of(hyp, thr);
+            if (c <= maxc && (c < best_c || (c == best_c && hyp < best_h))) {
+                best_c = c;
+                best_h = hyp;
+                if (!st.flat) thr = fminf(thr, c);
+            }
+        }
+    } else {
+        for (int hyp = 0; hyp < D; ++hyp) {
+            const float c = cost_of(hyp, kInfF);
+            // cost ascending, then depth (hypothesis) ascending: strict < keeps the first
+            if (c <= maxc && c < best_c) {
+                best_c = c;
+                best_h = hyp;
+            }
         }
     }
     bool ok = best_h >= 0;

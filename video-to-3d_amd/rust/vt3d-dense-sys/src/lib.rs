@@ -400,6 +400,7 @@ pub struct vt3d_mesh_result {
 pub struct vt3d_tuning {
     pub tile_width: i32,
     pub target_waves: i32,
+    pub reverse_bound: i32,
 }
 
 /// No counterpart in the reference (which solves one reference at a time on the caller's thread). Uneven solves -- different frame sizes, sources, hypotheses and patch sizes -- share each stage's launch; with use_graph the batch's launches are captured in a HIP graph on the first run and replayed afterwards.

@@ -528,9 +528,11 @@ class Session:
         _check(library().vt3d_session_last_timing(self._h, C.byref(t)))
         return t
 
-    def set_tuning(self, tile_width: int = 0, target_waves: int = 0) -> None:
-        """Launch shape of later solves and batches (0 keeps a default)."""
-        t = abi.Tuning(int(tile_width), int(target_waves))
+    def set_tuning(self, tile_width: int = 0, target_waves: int = 0,
+                   reverse_bound: int = 0) -> None:
+        """Launch shape of later solves and batches (0 keeps a default); reverse_bound -1
+        turns off the ZNCC reverse sweep's partial-patch bound (same results, more work)."""
+        t = abi.Tuning(int(tile_width), int(target_waves), int(reverse_bound))
         _check(library().vt3d_session_set_tuning(self._h, C.byref(t)))
 
     def batch(self, requests: Sequence[dict], use_graph: bool = False) -> "Batch":

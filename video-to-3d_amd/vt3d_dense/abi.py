@@ -126,7 +126,8 @@ class Timing(C.Structure):
 
 
 class Tuning(C.Structure):
-    _fields_ = [("tile_width", C.c_int32), ("target_waves", C.c_int32)]
+    _fields_ = [("tile_width", C.c_int32), ("target_waves", C.c_int32),
+                ("reverse_bound", C.c_int32)]
 
 
 class Request(C.Structure):
