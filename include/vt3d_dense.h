@@ -41,6 +41,10 @@ extern "C" {
 #define VT3D_E_BUDGET (-4)
 #define VT3D_E_BUSY (-5)
 #define VT3D_E_CANCELED (-6) /* a content hash polled a cancel ("reconstruction canceled") */
+/* The stage artifact's errors (ReconstructionStageError, reconstruction_stages.rs:62-97). */
+#define VT3D_E_NONFINITE (-7) /* NonFinite: a NaN or infinite float; the field path is reported */
+#define VT3D_E_SCHEMA (-8)    /* UnsupportedSchemaVersion: the version found is reported */
+#define VT3D_E_MALFORMED (-9) /* Malformed: not a DenseStageArtifact document */
 
 /* FrameInput (crates/video-to-3d-core/src/reconstruction/types.rs:16-21):
  * RGBA8 row-major, 4*width bytes per row. */
@@ -192,7 +196,9 @@ enum vt3d_skip_code {
     VT3D_SKIP_NO_REFERENCE_FRAME = 7,  /* dense_legacy.rs:205-207 */
     VT3D_SKIP_NO_SOURCES = 8,          /* dense_legacy.rs:232-236 */
     VT3D_SKIP_CANCELED = 9,            /* dense_legacy.rs:273-275 */
-    VT3D_SKIP_WORKING_SET = 10         /* dense.rs:385-404 (budget; formatted reason) */
+    VT3D_SKIP_WORKING_SET = 10,        /* dense.rs:385-404 (budget; formatted reason) */
+    VT3D_SKIP_OTHER = 11               /* a reason this library does not write, read from a
+                                          stored document (vt3d_dense_artifact_from_json) */
 };
 
 /* DenseAnalysis (dense_legacy.rs:61-68), library-owned. points/grid_sites/attributes are
@@ -444,6 +450,57 @@ int32_t vt3d_dense_estimate_with_budget(const vt3d_frame *frames, size_t n_frame
                                         uint32_t descriptor_radius,
                                         const vt3d_sweep_params *params, vt3d_poll_fn poll,
                                         void *poll_user, vt3d_multi_result **out);
+
+/* ---- the dense stage's stored document (reconstruction_stages.rs:150-160, 287-400) ------ */
+
+#define VT3D_DENSE_STAGE_ARTIFACT_SCHEMA_VERSION 1 /* reconstruction_stages.rs:37 */
+
+/* DenseStageArtifact (reconstruction_stages.rs:150-160) with its DenseEvidence (:244-252):
+ * `dense` holds the stats, points, grid sites and attributes; `warnings` the dense stage's
+ * warnings as NUL-terminated UTF-8 strings. Library-owned: free with
+ * vt3d_dense_artifact_free. */
+typedef struct vt3d_dense_artifact {
+    uint32_t schema_version;
+    uint32_t pad;
+    char request_hash[VT3D_CONTENT_HASH_LEN];
+    char cameras_hash[VT3D_CONTENT_HASH_LEN];
+    vt3d_multi_result *dense;
+    size_t n_warnings;
+    char **warnings;
+} vt3d_dense_artifact;
+
+/* The artifact step of reconstruct_dense_with_cancel (reconstruction_stages.rs:570-589) over a
+ * dense estimate (vt3d_dense_estimate_with_budget's result, which the reference computes at
+ * :762-771): copies `dense`, adds the dense stage's warnings (:772-782: the working-set
+ * notice when the view limit applied, then dense_warning), records the two identities and
+ * checks every float. VT3D_E_INVALID for a hash that is not "sha256:" and 64 lowercase hex
+ * digits (ContentHash::new, scene_project.rs:447-461); VT3D_E_NONFINITE with the dotted path
+ * of the first NaN or infinite float in `field` (check_finite_with_control, :375-390, e.g.
+ * "dense.points.3.x"). Host only. */
+int32_t vt3d_dense_artifact_create(const vt3d_multi_result *dense, const char *request_hash,
+                                   const char *cameras_hash, vt3d_dense_artifact **out,
+                                   char *field, size_t field_cap);
+/* to_json_with_cancel (reconstruction_stages.rs:319-330): the stored document, serde_json's
+ * compact encoding with f32 values as ryu writes them. Written into buf when cap exceeds its
+ * length (NUL-terminated); *len receives the length either way. VT3D_E_SCHEMA for an
+ * artifact whose schema_version is not 1; VT3D_E_NONFINITE as for create. */
+int32_t vt3d_dense_artifact_to_json(const vt3d_dense_artifact *artifact, char *buf, size_t cap,
+                                    size_t *len, char *field, size_t field_cap);
+/* DenseStageArtifact::content_hash (reconstruction_stages.rs:359-363): SHA-256 of to_json's
+ * bytes (non-finite floats written as null, as serde_json does), the identity a mesh artifact
+ * records as dense_hash. */
+int32_t vt3d_dense_artifact_content_hash(const vt3d_dense_artifact *artifact,
+                                         char out[VT3D_CONTENT_HASH_LEN]);
+/* DenseStageArtifact::from_json (reconstruction_stages.rs:295-317): VT3D_E_SCHEMA with
+ * *found_version for a document whose schema_version is not 1 (checked first);
+ * VT3D_E_MALFORMED with a message for invalid JSON, a missing schema_version or a document
+ * that is not a DenseStageArtifact (unknown, duplicate or missing fields, wrong types,
+ * integers out of range, an invalid content hash); VT3D_E_NONFINITE with the field path
+ * where a number overflows f32. The messages are this library's, not serde_json's text. */
+int32_t vt3d_dense_artifact_from_json(const char *document, size_t len,
+                                      vt3d_dense_artifact **out, uint32_t *found_version,
+                                      char *message, size_t message_cap);
+void vt3d_dense_artifact_free(vt3d_dense_artifact *artifact);
 
 /* ---- mesh consumer: reference-grid triangulation of the dense points -------------------- */
 

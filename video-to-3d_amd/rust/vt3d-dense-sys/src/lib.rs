@@ -18,10 +18,14 @@ pub const VT3D_E_NO_DEVICE: i32 = -3;
 pub const VT3D_E_BUDGET: i32 = -4;
 pub const VT3D_E_BUSY: i32 = -5;
 pub const VT3D_E_CANCELED: i32 = -6;
+pub const VT3D_E_NONFINITE: i32 = -7;
+pub const VT3D_E_SCHEMA: i32 = -8;
+pub const VT3D_E_MALFORMED: i32 = -9;
 pub const VT3D_CONTENT_HASH_LEN: usize = 72;
 pub const VT3D_MAX_REFERENCE_VIEWS: usize = 3;
 pub const VT3D_MAX_REFERENCE_ATTEMPTS: usize = 6;
 pub const VT3D_MIN_REFERENCE_PATCH_POINTS: i32 = 3;
+pub const VT3D_DENSE_STAGE_ARTIFACT_SCHEMA_VERSION: i32 = 1;
 
 pub const VT3D_COST_ZSAD: i32 = 0;
 pub const VT3D_COST_ZNCC: i32 = 1;
@@ -50,6 +54,7 @@ pub const VT3D_SKIP_NO_REFERENCE_FRAME: i32 = 7;
 pub const VT3D_SKIP_NO_SOURCES: i32 = 8;
 pub const VT3D_SKIP_CANCELED: i32 = 9;
 pub const VT3D_SKIP_WORKING_SET: i32 = 10;
+pub const VT3D_SKIP_OTHER: i32 = 11;
 
 /// ReconstructionControl poll: nonzero cancels (called on the calling thread).
 pub type vt3d_poll_fn = Option<unsafe extern "C" fn(user: *mut c_void) -> i32>;
@@ -334,6 +339,19 @@ pub struct vt3d_multi_result {
     pub reference_view_limit: i32,
 }
 
+/// DenseStageArtifact (reconstruction_stages.rs:150-160) with its DenseEvidence (:244-252): `dense` holds the stats, points, grid sites and attributes; `warnings` the dense stage's warnings as NUL-terminated UTF-8 strings. Library-owned: free with vt3d_dense_artifact_free.
+#[repr(C)]
+#[derive(Clone, Copy)]
+pub struct vt3d_dense_artifact {
+    pub schema_version: u32,
+    pub pad: u32,
+    pub request_hash: [c_char; VT3D_CONTENT_HASH_LEN],
+    pub cameras_hash: [c_char; VT3D_CONTENT_HASH_LEN],
+    pub dense: *mut vt3d_multi_result,
+    pub n_warnings: usize,
+    pub warnings: *mut *mut c_char,
+}
+
 /// MeshTriangle (mesh_legacy.rs:17-23): indices into the dense points.
 #[repr(C)]
 #[derive(Clone, Copy)]
@@ -474,6 +492,11 @@ extern "C" {
     pub fn vt3d_multi_result_free(result: *mut vt3d_multi_result);
     pub fn vt3d_estimate_working_set(frames: *const vt3d_frame, n_frames: usize, max_features: u64, descriptor_radius: u32, reference_views: usize, fine_grid: i32, out: *mut vt3d_working_set);
     pub fn vt3d_dense_estimate_with_budget(frames: *const vt3d_frame, n_frames: usize, cameras: *const vt3d_camera, n_cameras: usize, sparse_xyz: *const f64, n_sparse: usize, focal: f64, working_set_budget_bytes: u64, max_features: u64, descriptor_radius: u32, params: *const vt3d_sweep_params, poll: vt3d_poll_fn, poll_user: *mut c_void, out: *mut *mut vt3d_multi_result) -> i32;
+    pub fn vt3d_dense_artifact_create(dense: *const vt3d_multi_result, request_hash: *const c_char, cameras_hash: *const c_char, out: *mut *mut vt3d_dense_artifact, field: *mut c_char, field_cap: usize) -> i32;
+    pub fn vt3d_dense_artifact_to_json(artifact: *const vt3d_dense_artifact, buf: *mut c_char, cap: usize, len: *mut usize, field: *mut c_char, field_cap: usize) -> i32;
+    pub fn vt3d_dense_artifact_content_hash(artifact: *const vt3d_dense_artifact, out: *mut c_char) -> i32;
+    pub fn vt3d_dense_artifact_from_json(document: *const c_char, len: usize, out: *mut *mut vt3d_dense_artifact, found_version: *mut u32, message: *mut c_char, message_cap: usize) -> i32;
+    pub fn vt3d_dense_artifact_free(artifact: *mut vt3d_dense_artifact);
     pub fn vt3d_dense_mesh(points: *const vt3d_point3, n_points: usize, sites: *const vt3d_grid_site, n_sites: usize, dense: *const vt3d_dense_stats, patches: *const vt3d_reference_patch_stats, n_patches: usize, cameras: *const vt3d_camera, n_cameras: usize, width: u32, height: u32, focal: f64, poll: vt3d_poll_fn, poll_user: *mut c_void, out: *mut *mut vt3d_mesh_result) -> i32;
     pub fn vt3d_mesh_result_free(result: *mut vt3d_mesh_result);
     pub fn vt3d_mesh_warning(stats: *const vt3d_mesh_stats, buf: *mut c_char, cap: usize) -> usize;

@@ -99,6 +99,20 @@ def _bind(lib: C.CDLL) -> None:
     lib.vt3d_content_hash.restype = None
     lib.vt3d_multi_result_free.argtypes = [P(abi.MultiResult)]
     lib.vt3d_multi_result_free.restype = None
+    lib.vt3d_dense_artifact_create.argtypes = [P(abi.MultiResult), C.c_char_p, C.c_char_p,
+                                               P(P(abi.DenseArtifact)), C.c_char_p, C.c_size_t]
+    lib.vt3d_dense_artifact_create.restype = C.c_int32
+    lib.vt3d_dense_artifact_to_json.argtypes = [P(abi.DenseArtifact), C.c_void_p, C.c_size_t,
+                                                P(C.c_size_t), C.c_char_p, C.c_size_t]
+    lib.vt3d_dense_artifact_to_json.restype = C.c_int32
+    lib.vt3d_dense_artifact_content_hash.argtypes = [P(abi.DenseArtifact), C.c_char_p]
+    lib.vt3d_dense_artifact_content_hash.restype = C.c_int32
+    lib.vt3d_dense_artifact_from_json.argtypes = [C.c_char_p, C.c_size_t,
+                                                  P(P(abi.DenseArtifact)), P(C.c_uint32),
+                                                  C.c_char_p, C.c_size_t]
+    lib.vt3d_dense_artifact_from_json.restype = C.c_int32
+    lib.vt3d_dense_artifact_free.argtypes = [P(abi.DenseArtifact)]
+    lib.vt3d_dense_artifact_free.restype = None
     u8p = P(C.c_uint8)
     lib.vt3d_pack_dense_points.argtypes = [P(abi.Point3), C.c_size_t, u8p, u8p]
     lib.vt3d_pack_dense_points.restype = C.c_size_t
@@ -175,7 +189,9 @@ EXPORTED_SYMBOLS = (
     "vt3d_dense_warning", "vt3d_dense_mesh", "vt3d_mesh_result_free", "vt3d_mesh_warning",
     "vt3d_dense_plane_sweep_noalloc", "vt3d_estimate_working_set",
     "vt3d_dense_estimate_with_budget", "vt3d_reconstruction_options_default",
-    "vt3d_request_content_hash", "vt3d_content_hash",
+    "vt3d_request_content_hash", "vt3d_content_hash", "vt3d_dense_artifact_create",
+    "vt3d_dense_artifact_to_json", "vt3d_dense_artifact_content_hash",
+    "vt3d_dense_artifact_from_json", "vt3d_dense_artifact_free",
 )
 
 

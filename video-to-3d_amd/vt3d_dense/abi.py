@@ -223,6 +223,22 @@ class MultiResult(C.Structure):
                 ("has_reference_view_limit", C.c_int32), ("reference_view_limit", C.c_int32)]
 
 
+CONTENT_HASH_LEN = 72
+DENSE_STAGE_ARTIFACT_SCHEMA_VERSION = 1  # reconstruction_stages.rs:37
+VT3D_E_NONFINITE = -7
+VT3D_E_SCHEMA = -8
+VT3D_E_MALFORMED = -9
+
+
+class DenseArtifact(C.Structure):
+    """vt3d_dense_artifact: DenseStageArtifact (reconstruction_stages.rs:150-160)."""
+    _fields_ = [("schema_version", C.c_uint32), ("pad", C.c_uint32),
+                ("request_hash", C.c_char * CONTENT_HASH_LEN),
+                ("cameras_hash", C.c_char * CONTENT_HASH_LEN),
+                ("dense", C.POINTER(MultiResult)), ("n_warnings", C.c_size_t),
+                ("warnings", C.POINTER(C.c_char_p))]
+
+
 class MeshTriangle(C.Structure):
     _fields_ = [("a", C.c_uint64), ("b", C.c_uint64), ("c", C.c_uint64),
                 ("confidence", C.c_float), ("pad", C.c_uint32)]
@@ -335,6 +351,9 @@ class MultiAnalysis:
     combined: DenseAnalysis
     omitted_references: bool
     fine_grid: bool
+    # an object whose `ptr` is the C result (POINTER(MultiResult)) these arrays view, kept
+    # alive with them; None for copies
+    handle: object = dataclasses.field(default=None, repr=False, compare=False)
 
     @property
     def stats(self) -> Dict[str, object]:
@@ -373,7 +392,7 @@ def unpack_multi(m: MultiResult, owner=None) -> MultiAnalysis:
     st["full_working_set_bytes"] = int(m.full_working_set_bytes)
     st["reference_view_limit"] = (int(m.reference_view_limit) if m.has_reference_view_limit
                                   else None)
-    return MultiAnalysis(combined, bool(m.omitted_references), bool(m.fine_grid))
+    return MultiAnalysis(combined, bool(m.omitted_references), bool(m.fine_grid), owner)
 
 
 @dataclasses.dataclass
