@@ -468,8 +468,12 @@ __device__ __forceinline__ float patch_taps(const float (&rp)[(2 * R + 1) * (2 *
 // Whether a ZNCC patch cost certainly exceeds `thr`, from the sums over its first NS taps
 // (s = Σd, B = Σd², A = Σ r̂ d) and, of the unit, zero-mean reference vector r̂, the sum srS
 // and the centred energy E = Σ r̂² - srS² / NS over those taps. Whatever the unseen taps hold,
-// the patch's correlation is at most sqrt(c² + 1 - E), where c is the seen taps' centred
-// correlation (Cauchy-Schwarz over the unseen values, the patch mean included). The test
+// the patch's correlation is at most sqrt(max(c, 0)² + 1 - E), where c is the seen taps'
+// centred correlation: with N0 the seen taps' centred covariance and V their centred sum of
+// squares, the unseen values (the patch mean's shift included) form a vector w that adds at
+// most sqrt(1 - E)·|w| to the covariance and |w|² to the sum of squares, so the correlation is
+// at most (N0 + sqrt(1 - E)·|w|) / sqrt(V + |w|²): by Cauchy-Schwarz at most
+// sqrt(N0² / V + 1 - E), and below sqrt(1 - E) where N0 <= 0. The test
 // carries allowances for the FP32 rounding of the sums (3e-5 of their magnitudes, several
 // times the bound for NS <= 121 terms) and of r̂'s norm, and a 1e-4 margin on the cost, so a
 // hypothesis it drops costs more than `thr` in the kernel's own arithmetic as well.
@@ -484,7 +488,8 @@ __device__ __forceinline__ bool zncc_exceeds(float s, float B, float A, float sr
     const float V = fmaf(-s, sm, B);     // Σ (d - mean of the seen d)²
     const float Vlo = fmaf(-3.0e-5f, B, V);
     if (!(Vlo > 1.0e-6f * B) || !(Vlo > 0.f)) return false;
-    const float Nhi = fabsf(N0) + 3.0e-5f * (sqrtf(B) + fabsf(srS * sm));
+    // an upper bound on the exact N0, clamped at zero (a non-positive covariance adds nothing)
+    const float Nhi = fmaxf(N0 + 3.0e-5f * (sqrtf(B) + fabsf(srS * sm)), 0.f);
     const float rest = (1.0f + 1.0e-5f) - E;
     // correlation² <= Nhi² / Vlo + rest < lim²
     return fmaf(Nhi, Nhi, rest * Vlo) < lim * lim * Vlo;
@@ -1486,6 +1491,14 @@ __host__ __device__ constexpr int recip_bound_rows2(int R) { return R + 2 < 2 * 
 // the listed pairs only, so its waves are dense and the tail blocks of each list leave at
 // once. Each pair's result goes to its own slot: the order of the list changes which pairs
 // share a wave, never a result.
+//
+// A list is filled from both ends: pairs whose direct cost is at most kRecipGoodShare of the
+// photometric gate from the front, the others from the back. The direct cost is the cost the
+// reverse sweep is likely to find near its expected depth, so the front pairs reach a low
+// threshold early and their partial-patch bound drops most far hypotheses; a wave skips a
+// hypothesis's remaining rows only when all its lanes drop it, so keeping the poorly matched
+// pairs (whose threshold stays near the gate) in waves of their own lets the others skip.
+constexpr float kRecipGoodShare = 0.4f;
 template <int R, int COST, int ST>
 __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip_pre(Args a) {
     constexpr int P = 2 * R + 1;
@@ -1500,6 +1513,7 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip_pre(Args a) {
     const bool in = i < a.n_samples[p];
     const SourceDev &src = pd.src[s];
     bool ok = false;
+    float direct = kInfF;
     if (in) {
         const Decision &d = a.decision[pd.base + i];
         const TexRef<ST, true> src_tex(src.tex.quad, src.tex.w, src.tex.h);
@@ -1511,7 +1525,8 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip_pre(Args a) {
             double H[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) H[k] = fma(d.dr, src.A[k], src.B[k]);
-            ok = patch_cost<R, COST, ST, true>(rp, st, H, d.xr, d.yr, src_tex) <= pd.max_cost;
+            direct = patch_cost<R, COST, ST, true>(rp, st, H, d.xr, d.yr, src_tex);
+            ok = direct <= pd.max_cost;
         }
         double xs = 0.0, ys = 0.0, zs = 0.0;
         if (ok) ok = project_d(src.R, src.t, d.pos, src.tex.w, src.tex.h, pd.focal, &xs, &ys, &zs);
@@ -1527,16 +1542,25 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip_pre(Args a) {
         }
         if (!ok) a.recip[pd.recip_base + (size_t)s * pd.cap + i] = Recip{{0.0, 0.0, 0.0}, 0, -1};
     }
+    const bool good = ok && direct <= kRecipGoodShare * pd.max_cost;
     const uint64_t m = __ballot(ok);
     if (m == 0ull) return;
+    const uint64_t mg = __ballot(good), mp = m & ~mg;
     const int leader = __ffsll((unsigned long long)m) - 1;
-    uint32_t base = 0;
+    // one 64-bit atomic claims both ends: the front count in the low word, the back count in
+    // the high word (each below the list capacity, so the low word never carries)
+    uint64_t old = 0;
     if (__lane_id() == (unsigned)leader)
-        base = atomicAdd(&a.recip_count[p * kMaxRecipSources + s], (uint32_t)__popcll(m));
-    base = __shfl(base, leader, 64);
-    if (ok)
-        a.recip_list[pd.recip_base + (size_t)s * pd.cap + base +
-                     (uint32_t)__popcll(m & lanemask_lt())] = i;
+        old = atomicAdd((unsigned long long *)&a.recip_count[2u * (p * kMaxRecipSources + s)],
+                        (unsigned long long)__popcll(mg) |
+                            ((unsigned long long)__popcll(mp) << 32));
+    const uint32_t front = __shfl((uint32_t)old, leader, 64);
+    const uint32_t back = __shfl((uint32_t)(old >> 32), leader, 64);
+    uint32_t *list = a.recip_list + pd.recip_base + (size_t)s * pd.cap;
+    if (good)
+        list[front + (uint32_t)__popcll(mg & lanemask_lt())] = i;
+    else if (ok)
+        list[pd.cap - 1u - (back + (uint32_t)__popcll(mp & lanemask_lt()))] = i;
 }
 
 template <int R, int COST, int ST>
@@ -1550,7 +1574,8 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     const uint32_t local = item - pd.recip_item0, nb = pd.cap / kBlock;
     const int s = (int)(local / nb);
     const uint32_t e = (local % nb) * kBlock + threadIdx.x;
-    if (e >= a.recip_count[p * kMaxRecipSources + s]) return;
+    const uint32_t *cnt = a.recip_count + 2u * (p * kMaxRecipSources + s);
+    if (e >= cnt[0] && e < pd.cap - cnt[1]) return;  // between the two ends of the list
     const uint32_t i = a.recip_list[pd.recip_base + (size_t)s * pd.cap + e];
     const Decision &d = a.decision[pd.base + i];
     const SourceDev &src = pd.src[s];
@@ -1683,7 +1708,10 @@ __global__ __launch_bounds__(kBlock) void k_pair_totals(Args a, uint32_t *pairs)
     const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= a.n_prob) return;
     uint32_t t = 0;
-    for (int s = 0; s < a.prob[p].S; ++s) t += a.recip_count[p * kMaxRecipSources + s];
+    for (int s = 0; s < a.prob[p].S; ++s) {
+        const uint32_t *cnt = a.recip_count + 2u * (p * kMaxRecipSources + s);
+        t += cnt[0] + cnt[1];
+    }
     pairs[p] = t;
 }
 
