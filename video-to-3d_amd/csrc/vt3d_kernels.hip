@@ -837,12 +837,16 @@ __device__ __forceinline__ float lds_patch_cost(const float *__restrict__ L, int
     return finish_cost<COST, T>(rp, st, s1, s2, srs, sv);
 }
 
-// Source-side sums of the tile's ZNCC patches, shared between overlapping patches: for each
-// region row r and site column c, the sum and the sum of squares (FP64) of the P values that
-// the patch of column c reads in row r. A patch then adds P of these per sum instead of
-// reading its P * P taps twice more. In FP64 the sums of the shifted FP32 values are exact up
-// to the squares' last bits, so the variance is at least as accurate as a per-patch FP32 sum
-// around the patch's own reference mean. NaN (an invalid tap) carries through.
+// Box sums of the tile's ZNCC patches, shared between overlapping patches. For each region
+// row r and site column c, the FP64 sums of the P values v that the patch of column c reads in
+// row r, of their squares, and of their products with the reference values r under them; a
+// patch adds P of each instead of reading its P * P taps. The ZNCC of a patch then follows
+// from its sums: with a = Σv, b = Σv², c = Σrv and the patch's own Σr and Σ(r - r̄)²,
+//   covariance Σ(r - r̄)(v - v̄) = c - Σr · a / T,  variance Σ(v - v̄)² = b - a² / T.
+// The values are FP32 and the reference values integers below 256, so every product and
+// square is exact in FP64 and the sums hold them to the last bits of the largest, which keeps
+// the covariance and variance at least as accurate as a per-patch FP32 sum around the patch's
+// own mean. NaN (an invalid tap) carries through.
 struct BoxWalk {
     int off0;  // this thread's first entry: its first tap in the region (row-major, pitch RW)
     int step;  // region words between consecutive entries of one thread
@@ -856,58 +860,82 @@ __device__ __forceinline__ BoxWalk box_walk(int RW, int TX, int stride) {
     return BoxWalk{r * RW + c * stride, (kBlock / TX) * RW};
 }
 
+// The row sums of one step: (Σv, Σv²) into H and Σrv into C, from the warped values L and the
+// region's reference values Rv (both row-major, pitch RW).
 template <int R>
-__device__ __forceinline__ void box_rows(const float *__restrict__ L, int n, const BoxWalk &bw,
-                                         double2 *__restrict__ H) {
+__device__ __forceinline__ void box_rows(const float *__restrict__ L,
+                                         const double *__restrict__ Rv, int n, const BoxWalk &bw,
+                                         double2 *__restrict__ H, double *__restrict__ C) {
     constexpr int P = 2 * R + 1;
     int off = bw.off0;
     for (int e = (int)threadIdx.x; e < n; e += kBlock, off += bw.step) {
         const float *row = L + off;
+        const double *ref = Rv + off;
         double a = (double)row[0];
         double b = a * a;
+        double c = ref[0] * a;
 #pragma unroll
         for (int dx = 1; dx < P; ++dx) {
             const double v = (double)row[dx];
             a += v;
             b = fma(v, v, b);
+            c = fma(ref[dx], v, c);
         }
         H[e] = make_double2(a, b);
+        C[e] = c;
     }
 }
 
-// ZNCC cost of one patch: its source sums from the box sums (`Hp`: the entry of the patch's
-// first row and column, rows TX entries apart) and its own taps for the cross term. The LDS
-// values are the source values less a shift that is the same for every tap of the patch,
-// which leaves the mean-free sums unchanged.
+// A lane's reference patch as the box cost needs it: Σr, and 1 / sqrt(Σ(r - r̄)²) unless the
+// patch is flat (mean squared deviation at most kZnccMinVariance, whose cost is 1).
+struct BoxRef {
+    double sum;
+    float inv_sd;
+    bool flat;
+};
+
+template <int T>
+__device__ __forceinline__ BoxRef box_ref(const float (&rp)[T]) {
+    // integer values below 256: both sums are exact in FP32 (below 2^24 for T <= 225)
+    float s = 0.f, q = 0.f;
+#pragma unroll
+    for (int k = 0; k < T; ++k) {
+        s += rp[k];
+        q = fmaf(rp[k], rp[k], q);
+    }
+    const double sd = (double)s;
+    const double var = fma(-sd, sd * (1.0 / (double)T), (double)q);
+    BoxRef b;
+    b.sum = sd;
+    b.flat = !(var > (double)(kZnccMinVariance * (float)T));
+    b.inv_sd = b.flat ? 0.f : rsqrtf((float)var);
+    return b;
+}
+
+// ZNCC cost of one patch from the box sums (`Hp`, `Cp`: the entries of the patch's first row
+// and column, rows TX entries apart).
 template <int R>
-__device__ __forceinline__ float lds_zncc_cost(const float *__restrict__ L, int pitch,
-                                               const double2 *__restrict__ Hp, int TX,
-                                               const float (&rp)[(2 * R + 1) * (2 * R + 1)],
-                                               const RefStats &st) {
+__device__ __forceinline__ float lds_zncc_box(const double2 *__restrict__ Hp,
+                                              const double *__restrict__ Cp, int TX,
+                                              const BoxRef &ref) {
     constexpr int P = 2 * R + 1;
     constexpr int T = P * P;
     // entries are 16-byte aligned (one 128-bit LDS read each)
     const double2 *H = static_cast<const double2 *>(__builtin_assume_aligned(Hp, 16));
-    double s1 = H[0].x, s2 = H[0].y;
+    double s1 = H[0].x, s2 = H[0].y, sc = Cp[0];
 #pragma unroll
     for (int dy = 1; dy < P; ++dy) {
         const double2 h = H[dy * TX];
         s1 += h.x;
         s2 += h.y;
-    }
-    float srs = 0.f;
-#pragma unroll
-    for (int dy = 0; dy < P; ++dy) {
-        const float *row = L + dy * pitch;
-#pragma unroll
-        for (int dx = 0; dx < P; ++dx) srs = fmaf(rp[dy * P + dx], row[dx], srs);
+        sc += Cp[dy * TX];
     }
     if (__builtin_isnan(s1)) return kInfF;
     const double m = s1 * (1.0 / (double)T);
     const double var = fma(-s1, m, s2);
-    if (st.flat || !(var > (double)(kZnccMinVariance * (float)T))) return 1.f;
-    const float cov = fmaf(-(float)m, st.sum_hat, srs);
-    return 1.f - cov * rsqrtf((float)var);
+    if (ref.flat || !(var > (double)(kZnccMinVariance * (float)T))) return 1.f;
+    const double cov = fma(-ref.sum, m, sc);
+    return 1.f - ((float)cov * ref.inv_sd) * rsqrtf((float)var);
 }
 
 // ---- XCD-aware block order -----------------------------------------------------------------
@@ -1254,45 +1282,47 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
                 rp[(dy + R) * P + (dx + R)] =
                     active ? (float)l[(size_t)(y + dy) * w + (x + dx)] : 0.f;
     }
-    const RefStats st = ref_stats<COST, T>(rp);
+    // With box sums a lane needs only its patch's Σr and Σ(r - r̄)² (the patch values come
+    // from the region's reference values in LDS), so its P * P registers are free.
+    BoxRef bref{};
+    RefStats st{};
+    if constexpr (BOX)
+        bref = box_ref<T>(rp);
+    else
+        st = ref_stats<COST, T>(rp);
     // LDS: two buffers of warped values and the pixel coordinates, each `npad` words, each
-    // wave's table of region warps, then (with box sums) the row sums, RH x TX; tile_lds_bytes
-    // on the host is the same layout.
+    // wave's table of region warps, then (with box sums) the row sums (Σv, Σv²) and Σrv,
+    // RH x TX each, and the region's reference values (FP64); tile_lds_bytes on the host is
+    // the same layout.
     const int npad = region_pad(npx, R);
     half2_t *uv = reinterpret_cast<half2_t *>(lds + 2 * npad);
     constexpr int kTable = warp_table_of(SMAX);
     RegionWarp *table = reinterpret_cast<RegionWarp *>(lds + 3 * npad) + (threadIdx.x >> 6) * kTable;
     double2 *rows = reinterpret_cast<double2 *>(lds + 3 * npad +
                                                 (kBlock / 64) * kTable * (int)(sizeof(RegionWarp) / 4));
+    double *rows_c = reinterpret_cast<double *>(rows + RH * TX);
+    double *refv = rows_c + RH * TX;
     const int lane = (int)__lane_id();
-    // With box sums the values in LDS are the source values less the mean of the workgroup's
-    // active reference means, so the cross term sums values near zero.
-    __shared__ float2 wg_mean[kBlock / 64];
-    if constexpr (BOX) {
-        float sm = active ? st.mean : 0.f, cnt = active ? 1.f : 0.f;
-#pragma unroll
-        for (int off = 32; off >= 1; off >>= 1) {
-            sm += __shfl_xor(sm, off, 64);
-            cnt += __shfl_xor(cnt, off, 64);
-        }
-        if (lane == 0) wg_mean[threadIdx.x >> 6] = make_float2(sm, cnt);
-    }
-    region_coords(uv, RW, npx, npad);
-    __syncthreads();
-    const BoxWalk bw = box_walk(RW, TX, stride);
-    float shift = 0.f;
-    if constexpr (BOX) {
-        float sm = 0.f, cnt = 0.f;
-#pragma unroll
-        for (int w = 0; w < kBlock / 64; ++w) {
-            sm += wg_mean[w].x;
-            cnt += wg_mean[w].y;
-        }
-        shift = cnt > 0.f ? sm / cnt : 0.f;
-    }
     // Region origin (reference pixels) and this thread's walk over the region.
     const int X0 = (int)pd.border + bx * TX * stride - R;
     const int Y0 = (int)pd.border + by * TY * stride - R;
+    region_coords(uv, RW, npx, npad);
+    if constexpr (BOX) {
+        // the reference values under the region (pixels outside the frame, which only the
+        // patches of sites beyond the grid reach, read 0)
+        const uint8_t *l = pd.ref.luma;
+        const int w = (int)pd.ref.w, h = (int)pd.ref.h;
+        for (int q = (int)threadIdx.x; q < npx; q += kBlock) {
+            const int y = q / RW, x = q - y * RW;
+            const int gxp = X0 + x, gyp = Y0 + y;
+            refv[q] = gxp >= 0 && gyp >= 0 && gxp < w && gyp < h
+                          ? (double)l[(size_t)gyp * (size_t)w + (size_t)gxp]
+                          : 0.0;
+        }
+    }
+    __syncthreads();
+    const BoxWalk bw = box_walk(RW, TX, stride);
+    const float shift = 0.f;
     const int py0 = (int)threadIdx.x / RW, px0 = (int)threadIdx.x - py0 * RW;
     const int step_y = kBlock / RW, step_x = kBlock - step_y * RW;
     const int S = pd.S, D = pd.D;
@@ -1361,11 +1391,11 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
                     if constexpr (BOX) {
                         // the row sums of this step; the rows of the step before were read
                         // before the barrier above
-                        box_rows<R>(L, RH * TX, bw, rows);
+                        box_rows<R>(L, refv, RH * TX, bw, rows, rows_c);
                         __syncthreads();
                         if (active)
-                            c = lds_zncc_cost<R>(mine + buf * npad, RW,
-                                                 rows + (ly * stride) * TX + lx, TX, rp, st);
+                            c = lds_zncc_box<R>(rows + (ly * stride) * TX + lx,
+                                                rows_c + (ly * stride) * TX + lx, TX, bref);
                     } else {
                         // (at even strides every patch starts on an even column)
                         if (active) {
@@ -2302,7 +2332,7 @@ size_t tile_lds_bytes(int tile_w, int stride, int radius, int smax, bool box) {
     const int npad = region_pad(region_width(tile_w, stride, P) * RH, radius);
     return (2ull * 4ull + 4ull) * (size_t)npad +
            (size_t)(kBlock / 64) * (size_t)warp_table_of(smax) * sizeof(RegionWarp) +
-           (box ? (size_t)RH * (size_t)tile_w * 16ull : 0ull);
+           (box ? (size_t)RH * (size_t)tile_w * 24ull + 8ull * (size_t)npad : 0ull);
 }
 
 void launch_gate(const Args &a, hipStream_t s) {
