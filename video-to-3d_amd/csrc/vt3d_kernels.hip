@@ -585,11 +585,13 @@ __device__ __forceinline__ void load_h(const double *__restrict__ H, double (&h)
 // smaller for large patches, whose reference values already fill most registers.
 __host__ __device__ constexpr int warp_batch_of(int R) { return R >= 5 ? 4 : 6; }
 template <int R> constexpr int warp_batch() { return warp_batch_of(R); }
-// Region width in pixels: the tile's patches span (TX - 1) * stride + P columns, rounded up
-// to even so that every row starts 8-byte aligned (at even strides each patch's taps then
-// pair up into 64-bit LDS reads; the extra column is warped and never read).
-__host__ __device__ constexpr int region_width(int TX, int stride, int P) {
-    return ((TX - 1) * stride + P + 1) & ~1;
+// Region width in pixels: the tile's patches span (TX - 1) * stride + P columns. Per-tap
+// sums round it up to even so that every row starts 8-byte aligned (at even strides each
+// patch's taps then pair up into 64-bit LDS reads); box sums round it up to odd, so that the
+// row-sum pass, whose half-waves read two region rows at the same columns, puts the second
+// row on the other half of the LDS banks. The extra column is warped and never read.
+__host__ __device__ constexpr int region_width(int TX, int stride, int P, bool box) {
+    return box ? (((TX - 1) * stride + P) | 1) : (((TX - 1) * stride + P + 1) & ~1);
 }
 // The region's pixel count rounded up to whole warp batches of the workgroup.
 __host__ __device__ constexpr int region_pad(int npx, int R) {
@@ -861,25 +863,26 @@ __device__ __forceinline__ BoxWalk box_walk(int RW, int TX, int stride) {
 }
 
 // The row sums of one step: (Σv, Σv²) into H and Σrv into C, from the warped values L and the
-// region's reference values Rv (both row-major, pitch RW).
+// region's reference values Rv (both row-major, pitch RW; the reference values are integers,
+// exact in FP32).
 template <int R>
 __device__ __forceinline__ void box_rows(const float *__restrict__ L,
-                                         const double *__restrict__ Rv, int n, const BoxWalk &bw,
+                                         const float *__restrict__ Rv, int n, const BoxWalk &bw,
                                          double2 *__restrict__ H, double *__restrict__ C) {
     constexpr int P = 2 * R + 1;
     int off = bw.off0;
     for (int e = (int)threadIdx.x; e < n; e += kBlock, off += bw.step) {
         const float *row = L + off;
-        const double *ref = Rv + off;
+        const float *ref = Rv + off;
         double a = (double)row[0];
         double b = a * a;
-        double c = ref[0] * a;
+        double c = (double)ref[0] * a;
 #pragma unroll
         for (int dx = 1; dx < P; ++dx) {
             const double v = (double)row[dx];
             a += v;
             b = fma(v, v, b);
-            c = fma(ref[dx], v, c);
+            c = fma((double)ref[dx], v, c);
         }
         H[e] = make_double2(a, b);
         C[e] = c;
@@ -1262,7 +1265,7 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
     const int hb = (int)(local % (uint32_t)pd.n_hb);
     const int TX = pd.tile_w, TY = kBlock / pd.tile_w;
     const int stride = (int)pd.stride;
-    const int RW = region_width(TX, stride, P), RH = (TY - 1) * stride + P;
+    const int RW = region_width(TX, stride, P, BOX), RH = (TY - 1) * stride + P;
     const int npx = RW * RH;
     const int bx = tile % pd.tiles_x, by = tile / pd.tiles_x;
     const int lx = (int)threadIdx.x % TX, ly = (int)threadIdx.x / TX;
@@ -1292,8 +1295,8 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
         st = ref_stats<COST, T>(rp);
     // LDS: two buffers of warped values and the pixel coordinates, each `npad` words, each
     // wave's table of region warps, then (with box sums) the row sums (Σv, Σv²) and Σrv,
-    // RH x TX each, and the region's reference values (FP64); tile_lds_bytes on the host is
-    // the same layout.
+    // RH x TX each, and the region's reference values; tile_lds_bytes on the host is the same
+    // layout.
     const int npad = region_pad(npx, R);
     half2_t *uv = reinterpret_cast<half2_t *>(lds + 2 * npad);
     constexpr int kTable = warp_table_of(SMAX);
@@ -1301,7 +1304,7 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
     double2 *rows = reinterpret_cast<double2 *>(lds + 3 * npad +
                                                 (kBlock / 64) * kTable * (int)(sizeof(RegionWarp) / 4));
     double *rows_c = reinterpret_cast<double *>(rows + RH * TX);
-    double *refv = rows_c + RH * TX;
+    float *refv = reinterpret_cast<float *>(rows_c + RH * TX);
     const int lane = (int)__lane_id();
     // Region origin (reference pixels) and this thread's walk over the region.
     const int X0 = (int)pd.border + bx * TX * stride - R;
@@ -1316,8 +1319,8 @@ __global__ __launch_bounds__(kBlock) VT3D_TILE_WAVES void k_sweep_tile(Args a) {
             const int y = q / RW, x = q - y * RW;
             const int gxp = X0 + x, gyp = Y0 + y;
             refv[q] = gxp >= 0 && gyp >= 0 && gxp < w && gyp < h
-                          ? (double)l[(size_t)gyp * (size_t)w + (size_t)gxp]
-                          : 0.0;
+                          ? (float)l[(size_t)gyp * (size_t)w + (size_t)gxp]
+                          : 0.f;
         }
     }
     __syncthreads();
@@ -2329,10 +2332,10 @@ size_t tile_lds_bytes(int tile_w, int stride, int radius, int smax, bool box) {
     const int P = 2 * radius + 1;
     const int TY = kBlock / tile_w;
     const int RH = (TY - 1) * stride + P;
-    const int npad = region_pad(region_width(tile_w, stride, P) * RH, radius);
+    const int npad = region_pad(region_width(tile_w, stride, P, box) * RH, radius);
     return (2ull * 4ull + 4ull) * (size_t)npad +
            (size_t)(kBlock / 64) * (size_t)warp_table_of(smax) * sizeof(RegionWarp) +
-           (box ? (size_t)RH * (size_t)tile_w * 24ull + 8ull * (size_t)npad : 0ull);
+           (box ? (size_t)RH * (size_t)tile_w * 24ull + 4ull * (size_t)npad : 0ull);
 }
 
 void launch_gate(const Args &a, hipStream_t s) {
