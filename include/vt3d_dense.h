@@ -688,7 +688,8 @@ typedef struct vt3d_server vt3d_server;
 
 typedef struct vt3d_server_config {
     uint32_t struct_size;          /* sizeof(vt3d_server_config) */
-    int32_t max_batch_requests;    /* requests per launch (0: 64) */
+    int32_t max_batch_requests;    /* most requests per launch (0: 64); a launch takes the
+                                      queue's share over two launches per in-flight slot */
     int32_t max_inflight_batches;  /* batches on the device at once, 1..8 (0: 2) */
     int32_t pad;
     uint64_t hbm_budget_bytes;     /* frames + every in-flight batch's buffers (0: 64 GiB) */

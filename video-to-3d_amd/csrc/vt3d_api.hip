@@ -2397,6 +2397,12 @@ void admit(vt3d_server *sv, Slot &sl, std::vector<std::unique_ptr<ServerReq>> ta
     sv->stats.workspace_bytes = std::max<uint64_t>(sv->stats.workspace_bytes, ws_bytes);
 }
 
+// Launches per in-flight slot over which the server spreads its backlog (serve()).
+// Measured on the served stream (Poisson 200/s, 2 slots): at most 64 requests per launch give
+// p99 20.8 ms, at most 4 give 15.4 ms and at most 2 give 12.8 ms, and fixed small launches
+// cost 8-15% of the saturated rate, which the backlog share keeps.
+constexpr size_t kBacklogLaunchesPerSlot = 2;
+
 void serve(vt3d_server *sv) {
     (void)hipSetDevice(sv->session->device);
     std::unique_lock<std::mutex> lk(sv->mu);
@@ -2435,8 +2441,15 @@ void serve(vt3d_server *sv) {
             continue;
         }
         if (free_slot && !sv->pending.empty()) {
+            // A launch takes its share of the backlog spread over kBacklogLaunchesPerSlot
+            // launches per slot, up to max_batch: a burst then runs as several short
+            // launches, so no request waits for the whole burst's device time, while a long
+            // backlog (the throughput-bound case) still fills launches to max_batch.
+            const size_t spread = kBacklogLaunchesPerSlot * sv->slots.size();
+            const int want = (int)std::min<size_t>(
+                (size_t)sv->max_batch, (sv->pending.size() + spread - 1) / spread);
             std::vector<std::unique_ptr<ServerReq>> taken;
-            while (!sv->pending.empty() && (int)taken.size() < sv->max_batch) {
+            while (!sv->pending.empty() && (int)taken.size() < want) {
                 taken.push_back(std::move(sv->pending.front()));
                 sv->pending.pop_front();
             }
