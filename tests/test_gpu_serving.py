@@ -71,6 +71,26 @@ def test_served_results_match_alone(product, c4):
     assert out["latency_ms"]["p99"] >= out["latency_ms"]["p50"] > 0
 
 
+def test_launches_share_the_backlog(product, c4):
+    # A launch takes its share of the queue over two launches per in-flight slot (at most
+    # max_batch): 40 requests at once on 2 slots run as launches of at most 10, not one of 40.
+    sess, requests, arrivals, alone = c4
+    server = product.Server(sess, max_batch=64, max_inflight=2)
+    try:
+        out = serving.saturate(server, requests, keep_results=True)
+        st = server.stats()
+    finally:
+        server.close()
+    n = len(requests)
+    assert st["completed"] == n
+    assert st["max_batch_requests_seen"] <= (n + 3) // 4
+    assert st["batches"] >= 4
+    done = out["completions"]
+    for i in range(0, n, 7):
+        assert done[i]["status"] == abi.VT3D_OK
+        _same(done[i]["result"], alone[i], f"request {i}")
+
+
 def test_budget_admission(product, c4):
     """A slot budget that holds only the smaller requests: those are served (and match),
     the rest complete with VT3D_E_BUDGET."""
