@@ -1041,7 +1041,7 @@ int upload_plan(vt3d_batch &b) {
         (rc = ws.decision.ensure(sizeof(Decision) * pl.slots)) ||
         (rc = ws.recip.ensure(sizeof(Recip) * pl.recip_slots)) ||
         (rc = ws.recip_list.ensure(4ull * pl.recip_slots)) ||
-        (rc = ws.recip_count.ensure(8ull * kMaxRecipSources * n)) ||
+        (rc = ws.recip_count.ensure(4ull * kMaxRecipSources * n)) ||
         (rc = ws.out.ensure(sizeof(SampleOut) * pl.slots)) ||
         (rc = ws.block_counts.ensure(4ull * kCounters * pl.n_blocks)) ||
         (rc = ws.block_offsets.ensure(4ull * kCounters * pl.n_blocks)) ||
@@ -1174,7 +1174,7 @@ Args batch_args(const vt3d_batch &b) {
 // The reciprocal check of every problem: list lengths cleared (a fill kernel, so a captured
 // graph holds no memset), the gates and pair lists of every group, then the reverse sweeps.
 void enqueue_recip(const Plan &pl, const Args &a, hipStream_t st, hipEvent_t before_sweep) {
-    launch_fill_u32(a.recip_count, 2u * a.n_prob * (uint32_t)kMaxRecipSources, st);
+    launch_fill_u32(a.recip_count, a.n_prob * (uint32_t)kMaxRecipSources, st);
     for (int kind : {3, 1}) {
         if (kind == 1 && before_sweep) (void)hipEventRecord(before_sweep, st);
         for (const Group &g : pl.recip_groups) {

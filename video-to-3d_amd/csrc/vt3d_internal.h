@@ -180,9 +180,7 @@ struct Args {
     Decision *decision;
     Recip *recip;
     uint32_t *recip_list;           // per Recip slot: the pairs that passed the gates, listed
-    uint32_t *recip_count;          // [n_prob][kMaxRecipSources][2] list lengths: pairs
-                                    // listed from the front, then from the back (8-byte
-                                    // aligned pairs, claimed with one 64-bit atomic)
+    uint32_t *recip_count;          // [n_prob][kMaxRecipSources] list lengths
     SampleOut *out;
     // compaction scans
     uint32_t *block_counts;         // [n_blocks][kCounters]

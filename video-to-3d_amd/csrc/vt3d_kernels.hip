@@ -400,10 +400,14 @@ template <int T, int COST> struct TapSums {
 // Rows [Y0, Y1) of a valid patch's taps (dy = Y0 - R .. Y1 - 1 - R), each warped relative to
 // the centre's cell, gathered and added to the sums. Splitting a patch into row ranges keeps
 // every addition in the order of the whole patch's loop.
-template <int R, int COST, int ST, bool BUF, int Y0, int Y1>
+// With REMOTE, the reference values are those of the lane at byte address `src` (the
+// lane index times 4): each is read from that lane's copy of rp[k] by a cross-lane permute,
+// which every lane of the wave must execute together.
+template <int R, int COST, int ST, bool BUF, int Y0, int Y1, bool REMOTE = false>
 __device__ __forceinline__ void tap_rows(const float (&rp)[(2 * R + 1) * (2 * R + 1)],
                                          float pivot, const Warp &w, const TexRef<ST, BUF> &tex,
-                                         TapSums<(2 * R + 1) * (2 * R + 1), COST> &ts) {
+                                         TapSums<(2 * R + 1) * (2 * R + 1), COST> &ts,
+                                         int src = 0) {
     constexpr int P = 2 * R + 1;
     const uint32_t sw = tex.w;
     const uint32_t base = (uint32_t)(w.iyc * (int)sw + w.ixc);
@@ -444,10 +448,13 @@ __device__ __forceinline__ void tap_rows(const float (&rp)[(2 * R + 1) * (2 * R 
                 ts.sv[k] = v;
                 s1 += v;
             } else {
+                float r = rp[k];
+                if constexpr (REMOTE)
+                    r = __int_as_float(__builtin_amdgcn_ds_bpermute(src, __float_as_int(r)));
                 const float d = v - pivot;
                 s1 += d;
                 s2 = fmaf(d, d, s2);
-                srs = fmaf(rp[k], d, srs);
+                srs = fmaf(r, d, srs);
             }
         }
     }
@@ -1503,10 +1510,17 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
 
 // ---- k_recip_pre / k_recip -----------------------------------------------------------------
 
-// Rows of a P x P patch (P = 2R + 1) summed before the reverse sweep's first and second test
-// of its partial-patch bound (1 <= first < second < P).
-__host__ __device__ constexpr int recip_bound_rows1(int R) { return R; }
-__host__ __device__ constexpr int recip_bound_rows2(int R) { return R + 2 < 2 * R + 1 ? R + 2 : 2 * R; }
+// Rows of a P x P patch (P = 2R + 1) summed before the reverse sweep tests its partial-patch
+// bound (1 <= rows < P). With the continuation queue an early test pays: on the 120-keyframe
+// batch (7x7) 2 rows beat 3 and 1 by 2% and 11%, on the 1920x1080 solve (11x11) 3 rows beat
+// 4, 5 and 2 by 3%, 9% and 2%.
+__host__ __device__ constexpr int recip_bound_rows(int R) { return (R + 1) / 2; }
+
+// The reverse sweep holds a pair's P * P reference values in registers (the continuation
+// queue reads them from there) beside a row of gathers: at 7x7 that needs about 160 registers,
+// so three waves per SIMD instead of four, which measured 5% faster than four waves with the
+// excess spilled to scratch.
+template <int R> constexpr int recip_waves() { return R >= 5 ? 2 : 3; }
 
 // reciprocal_depth_observations (dense_legacy.rs:795-856) in two launches over the same work
 // items, a work item being (problem, source, block of 256 samples), so every wave shares the
@@ -1523,15 +1537,9 @@ __host__ __device__ constexpr int recip_bound_rows2(int R) { return R + 2 < 2 * 
 // k_recip runs the reverse sweep (estimate_single_view_depth, dense_legacy.rs:887-949) for
 // the listed pairs only, so its waves are dense and the tail blocks of each list leave at
 // once. Each pair's result goes to its own slot: the order of the list changes which pairs
-// share a wave, never a result.
-//
-// A list is filled from both ends: pairs whose direct cost is at most kRecipGoodShare of the
-// photometric gate from the front, the others from the back. The direct cost is the cost the
-// reverse sweep is likely to find near its expected depth, so the front pairs reach a low
-// threshold early and their partial-patch bound drops most far hypotheses; a wave skips a
-// hypothesis's remaining rows only when all its lanes drop it, so keeping the poorly matched
-// pairs (whose threshold stays near the gate) in waves of their own lets the others skip.
-constexpr float kRecipGoodShare = 0.4f;
+// share a wave, never a result. (With the reverse sweep's continuation queue, which packs the
+// doubtful hypotheses of a wave's pairs, grouping the pairs by direct cost no longer pays:
+// lists filled from both ends by direct cost measured 0.5% slower on the 120-keyframe batch.)
 template <int R, int COST, int ST>
 __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip_pre(Args a) {
     constexpr int P = 2 * R + 1;
@@ -1575,29 +1583,59 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip_pre(Args a) {
         }
         if (!ok) a.recip[pd.recip_base + (size_t)s * pd.cap + i] = Recip{{0.0, 0.0, 0.0}, 0, -1};
     }
-    const bool good = ok && direct <= kRecipGoodShare * pd.max_cost;
     const uint64_t m = __ballot(ok);
     if (m == 0ull) return;
-    const uint64_t mg = __ballot(good), mp = m & ~mg;
     const int leader = __ffsll((unsigned long long)m) - 1;
-    // one 64-bit atomic claims both ends: the front count in the low word, the back count in
-    // the high word (each below the list capacity, so the low word never carries)
-    uint64_t old = 0;
+    uint32_t first = 0;
     if (__lane_id() == (unsigned)leader)
-        old = atomicAdd((unsigned long long *)&a.recip_count[2u * (p * kMaxRecipSources + s)],
-                        (unsigned long long)__popcll(mg) |
-                            ((unsigned long long)__popcll(mp) << 32));
-    const uint32_t front = __shfl((uint32_t)old, leader, 64);
-    const uint32_t back = __shfl((uint32_t)(old >> 32), leader, 64);
-    uint32_t *list = a.recip_list + pd.recip_base + (size_t)s * pd.cap;
-    if (good)
-        list[front + (uint32_t)__popcll(mg & lanemask_lt())] = i;
-    else if (ok)
-        list[pd.cap - 1u - (back + (uint32_t)__popcll(mp & lanemask_lt()))] = i;
+        first = atomicAdd(&a.recip_count[p * kMaxRecipSources + s], (uint32_t)__popcll(m));
+    first = __shfl(first, leader, 64);
+    if (ok)
+        a.recip_list[pd.recip_base + (size_t)s * pd.cap + first +
+                     (uint32_t)__popcll(m & lanemask_lt())] = i;
+}
+
+// The bounded reverse sweep's continuation queue. Every lane takes its own pair's next
+// hypothesis through the warp and the first K1 rows of taps; a hypothesis the bound leaves in
+// doubt becomes an entry here (its warp and partial sums), and once the wave holds 64 entries
+// its lanes finish them together, one entry each, whichever pair it belongs to, reading the
+// owning pair's reference values from the owner's registers. A wave then pays for the
+// remaining rows once per 64 doubtful hypotheses instead of once per step in which any lane
+// has one. Entries are 64 bytes, read and written as four 16-byte words.
+struct alignas(16) RecipEntry {
+    int ixc, iyc;
+    float fxc, fyc, g, e, ax, bx, ay, by;
+    float s1, s2, srs;
+    uint32_t tag;  // hypothesis in the low 16 bits, the owning lane above
+    uint32_t pad[2];
+};
+// Entries per wave: a drain leaves at most 63, and one step adds at most 64.
+constexpr int kRecipQueue = 128;
+
+// Order-preserving key of (cost, hypothesis): an unsigned comparison of keys orders costs
+// ascending (negative costs included) and then hypotheses ascending, the order in which the
+// in-order sweep keeps its first minimum. kNoKey: no cost yet.
+constexpr uint64_t kNoKey = ~0ull;
+__device__ __forceinline__ uint64_t cost_key(float c, int hyp) {
+    const uint32_t b = __float_as_uint(c);
+    const uint32_t k = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+    return ((uint64_t)k << 32) | (uint32_t)hyp;
+}
+__device__ __forceinline__ float key_cost(uint64_t key) {
+    const uint32_t k = (uint32_t)(key >> 32);
+    return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// Orders a wave's LDS writes before its later LDS reads by other lanes (a wave's LDS accesses
+// complete in order; the fences keep the compiler from moving them across).
+__device__ __forceinline__ void wave_sync_lds() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 template <int R, int COST, int ST>
-__global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(recip_waves<R>()))) void k_recip(Args a) {
     constexpr int P = 2 * R + 1;
     constexpr int T = P * P;
     const uint32_t item = xcd_chunk(blockIdx.x, gridDim.x);
@@ -1607,10 +1645,12 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     const uint32_t local = item - pd.recip_item0, nb = pd.cap / kBlock;
     const int s = (int)(local / nb);
     const uint32_t e = (local % nb) * kBlock + threadIdx.x;
-    const uint32_t *cnt = a.recip_count + 2u * (p * kMaxRecipSources + s);
-    if (e >= cnt[0] && e < pd.cap - cnt[1]) return;  // between the two ends of the list
-    const uint32_t i = a.recip_list[pd.recip_base + (size_t)s * pd.cap + e];
-    const Decision &d = a.decision[pd.base + i];
+    const bool listed = e < a.recip_count[p * kMaxRecipSources + s];
+    // With ZNCC and the bound on, the wave works as one (its continuation queue); otherwise
+    // each lane sweeps its own pair alone.
+    const bool bounded = COST == VT3D_COST_ZNCC && pd.recip_bound != 0;  // uniform
+    if (bounded ? __ballot(listed) == 0ull : !listed) return;
+    const uint32_t i = listed ? a.recip_list[pd.recip_base + (size_t)s * pd.cap + e] : 0u;
     const SourceDev &src = pd.src[s];
     const TexRef<ST, true> ref_tex(pd.ref.quad, pd.ref.w, pd.ref.h);
     const int D = pd.D;
@@ -1618,39 +1658,24 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
     // the pair passed k_recip_pre's gates: the projection and the source patch repeat them
     // with the same arithmetic, so they succeed again
     double xs = 0.0, ys = 0.0, zs = 0.0;
-    project_d(src.R, src.t, d.pos, src.tex.w, src.tex.h, pd.focal, &xs, &ys, &zs);
     float rp[T];
-    sample_patch<R>(src.tex.luma, src.tex.w, src.tex.h, xs, ys, rp);
-    const RefStats st = ref_stats<COST, T>(rp);
-    Recip out{{0.0, 0.0, 0.0}, 0, -1};
+    RefStats st{0.f, 0.f, true};
+    if (listed) {
+        const Decision &d = a.decision[pd.base + i];
+        project_d(src.R, src.t, d.pos, src.tex.w, src.tex.h, pd.focal, &xs, &ys, &zs);
+        sample_patch<R>(src.tex.luma, src.tex.w, src.tex.h, xs, ys, rp);
+        st = ref_stats<COST, T>(rp);
+    } else {
+#pragma unroll
+        for (int k = 0; k < T; ++k) rp[k] = 0.f;
+    }
     float best_c = kInfF;
     int best_h = -1;
     const double *Hs = pd.Hr + (size_t)s * D * kHrStride;
     const float midx = 0.5f * (float)(pd.ref.w - 1), midy = 0.5f * (float)(pd.ref.h - 1);
-    // The sweep keeps the first minimum in hypothesis order, so a hypothesis that certainly
-    // costs more than a cost some hypothesis reaches cannot be it. With ZNCC, a bound from the
-    // patch's first rows of taps (zncc_exceeds, tested after K1 and after K2 rows) drops such
-    // hypotheses before their remaining rows; the threshold is the photometric gate, then the
-    // best cost found. Hypotheses that are not dropped are summed exactly as without the
-    // bound, so the result is the same.
-    // rows of taps before the first and the second test of the bound
-    constexpr int K1 = recip_bound_rows1(R), K2 = recip_bound_rows2(R);
-    constexpr int N1 = K1 * P, N2 = K2 * P;
-    float srS1 = 0.f, E1 = 0.f, srS2 = 0.f, E2 = 0.f;
-    const bool bounded = COST == VT3D_COST_ZNCC && pd.recip_bound != 0;  // uniform
-    if (bounded) {
-        float q = 0.f;
-#pragma unroll
-        for (int k = 0; k < N2; ++k) {
-            if (k == N1) E1 = fmaf(-srS1, srS1 * (1.0f / (float)N1), q);
-            srS2 += rp[k];
-            q = fmaf(rp[k], rp[k], q);
-            if (k < N1) srS1 = srS2;
-        }
-        E2 = fmaf(-srS2, srS2 * (1.0f / (float)N2), q);
-    }
-    // The cost of hypothesis `hyp`, or +inf where the bound shows it exceeds `thr`.
-    auto cost_of = [&](int hyp, float thr) -> float {
+    // The warp of hypothesis `hyp` and whether its patch lies inside the reference frame with
+    // every tap's depth above 1e-4.
+    auto warp_of = [&](int hyp, bool *valid) -> Warp {
         const double *rec = Hs + (size_t)hyp * kHrStride;
         double h[9];
         load_h(rec, h);
@@ -1661,58 +1686,128 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
         bool certain;
         const Warp w = make_warp_rev<R>(h, hf, xs, ys, pd.ref.w, pd.ref.h, midx, midy, &certain);
         // the exact corner test only where the bound leaves the patch in doubt (near an edge)
-        bool valid = certain;
-        if (!certain && w.ok) valid = corners_inside<R>(w, pd.ref.w, pd.ref.h);
-        float c = kInfF;
-        if (valid) {
-            if constexpr (COST == VT3D_COST_ZNCC) {
-                TapSums<T, COST> ts;
-                tap_rows<R, COST, ST, true, 0, K1>(rp, st.mean, w, ref_tex, ts);
-                if (!zncc_exceeds<N1>(ts.s1, ts.s2, ts.srs, srS1, E1, thr)) {
-                    tap_rows<R, COST, ST, true, K1, K2>(rp, st.mean, w, ref_tex, ts);
-                    if (!zncc_exceeds<N2>(ts.s1, ts.s2, ts.srs, srS2, E2, thr)) {
-                        tap_rows<R, COST, ST, true, K2, P>(rp, st.mean, w, ref_tex, ts);
-                        c = finish_cost<COST, T>(rp, st, ts.s1, ts.s2, ts.srs, ts.sv);
+        bool ok = certain;
+        if (!certain && w.ok) ok = corners_inside<R>(w, pd.ref.w, pd.ref.h);
+        *valid = ok;
+        return w;
+    };
+    if constexpr (COST == VT3D_COST_ZNCC) {
+        if (bounded) {
+            // The sweep keeps the first minimum in hypothesis order, so a hypothesis that
+            // certainly costs more than a cost some hypothesis reaches cannot be it. A bound
+            // from the patch's first K1 rows of taps (zncc_exceeds) drops such hypotheses
+            // before their remaining rows; the threshold is the photometric gate, then the
+            // best cost the pair's finished hypotheses reach. Hypotheses that are not dropped
+            // are summed exactly as without the bound, so the result is the same.
+            constexpr int K1 = recip_bound_rows(R);
+            constexpr int N1 = K1 * P;
+            float srS1 = 0.f, q = 0.f;
+#pragma unroll
+            for (int k = 0; k < N1; ++k) {
+                srS1 += rp[k];
+                q = fmaf(rp[k], rp[k], q);
+            }
+            const float E1 = fmaf(-srS1, srS1 * (1.0f / (float)N1), q);
+            __shared__ RecipEntry queues[kBlock / 64][kRecipQueue];
+            __shared__ uint64_t bests[kBlock];
+            const int lane = (int)__lane_id();
+            RecipEntry *queue = queues[threadIdx.x >> 6];
+            uint64_t *best = bests + (threadIdx.x & ~63u);
+            best[lane] = kNoKey;
+            wave_sync_lds();
+            // Each lane walks its hypotheses outward from the one nearest the expected source
+            // depth zs, so the lanes of a wave meet their likely minima first and the far
+            // hypotheses, which the bound drops, together.
+            int h0 = 0;
+            if (listed) {
+                const double inv_min = 1.0 / src.min_depth, inv_max = 1.0 / src.max_depth;
+                if (D > 1 && zs > 0.0 && isfinite(zs) && inv_min != inv_max) {
+                    const double t = (inv_min - 1.0 / zs) / (inv_min - inv_max) * (double)(D - 1);
+                    h0 = (int)fmin(fmax(rint(t), 0.0), (double)(D - 1));
+                }
+            }
+            const int m = min(h0, D - 1 - h0);
+            const bool right = h0 < D - 1 - h0;
+            uint32_t head = 0, tail = 0;  // the queue's extent (uniform across the wave)
+            for (int j = 0; j < D; ++j) {
+                bool doubt = false;
+                RecipEntry ent;
+                if (listed) {
+                    int hyp;
+                    if (j <= 2 * m) {
+                        const int k = (j + 1) >> 1;
+                        hyp = (j & 1) ? h0 + k : h0 - k;
+                    } else {
+                        hyp = right ? h0 + j - m : h0 - (j - m);
+                    }
+                    bool valid;
+                    const Warp w = warp_of(hyp, &valid);
+                    if (valid) {
+                        const uint64_t bk = best[lane];
+                        const float thr = st.flat ? kInfF : bk == kNoKey ? maxc : key_cost(bk);
+                        TapSums<T, COST> ts;
+                        tap_rows<R, COST, ST, true, 0, K1>(rp, st.mean, w, ref_tex, ts);
+                        doubt = !zncc_exceeds<N1>(ts.s1, ts.s2, ts.srs, srS1, E1, thr);
+                        ent = RecipEntry{w.ixc, w.iyc, w.fxc, w.fyc, w.g, w.e, w.ax, w.bx, w.ay,
+                                         w.by, ts.s1, ts.s2, ts.srs,
+                                         (uint32_t)hyp | ((uint32_t)lane << 16), {0u, 0u}};
                     }
                 }
-            } else {
-                c = patch_taps<R, COST, ST, true>(rp, st, w, ref_tex);
+                const uint64_t dm = __ballot(doubt);
+                if (doubt)
+                    queue[(tail + (uint32_t)__popcll(dm & lanemask_lt())) & (kRecipQueue - 1)] =
+                        ent;
+                tail += (uint32_t)__popcll(dm);
+                wave_sync_lds();
+                // finish the queued hypotheses 64 at a time (all of them after the last step)
+                while (tail - head >= 64u || (j == D - 1 && tail != head)) {
+                    const uint32_t n = min(tail - head, 64u);
+                    // lanes past the last entry repeat the first one and keep nothing
+                    const RecipEntry q =
+                        queue[(head + ((uint32_t)lane < n ? (uint32_t)lane : 0u)) &
+                              (kRecipQueue - 1)];
+                    const int owner = (int)(q.tag >> 16), hyp = (int)(q.tag & 0xffffu);
+                    const int src_lane = owner * 4;
+                    Warp w;
+                    w.ixc = q.ixc;
+                    w.iyc = q.iyc;
+                    w.fxc = q.fxc;
+                    w.fyc = q.fyc;
+                    w.g = q.g;
+                    w.e = q.e;
+                    w.ax = q.ax;
+                    w.bx = q.bx;
+                    w.ay = q.ay;
+                    w.by = q.by;
+                    RefStats ost;
+                    ost.mean = __int_as_float(
+                        __builtin_amdgcn_ds_bpermute(src_lane, __float_as_int(st.mean)));
+                    ost.sum_hat = __int_as_float(
+                        __builtin_amdgcn_ds_bpermute(src_lane, __float_as_int(st.sum_hat)));
+                    ost.flat = __builtin_amdgcn_ds_bpermute(src_lane, st.flat ? 1 : 0) != 0;
+                    TapSums<T, COST> ts;
+                    ts.s1 = q.s1;
+                    ts.s2 = q.s2;
+                    ts.srs = q.srs;
+                    tap_rows<R, COST, ST, true, K1, P, true>(rp, ost.mean, w, ref_tex, ts,
+                                                             src_lane);
+                    const float c = finish_cost<COST, T>(rp, ost, ts.s1, ts.s2, ts.srs, ts.sv);
+                    if ((uint32_t)lane < n && c <= maxc)
+                        atomicMin((unsigned long long *)&best[owner],
+                                  (unsigned long long)cost_key(c, hyp));
+                    head += n;
+                    wave_sync_lds();
+                }
             }
+            const uint64_t bk = best[lane];
+            if (bk != kNoKey) best_h = (int)(uint32_t)bk;
         }
-        return c;
-    };
-    if (bounded) {
-        // Each lane walks its hypotheses outward from the one nearest the expected source
-        // depth zs, so the lanes of a wave meet their likely minima first and the far
-        // hypotheses, which the bound drops, together. The minimum is kept by (cost,
-        // hypothesis), the order the in-order sweep's first minimum has.
-        int h0 = 0;
-        const double inv_min = 1.0 / src.min_depth, inv_max = 1.0 / src.max_depth;
-        if (D > 1 && zs > 0.0 && isfinite(zs) && inv_min != inv_max) {
-            const double t = (inv_min - 1.0 / zs) / (inv_min - inv_max) * (double)(D - 1);
-            h0 = (int)fmin(fmax(rint(t), 0.0), (double)(D - 1));
-        }
-        const int m = min(h0, D - 1 - h0);
-        const bool right = h0 < D - 1 - h0;
-        float thr = st.flat ? kInfF : maxc;
-        for (int j = 0; j < D; ++j) {
-            int hyp;
-            if (j <= 2 * m) {
-                const int k = (j + 1) >> 1;
-                hyp = (j & 1) ? h0 + k : h0 - k;
-            } else {
-                hyp = right ? h0 + j - m : h0 - (j - m);
-            }
-            const float c = cost_of(hyp, thr);
-            if (c <= maxc && (c < best_c || (c == best_c && hyp < best_h))) {
-                best_c = c;
-                best_h = hyp;
-                if (!st.flat) thr = fminf(thr, c);
-            }
-        }
-    } else {
+    }
+    if (!bounded) {
         for (int hyp = 0; hyp < D; ++hyp) {
-            const float c = cost_of(hyp, kInfF);
+            bool valid;
+            const Warp w = warp_of(hyp, &valid);
+            const float c = valid ? patch_taps<R, COST, ST, true>(rp, st, w, ref_tex) : kInfF;
             // cost ascending, then depth (hypothesis) ascending: strict < keeps the first
             if (c <= maxc && c < best_c) {
                 best_c = c;
@@ -1720,6 +1815,8 @@ __global__ __launch_bounds__(kBlock) VT3D_WAVES void k_recip(Args a) {
             }
         }
     }
+    if (!listed) return;
+    Recip out{{0.0, 0.0, 0.0}, 0, -1};
     bool ok = best_h >= 0;
     out.rh = best_h;
     if (ok) {
@@ -1742,8 +1839,7 @@ __global__ __launch_bounds__(kBlock) void k_pair_totals(Args a, uint32_t *pairs)
     if (p >= a.n_prob) return;
     uint32_t t = 0;
     for (int s = 0; s < a.prob[p].S; ++s) {
-        const uint32_t *cnt = a.recip_count + 2u * (p * kMaxRecipSources + s);
-        t += cnt[0] + cnt[1];
+        t += a.recip_count[p * kMaxRecipSources + s];
     }
     pairs[p] = t;
 }
