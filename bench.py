@@ -9,10 +9,10 @@ Four legs, all with frames resident in HBM; units = texture-gated samples x hypo
   96 inverse-depth hypotheses over 0.5-8 m, 7x7 ZNCC, stride 2, FP16 texel storage with FP32
   accumulation, reciprocal check and fusion. All 120 solves form one batch; the launch shape
   is autotuned. A step plans a new batch of the 120 (host selection, layout, FP64 tables,
-  upload), runs it and downloads and unpacks all 120 results; two batches are in flight, the
-  next one planned while they run. The same batch planned once and replayed from a captured
-  HIP graph (one at a time, and two in flight), and planned afresh but run one at a time, are
-  reported beside it.
+  upload), runs it and downloads and unpacks all 120 results; three batches are in flight,
+  each submitted as soon as it is planned. The same batch planned once and replayed from a
+  captured HIP graph (one at a time, and three in flight), and planned afresh but run one at a
+  time, are reported beside it.
 * plain: 1 reference + 8 sources at 640x360 from a seeded 6-plane box room, 128 hypotheses,
   7x7 ZNCC, stride 2, FP32 storage; a step is one solve (selection, table upload, kernel
   pipeline, download).
@@ -199,39 +199,37 @@ def plain_leg(V, scenes, steps: int, warmup: int, storage: str):
     }
 
 
-def _pipelined(sess, reqs, steps: int, use_graph: bool):
+def _pipelined(sess, reqs, steps: int, use_graph: bool, depth: int = 3):
     """`steps` batches of `reqs`, each planned afresh (host selection, layout, FP64 tables,
-    upload) and run, two in flight: batch k + 1 is submitted and batch k + 2 planned before
-    batch k is collected, so the host's planning and result building overlap device work and
-    the two launches fill each other's tails. Returns (seconds per step, batch latencies in ms
-    from planning to collected results, the last batch's results)."""
+    upload) and submitted as soon as it is planned, `depth` in flight: batch k + depth - 1 is
+    planned and submitted before batch k is collected, so the host's planning and result
+    building overlap device work. With two in flight the two batches share the device, end
+    together, and the device idles while the host builds the older one's results before it
+    can submit the next; a third keeps it fed. Returns (seconds per step, batch latencies in
+    ms from planning to collected results, the last batch's results)."""
+    from collections import deque
     t0 = time.perf_counter()
-    born = [time.perf_counter()]
-    cur = sess.batch(reqs, use_graph=use_graph)
-    cur.submit()
-    nxt = None
-    if steps > 1:
-        born.append(time.perf_counter())
-        nxt = sess.batch(reqs, use_graph=use_graph)
+    live = deque()  # (planning start, batch), oldest first, all submitted
     lat, res = [], None
+    planned = 0
     for k in range(steps):
-        if nxt is not None:
-            nxt.submit()
-        after = None
-        if k + 2 < steps:
-            born.append(time.perf_counter())
-            after = sess.batch(reqs, use_graph=use_graph)
+        while planned < steps and len(live) < depth:
+            born = time.perf_counter()
+            b = sess.batch(reqs, use_graph=use_graph)
+            b.submit()
+            live.append((born, b))
+            planned += 1
+        born, cur = live.popleft()
         res = cur.collect()
-        lat.append(1e3 * (time.perf_counter() - born[k]))
+        lat.append(1e3 * (time.perf_counter() - born))
         cur.close()
-        cur, nxt = nxt, after
     return (time.perf_counter() - t0) / steps, lat, res
 
 
 def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log):
     """All `count` keyframe solves of the batched clip as one batch per step. The headline
-    plans every step's batch afresh and keeps two in flight; the same batch replayed from a
-    captured graph (planned once), serially and two in flight, and planned afresh but run
+    plans every step's batch afresh and keeps three in flight; the same batch replayed from a
+    captured graph (planned once), serially and three in flight, and planned afresh but run
     one at a time (as the plain leg's solve is) are reported beside it."""
     from vt3d_dense import autotune
 
@@ -251,10 +249,10 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
     if tune:
         tuned = autotune.tune(sess, reqs, use_graph=True, log=log)
         reqs = tuned["requests"]
-    # The headline: planned afresh every step, two batches in flight.
-    _pipelined(sess, reqs, max(2, warmup + 1), False)
+    # The headline: planned afresh every step, three batches in flight.
+    _pipelined(sess, reqs, max(3, warmup + 1), False)
     step_s, lat, res = _pipelined(sess, reqs, steps, False)
-    # One batch planned once and replayed from its graph: serially, then two in flight.
+    # One batch planned once and replayed from its graph: serially, then three in flight.
     batch = sess.batch(reqs, use_graph=True)
     for _ in range(max(1, warmup)):
         batch.run()
@@ -269,15 +267,16 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
         units += tm.units
     batch.close()
     per_step_units = units // max(1, steps)
-    pipe = [sess.batch(reqs, use_graph=True) for _ in range(2)]
+    pipe = [sess.batch(reqs, use_graph=True) for _ in range(3)]
     for pb in pipe:
         pb.run()
     t1 = time.perf_counter()
-    pipe[0].submit()
+    for k in range(min(2, steps)):
+        pipe[k].submit()
     for k in range(steps):
-        if k + 1 < steps:
-            pipe[(k + 1) % 2].submit()
-        pipe[k % 2].collect()
+        if k + 2 < steps:
+            pipe[(k + 2) % 3].submit()
+        pipe[k % 3].collect()
     replay_pipelined_s = (time.perf_counter() - t1) / steps
     for pb in pipe:
         pb.close()
@@ -320,8 +319,8 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
         "device_value": units / (1e-3 * sum(dev_ms)),
         "ms_per_step": 1e3 * step_s,
         "what": "each step plans a new batch of the keyframes (host selection, layout, FP64 "
-                "tables, upload) and runs it eagerly; two batches in flight, the next one "
-                "planned while they run",
+                "tables, upload) and runs it eagerly; three batches in flight, each "
+                "submitted as soon as it is planned",
         "latency_ms": {"p50": _pct(lat, 0.5), "p99": _pct(lat, 0.99),
                        "what": "one batch, planning to collected results"},
         "device_ms": {"graph_total": statistics.mean(dev_ms),
@@ -337,8 +336,8 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
                            "step, one at a time"},
         "replay_pipelined": {"value": per_step_units / replay_pipelined_s,
                              "ms_per_step": 1e3 * replay_pipelined_s,
-                             "what": "two replayed batches in flight: step k + 1 submitted "
-                                     "before step k is collected"},
+                             "what": "three replayed batches in flight: steps k + 1 and "
+                                     "k + 2 submitted before step k is collected"},
         "planning_included": {"value": per_step_units / statistics.mean(planned),
                               "ms_per_step": 1e3 * statistics.mean(planned),
                               "what": "planned afresh and run one at a time: build, run, "
@@ -610,7 +609,7 @@ def main() -> int:
         "config": {"workload": f"batched: {bat['solves']} keyframes (640x360 / 360x640 / "
                                "480x270 mix) each with its 8 nearest frames, 96 hyps 0.5-8 m, "
                                "7x7 ZNCC, stride 2, FP16 storage, reciprocal + fusion; each "
-                               "step plans a new batch and runs it, two batches in flight",
+                               "step plans a new batch and runs it, three batches in flight",
                    "solves": bat["solves"], "sources": 8,
                    "mean_sources_kept": round(mean_sources, 3), "hypotheses": D, "patch": P,
                    "stride": 2, "storage": "f16",
