@@ -44,6 +44,8 @@ import time
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "video-to-3d_amd"))
 
+from vt3d_dense.serving import frozen_heap  # noqa: E402
+
 PEAK_FP32_TFLOPS = 157.3          # MI355X vector FP32, packed (v_pk_fma_f32)
 PEAK_FP32_NON_PACKED_TFLOPS = 78.6  # one wave64 FMA per SIMD per 4 cycles: this build's ceiling
 PEAK_HBM_GBS = 8000.0
@@ -169,17 +171,18 @@ def plain_leg(V, scenes, steps: int, warmup: int, storage: str):
         solve()
     walls, units, sweep_ms, recip_ms, rev_ms, dev_ms = [], [], [], [], [], []
     pairs = 0
-    for _ in range(steps):
-        t0 = time.perf_counter()
-        res = solve()
-        walls.append(time.perf_counter() - t0)
-        tm = sess.timing()
-        units.append(tm.units)
-        sweep_ms.append(tm.sweep_ms)
-        recip_ms.append(tm.reciprocal_ms)
-        rev_ms.append(tm.reverse_sweep_ms)
-        dev_ms.append(tm.total_ms)
-        pairs = int(tm.reciprocal_pairs)
+    with frozen_heap():
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            res = solve()
+            walls.append(time.perf_counter() - t0)
+            tm = sess.timing()
+            units.append(tm.units)
+            sweep_ms.append(tm.sweep_ms)
+            recip_ms.append(tm.reciprocal_ms)
+            rev_ms.append(tm.reverse_sweep_ms)
+            dev_ms.append(tm.total_ms)
+            pairs = int(tm.reciprocal_pairs)
     sess.close()
     walls_ms = [w * 1e3 for w in walls]
     return {
@@ -208,22 +211,23 @@ def _pipelined(sess, reqs, steps: int, use_graph: bool, depth: int = 3):
     can submit the next; a third keeps it fed. Returns (seconds per step, batch latencies in
     ms from planning to collected results, the last batch's results)."""
     from collections import deque
-    t0 = time.perf_counter()
-    live = deque()  # (planning start, batch), oldest first, all submitted
-    lat, res = [], None
-    planned = 0
-    for k in range(steps):
-        while planned < steps and len(live) < depth:
-            born = time.perf_counter()
-            b = sess.batch(reqs, use_graph=use_graph)
-            b.submit()
-            live.append((born, b))
-            planned += 1
-        born, cur = live.popleft()
-        res = cur.collect()
-        lat.append(1e3 * (time.perf_counter() - born))
-        cur.close()
-    return (time.perf_counter() - t0) / steps, lat, res
+    with frozen_heap():
+        t0 = time.perf_counter()
+        live = deque()  # (planning start, batch), oldest first, all submitted
+        lat, res = [], None
+        planned = 0
+        for k in range(steps):
+            while planned < steps and len(live) < depth:
+                born = time.perf_counter()
+                b = sess.batch(reqs, use_graph=use_graph)
+                b.submit()
+                live.append((born, b))
+                planned += 1
+            born, cur = live.popleft()
+            res = cur.collect()
+            lat.append(1e3 * (time.perf_counter() - born))
+            cur.close()
+        return (time.perf_counter() - t0) / steps, lat, res
 
 
 def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log):
@@ -508,7 +512,7 @@ def main() -> int:
     ap.add_argument("--serving-requests", type=int, default=2000,
                     help="measured requests of the serving stream (0: skip the serving leg)")
     ap.add_argument("--serving-warmup", type=int, default=200)
-    ap.add_argument("--serving-inflight", type=int, default=2,
+    ap.add_argument("--serving-inflight", type=int, default=3,
                     help="batches the server keeps in flight (1-8)")
     ap.add_argument("--precision-rows", type=int, default=2,
                     help="grid rows of the large solve checked against the FP64 oracle "

@@ -7,11 +7,28 @@ sample units per second.
 """
 from __future__ import annotations
 
+import contextlib
+import gc
 import threading
 import time
 from typing import Dict, List, Sequence
 
 import numpy as np
+
+
+@contextlib.contextmanager
+def frozen_heap():
+    """Moves every object alive at the start out of the collector's reach while a timed run
+    goes on: a full collection over a large heap (a caller's earlier results) holds the
+    interpreter lock for tens of milliseconds, which would hold back a submitting thread and
+    turn an arrival schedule into bursts, or stall the host side of a pipeline. Objects made
+    during the run are still collected."""
+    gc.collect()
+    gc.freeze()
+    try:
+        yield
+    finally:
+        gc.unfreeze()
 
 
 def _percentiles(v: Sequence[float]) -> Dict[str, float]:
@@ -53,12 +70,13 @@ def open_loop(server, requests: Sequence[dict], arrivals: Sequence[float], warmu
         except BaseException as e:  # surfaced after the join
             errors.append(e)
 
-    th = threading.Thread(target=feed, daemon=True)
-    th.start()
-    try:
-        _collect(server, n, done, timeout_s)
-    finally:
-        th.join()
+    with frozen_heap():
+        th = threading.Thread(target=feed, daemon=True)
+        th.start()
+        try:
+            _collect(server, n, done, timeout_s)
+        finally:
+            th.join()
     if errors:
         raise errors[0]
     return _summary(done, submit_t, warmup, keep_results)
@@ -70,10 +88,11 @@ def saturate(server, requests: Sequence[dict], timeout_s: float = 300.0,
     n = len(requests)
     done: Dict[int, dict] = {}
     submit_t = [0.0] * n
-    for i, rq in enumerate(requests):
-        submit_t[i] = time.perf_counter()
-        server.submit(rq, i)
-    _collect(server, n, done, timeout_s)
+    with frozen_heap():
+        for i, rq in enumerate(requests):
+            submit_t[i] = time.perf_counter()
+            server.submit(rq, i)
+        _collect(server, n, done, timeout_s)
     return _summary(done, submit_t, 0, keep_results)
 
 
