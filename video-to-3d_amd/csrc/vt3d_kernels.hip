@@ -1701,13 +1701,13 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(recip_wa
             // are summed exactly as without the bound, so the result is the same.
             constexpr int K1 = recip_bound_rows(R);
             constexpr int N1 = K1 * P;
-            float srS1 = 0.f, q = 0.f;
+            float srS1 = 0.f, sq1 = 0.f;
 #pragma unroll
             for (int k = 0; k < N1; ++k) {
                 srS1 += rp[k];
-                q = fmaf(rp[k], rp[k], q);
+                sq1 = fmaf(rp[k], rp[k], sq1);
             }
-            const float E1 = fmaf(-srS1, srS1 * (1.0f / (float)N1), q);
+            const float E1 = fmaf(-srS1, srS1 * (1.0f / (float)N1), sq1);
             __shared__ RecipEntry queues[kBlock / 64][kRecipQueue];
             __shared__ uint64_t bests[kBlock];
             const int lane = (int)__lane_id();
