@@ -1511,10 +1511,11 @@ __global__ __launch_bounds__(kBlock) void k_decide(Args a) {
 // ---- k_recip_pre / k_recip -----------------------------------------------------------------
 
 // Rows of a P x P patch (P = 2R + 1) summed before the reverse sweep tests its partial-patch
-// bound (1 <= rows < P). With the continuation queue an early test pays: on the 120-keyframe
-// batch (7x7) 2 rows beat 3 and 1 by 2% and 11%, on the 1920x1080 solve (11x11) 3 rows beat
-// 4, 5 and 2 by 3%, 9% and 2%.
-__host__ __device__ constexpr int recip_bound_rows(int R) { return (R + 1) / 2; }
+// bound (1 <= rows < P). With the continuation queue an early test pays. Measured: at 5x5 2
+// rows beat 1 by 13%; on the 120-keyframe batch (7x7) 2 rows beat 3 and 1 by 2% and 11%; at
+// 9x9 3 rows beat 2 by 2%; on the 1920x1080 solve (11x11) 3 rows beat 4, 5 and 2 by 3%, 9%
+// and 2%.
+__host__ __device__ constexpr int recip_bound_rows(int R) { return R / 2 + 1; }
 
 // The reverse sweep holds a pair's P * P reference values in registers (the continuation
 // queue reads them from there) beside a row of gathers: at 7x7 that needs about 160 registers,
