@@ -213,12 +213,12 @@ __global__ __launch_bounds__(kBlock) void k_build_view(const uint8_t *__restrict
 
 // Occupancy caps (waves per SIMD; a cap of n allows 512 / n VGPRs per lane), by patch radius.
 // A lane holds its reference patch (P * P values) and one row of gathers in registers. Up
-// to 7 x 7 the direct sweep and the reciprocal kernel run best at 4 waves, where the compiler
-// keeps fewer gathers in flight per wave and the SIMD holds more waves, and the tiled sweep
-// at 3, where its warp batch fits beside the patch; larger patches take fewer waves rather
-// than spill.
+// to 7 x 7 the direct sweep and k_recip_pre run best at 4 waves, where the compiler keeps
+// fewer gathers in flight per wave and the SIMD holds more waves; the tiled sweep at 4 up to
+// 5 x 5 (7% faster than 3 there) and at 3 at 7 x 7, where its warp batch fits beside the
+// patch; larger patches take fewer waves rather than spill.
 template <int R> constexpr int direct_waves() { return R >= 5 ? 2 : R >= 4 ? 3 : 4; }
-template <int R> constexpr int tile_waves() { return R >= 4 ? 2 : 3; }
+template <int R> constexpr int tile_waves() { return R >= 4 ? 2 : R >= 3 ? 3 : 4; }
 #define VT3D_WAVES __attribute__((amdgpu_waves_per_eu(direct_waves<R>())))
 #define VT3D_TILE_WAVES __attribute__((amdgpu_waves_per_eu(tile_waves<R>())))
 
