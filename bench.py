@@ -41,6 +41,16 @@ import statistics
 import sys
 import time
 
+# Each batch in flight and each serving slot runs on a stream of its own. The HIP runtime
+# maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless set), and two
+# in-flight batches whose streams share a queue run one after the other: a trace of the
+# three-in-flight leg showed two of its streams on one queue and the device idle 20 ms every
+# second step. Eight queues cover the session's own stream, three batches and the slots. The
+# runtime reads the variable at its first call, which comes after this.
+_QUEUES_NEEDED = 8
+if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < _QUEUES_NEEDED:
+    os.environ["GPU_MAX_HW_QUEUES"] = str(_QUEUES_NEEDED)
+
 REPO = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(REPO, "video-to-3d_amd"))
 
