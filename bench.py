@@ -9,9 +9,9 @@ Four legs, all with frames resident in HBM; units = texture-gated samples x hypo
   96 inverse-depth hypotheses over 0.5-8 m, 7x7 ZNCC, stride 2, FP16 texel storage with FP32
   accumulation, reciprocal check and fusion. All 120 solves form one batch; the launch shape
   is autotuned. A step plans a new batch of the 120 (host selection, layout, FP64 tables,
-  upload), runs it and downloads and unpacks all 120 results; three batches are in flight,
+  upload), runs it and downloads and unpacks all 120 results; four batches are in flight,
   each submitted as soon as it is planned. The same batch planned once and replayed from a
-  captured HIP graph (one at a time, and three in flight), and planned afresh but run one at a
+  captured HIP graph (one at a time, and four in flight), and planned afresh but run one at a
   time, are reported beside it.
 * plain: 1 reference + 8 sources at 640x360 from a seeded 6-plane box room, 128 hypotheses,
   7x7 ZNCC, stride 2, FP32 storage; a step is one solve (selection, table upload, kernel
@@ -45,7 +45,7 @@ import time
 # maps a process's streams onto GPU_MAX_HW_QUEUES hardware queues (4 unless set), and two
 # in-flight batches whose streams share a queue run one after the other: a trace of the
 # three-in-flight leg showed two of its streams on one queue and the device idle 20 ms every
-# second step. Eight queues cover the session's own stream, three batches and the slots. The
+# second step. Eight queues cover the session's own stream, four batches and the slots. The
 # runtime reads the variable at its first call, which comes after this.
 _QUEUES_NEEDED = 8
 if int(os.environ.get("GPU_MAX_HW_QUEUES", "0") or 0) < _QUEUES_NEEDED:
@@ -212,14 +212,20 @@ def plain_leg(V, scenes, steps: int, warmup: int, storage: str):
     }
 
 
-def _pipelined(sess, reqs, steps: int, use_graph: bool, depth: int = 3):
+# Batches in flight in the batched legs. With two, the pair shared the device, ended together
+# and left it idle while the host built the older one's results; three or more batches that
+# start together also end together, and the host then replaces them one by one while the
+# device runs out of work: 109.8-110.2 ms per step with three, 106.8-108.7 with four, 106.2
+# with five.
+IN_FLIGHT = 4
+
+
+def _pipelined(sess, reqs, steps: int, use_graph: bool, depth: int = IN_FLIGHT):
     """`steps` batches of `reqs`, each planned afresh (host selection, layout, FP64 tables,
     upload) and submitted as soon as it is planned, `depth` in flight: batch k + depth - 1 is
     planned and submitted before batch k is collected, so the host's planning and result
-    building overlap device work. With two in flight the two batches share the device, end
-    together, and the device idles while the host builds the older one's results before it
-    can submit the next; a third keeps it fed. Returns (seconds per step, batch latencies in
-    ms from planning to collected results, the last batch's results)."""
+    building overlap device work (see IN_FLIGHT). Returns (seconds per step, batch latencies
+    in ms from planning to collected results, the last batch's results)."""
     from collections import deque
     with frozen_heap():
         t0 = time.perf_counter()
@@ -242,8 +248,8 @@ def _pipelined(sess, reqs, steps: int, use_graph: bool, depth: int = 3):
 
 def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log):
     """All `count` keyframe solves of the batched clip as one batch per step. The headline
-    plans every step's batch afresh and keeps three in flight; the same batch replayed from a
-    captured graph (planned once), serially and three in flight, and planned afresh but run
+    plans every step's batch afresh and keeps four in flight; the same batch replayed from a
+    captured graph (planned once), serially and four in flight, and planned afresh but run
     one at a time (as the plain leg's solve is) are reported beside it."""
     from vt3d_dense import autotune
 
@@ -263,10 +269,10 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
     if tune:
         tuned = autotune.tune(sess, reqs, use_graph=True, log=log)
         reqs = tuned["requests"]
-    # The headline: planned afresh every step, three batches in flight.
-    _pipelined(sess, reqs, max(3, warmup + 1), False)
+    # The headline: planned afresh every step, IN_FLIGHT batches in flight.
+    _pipelined(sess, reqs, max(IN_FLIGHT, warmup + 1), False)
     step_s, lat, res = _pipelined(sess, reqs, steps, False)
-    # One batch planned once and replayed from its graph: serially, then three in flight.
+    # One batch planned once and replayed from its graph: serially, then IN_FLIGHT in flight.
     batch = sess.batch(reqs, use_graph=True)
     for _ in range(max(1, warmup)):
         batch.run()
@@ -281,16 +287,16 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
         units += tm.units
     batch.close()
     per_step_units = units // max(1, steps)
-    pipe = [sess.batch(reqs, use_graph=True) for _ in range(3)]
+    pipe = [sess.batch(reqs, use_graph=True) for _ in range(IN_FLIGHT)]
     for pb in pipe:
         pb.run()
     t1 = time.perf_counter()
-    for k in range(min(2, steps)):
+    for k in range(min(IN_FLIGHT - 1, steps)):
         pipe[k].submit()
     for k in range(steps):
-        if k + 2 < steps:
-            pipe[(k + 2) % 3].submit()
-        pipe[k % 3].collect()
+        if k + IN_FLIGHT - 1 < steps:
+            pipe[(k + IN_FLIGHT - 1) % IN_FLIGHT].submit()
+        pipe[k % IN_FLIGHT].collect()
     replay_pipelined_s = (time.perf_counter() - t1) / steps
     for pb in pipe:
         pb.close()
@@ -333,7 +339,7 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
         "device_value": units / (1e-3 * sum(dev_ms)),
         "ms_per_step": 1e3 * step_s,
         "what": "each step plans a new batch of the keyframes (host selection, layout, FP64 "
-                "tables, upload) and runs it eagerly; three batches in flight, each "
+                f"tables, upload) and runs it eagerly; {IN_FLIGHT} batches in flight, each "
                 "submitted as soon as it is planned",
         "latency_ms": {"p50": _pct(lat, 0.5), "p99": _pct(lat, 0.99),
                        "what": "one batch, planning to collected results"},
@@ -350,8 +356,9 @@ def batched_leg(V, scenes, steps: int, warmup: int, count: int, tune: bool, log)
                            "step, one at a time"},
         "replay_pipelined": {"value": per_step_units / replay_pipelined_s,
                              "ms_per_step": 1e3 * replay_pipelined_s,
-                             "what": "three replayed batches in flight: steps k + 1 and "
-                                     "k + 2 submitted before step k is collected"},
+                             "what": f"{IN_FLIGHT} replayed batches in flight: steps k + 1 "
+                                     f"to k + {IN_FLIGHT - 1} submitted before step k is "
+                                     "collected"},
         "planning_included": {"value": per_step_units / statistics.mean(planned),
                               "ms_per_step": 1e3 * statistics.mean(planned),
                               "what": "planned afresh and run one at a time: build, run, "
@@ -623,7 +630,8 @@ def main() -> int:
         "config": {"workload": f"batched: {bat['solves']} keyframes (640x360 / 360x640 / "
                                "480x270 mix) each with its 8 nearest frames, 96 hyps 0.5-8 m, "
                                "7x7 ZNCC, stride 2, FP16 storage, reciprocal + fusion; each "
-                               "step plans a new batch and runs it, three batches in flight",
+                               f"step plans a new batch and runs it, {IN_FLIGHT} batches in "
+                               "flight",
                    "solves": bat["solves"], "sources": 8,
                    "mean_sources_kept": round(mean_sources, 3), "hypotheses": D, "patch": P,
                    "stride": 2, "storage": "f16",
